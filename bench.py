@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Throughput bench of the batched Glynn permanent on one MI355X.
+
+Workload (one "step"): the plain permanent over
+  * 65536 complex128 16x16 matrices, each 16 random rows (with repetition) of
+    the first 16 columns of a Haar-random 64x64 unitary (QR of a complex
+    Gaussian, phases fixed), numpy default_rng(1); and
+  * 1024 matrices at n = 24 drawn the same way from the same unitary,
+in the fp64 tier (two kernel launches: n=16 and n=24). Inputs are resident in
+HBM before timing starts. The dd tier is timed on the same inputs and reported
+beside it. Prints ONE JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--dd-steps K2] [--cpu-seconds S]
+
+The value is evals/s of the whole job (65536 + 1024 evaluations per step).
+The roofline object describes the dominant kernel (n = 24): canonical flops
+2^(n-1) (8n - 4) per evaluation (complex MAC = 8 real flops) over its average
+launch duration measured with HIP events on the launch stream, against the
+MI355X FP64 vector peak of 78.6 TFLOP/s. cpu_baseline times the CPU oracle
+(a restatement of the reference's cpu_ref Glynn walk) on the host cores on a
+bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "evals/s perm/haf/lhaf n=16,24 fp64+dd; served p50/p99 latency; rel err vs mpmath"
+PEAK_FP64 = 78.6e12
+PEAK_HBM = 8.0e12
+
+
+def haar_unitary(m: int, rng: np.random.Generator) -> np.ndarray:
+    z = (rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))) / np.sqrt(2.0)
+    q, r = np.linalg.qr(z)
+    ph = np.diagonal(r).copy()
+    ph /= np.abs(ph)
+    return q * ph
+
+
+def workload():
+    rng = np.random.default_rng(1)
+    U = haar_unitary(64, rng)
+    rows16 = rng.integers(0, 64, (65536, 16))
+    m16 = np.ascontiguousarray(U[rows16][:, :, :16])
+    rows24 = rng.integers(0, 64, (1024, 24))
+    m24 = np.ascontiguousarray(U[rows24][:, :, :24])
+    return m16, m24
+
+
+def perm_flops(n: int) -> float:
+    return float(2 ** (n - 1) * (8 * n - 4))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--dd-steps", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              "(instances are independent; see DESIGN.md for how the path shards)", file=sys.stderr)
+        return 2
+
+    import torch
+    from gbskernels_amd import _lib
+
+    L = _lib.lib()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    sp = stream.cuda_stream
+
+    m16, m24 = workload()
+    cases = [(16, m16), (24, m24)]
+    dev_in, dev_out, dev_ws = {}, {}, {}
+    for n, m in cases:
+        dev_in[n] = torch.from_numpy(m.view(np.float64)).to(dev)
+        dev_out[n] = torch.empty(2 * m.shape[0], dtype=torch.float64, device=dev)
+        for tier in (0, 1):
+            wsb = int(L.gbs_workspace_bytes(0, tier, n, m.shape[0]))
+            dev_ws[(n, tier)] = torch.empty(max(wsb, 256), dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+
+    def launch(n, tier):
+        ws = dev_ws[(n, tier)]
+        rc = L.gbs_eval_device(0, tier, dev_in[n].data_ptr(), n, dev_in[n].numel() // (2 * n * n),
+                               dev_out[n].data_ptr(), None, ws.data_ptr(), ws.numel(), sp)
+        if rc:
+            raise RuntimeError(f"gbs_eval_device rc={rc}")
+
+    def run_steps(tier, steps, warmup):
+        for _ in range(warmup):
+            for n, _m in cases:
+                launch(n, tier)
+        torch.cuda.synchronize()
+        evs = {n: [] for n, _ in cases}
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for n, _m in cases:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                launch(n, tier)
+                e1.record(stream)
+                evs[n].append((e0, e1))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kern_ms = {n: float(np.mean([a.elapsed_time(b) for a, b in evs[n]])) for n in evs}
+        return wall, kern_ms
+
+    print(f"bench: fp64 warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
+    wall, kern_ms = run_steps(0, args.steps, args.warmup)
+    evals_per_step = sum(m.shape[0] for _, m in cases)
+    value = evals_per_step * args.steps / wall
+    out16 = dev_out[16].cpu().numpy().view(np.complex128).copy()
+    out24 = dev_out[24].cpu().numpy().view(np.complex128).copy()
+
+    print(f"bench: dd steps={args.dd_steps}", file=sys.stderr, flush=True)
+    dd_wall, dd_kern_ms = run_steps(1, args.dd_steps, 1)
+    dd16 = dev_out[16].cpu().numpy().view(np.complex128).copy()
+    dd24 = dev_out[24].cpu().numpy().view(np.complex128).copy()
+
+    # parity gate on a sample (oracle = CPU restatement of cpu_ref's Glynn walk)
+    from oracle import oracle as O
+    s16 = np.arange(0, 65536, 4099)[:16]
+    ref16 = O.perm_dd(m16[s16])
+    ref24 = O.perm_dd(m24[:2])
+    rel = lambda a, b: float(np.max(np.abs(a - b) / np.abs(b)))
+    parity = {"n16_fp64_vs_oracle_dd": rel(out16[s16], ref16), "n24_fp64_vs_oracle_dd": rel(out24[:2], ref24),
+              "n16_dd_vs_oracle_dd": rel(dd16[s16], ref16), "n24_dd_vs_oracle_dd": rel(dd24[:2], ref24),
+              "samples": [len(s16), 2]}
+    if max(parity["n16_fp64_vs_oracle_dd"], parity["n24_fp64_vs_oracle_dd"]) > 1e-10:
+        print(f"bench: PARITY FAILURE {parity}", file=sys.stderr)
+        return 1
+
+    dom_n = 24
+    dom_flops = perm_flops(dom_n) * m24.shape[0]
+    dom_s = kern_ms[dom_n] * 1e-3
+    # compute-bound ("mfma" class of the schema), but on the FP64 vector ALU: the
+    # Glynn walk has no matrix-product shape; peak = MI355X FP64 vector 78.6 TFLOP/s
+    roofline = {"bound": "mfma", "achieved": dom_flops / dom_s / 1e12, "peak": PEAK_FP64 / 1e12,
+                "unit": "TFLOP/s", "frac": dom_flops / dom_s / PEAK_FP64, "traffic": None,
+                "kernel": f"perm_kernel<{dom_n},fp64>", "flops_per_launch": dom_flops,
+                "avg_launch_ms": kern_ms[dom_n], "pipe": "FP64 VALU (no MFMA)"}
+
+    per_kernel = {}
+    for n, m in cases:
+        f = perm_flops(n) * m.shape[0]
+        per_kernel[f"perm_fp64_n{n}"] = {"batch": m.shape[0], "avg_ms": kern_ms[n],
+                                         "evals_per_s": m.shape[0] / (kern_ms[n] * 1e-3),
+                                         "tflops": f / (kern_ms[n] * 1e-3) / 1e12}
+        per_kernel[f"perm_dd_n{n}"] = {"batch": m.shape[0], "avg_ms": dd_kern_ms[n],
+                                       "evals_per_s": m.shape[0] / (dd_kern_ms[n] * 1e-3)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(O, m16, m24, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "perm Glynn fp64: 65536 x 16x16 + 1024 x 24x24 complex128 rows of a "
+                               "Haar 64x64 unitary (default_rng(1))",
+                   "global_batch": evals_per_step, "tier": "fp64"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "dd": {"evals_per_s": evals_per_step * args.dd_steps / dd_wall,
+               "ms_per_step": dd_wall / args.dd_steps * 1e3, "steps": args.dd_steps},
+        "per_kernel": per_kernel,
+        "parity": parity,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+def cpu_baseline(O, m16, m24, budget_s):
+    """Oracle Glynn (fp64) on the host cores, OpenMP over instances, on a bounded
+    sample; extrapolated to the per-step mix (65536 n=16 + 1024 n=24)."""
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    O.set_threads(cores)
+    O.perm(m16[:cores])  # warm
+    t = time.perf_counter()
+    k16 = 512
+    O.perm(m16[:k16])
+    t16 = (time.perf_counter() - t) / k16
+    # n = 24: size the sample to the remaining budget, at least one eval per core
+    est24 = t16 * 2 ** 8 * 1.5
+    k24 = int(max(cores, min(256, (budget_s * 0.8) / max(est24, 1e-9))))
+    k24 = max(cores, (k24 // cores) * cores)
+    t = time.perf_counter()
+    O.perm(m24[:k24])
+    t24 = (time.perf_counter() - t) / k24
+    step_s = 65536 * t16 + 1024 * t24
+    return {"value": (65536 + 1024) / step_s, "unit": "evals/s", "cores": cores, "kind": "port",
+            "sample": f"oracle Glynn fp64 (cpu_ref restated in C++), {k16} n=16 + {k24} n=24 evals "
+                      f"of the same workload, OpenMP {cores} threads; per-eval times "
+                      f"{t16 * 1e3:.3f} ms (n=16), {t24:.3f} s (n=24) extrapolated to the step mix"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,404 @@
+"""gbskernels_amd -- MI355X (gfx950) batched backend for the permanent, hafnian,
+loop hafnian and torontonian, behind the reference GBSKernels Python API.
+
+Public surface (mirrors the reference ``gbskernels/__init__.py:54-63,517-558,734-908``
+for ``backend="gpu"``): ``perm/haf/lhaf/tor`` and ``*_batched`` with
+``precision in {"fp64", "dd", "auto", "certified", "ref"}``, ``return_diagnostics``
+and ``rtol``; :class:`Workspace` for ragged, device-resident batches;
+:func:`gpu_available` and :func:`gpu_backend_kind`.
+
+Tiers (semantics of the reference, reference ``gbskernels/__init__.py:1-37``):
+
+* ``fp64`` -- one launch of the hand-written HIP kernel per uniform stack;
+* ``dd`` -- the same walk in double-double;
+* ``auto`` -- fp64 plus the on-device cancellation indicator
+  ``kappa = sum|term| / |value|``; elements with ``kappa >= 1e8`` are re-run in
+  ``dd`` on the GPU (``ref`` beyond the dd domain);
+* ``certified`` -- the fp64 value (bit-identical to ``fp64``) plus a rigorous
+  bound; ``rtol`` escalates per element through certified-dd, then ``ref``;
+* ``ref`` -- mpmath ground truth (CPU, tiny sizes), as in the reference.
+
+``backend="cpu"`` serves only ``precision="ref"``: the reference's numpy CPU
+tiers are not part of this package. Nothing here falls back to the CPU
+silently: a missing HIP library raises.
+"""
+
+from __future__ import annotations
+
+from typing import Any, Iterable
+
+import numpy as np
+
+from . import _lib, _ref
+
+__version__ = "0.1.0"
+__all__ = [
+    "perm", "perm_batched", "haf", "haf_batched", "lhaf", "lhaf_batched",
+    "tor", "tor_batched", "gpu_available", "gpu_backend_kind", "Workspace", "PRECISIONS",
+]
+
+PRECISIONS = ("fp64", "ref", "dd", "auto", "certified")
+_FUNCS = ("perm", "haf", "lhaf", "tor")
+# reference gbskernels/__init__.py:240 (calibrated there; kept as the same contract)
+_AUTO_KAPPA_MAX = 1e8
+
+
+def _ext():
+    from .ext import gbskernels_ext as ext
+    _lib.lib()  # raise now if the HIP library is missing
+    return ext
+
+
+def gpu_available() -> bool:
+    """True when ``libgbsk.so`` loads (it always carries device code)."""
+    try:
+        _lib.lib()
+        return True
+    except (OSError, RuntimeError):
+        return False
+
+
+def gpu_backend_kind() -> str:
+    """``"gpu"`` for the loaded HIP library, ``"none"`` if it is absent
+    (reference ``gbskernels/__init__.py:699-713``; there is no host-shim build)."""
+    return "gpu" if gpu_available() else "none"
+
+
+def _max_dim(func: str, tier: str) -> int:
+    return _lib.max_dim(func, tier)
+
+
+# --- validation (reference gbskernels/__init__.py:158-219) ---------------------
+
+def _prepare(func: str, matrices: Iterable[Any], tier: str) -> np.ndarray:
+    """Uniform (B, d, d) complex128 stack for one launch; augments odd loop
+    hafnians; enforces the kernel's size range."""
+    try:
+        stack = np.ascontiguousarray(np.asarray(matrices, dtype=np.complex128))
+    except ValueError:
+        stack = None
+    if stack is None or stack.ndim != 3 or stack.shape[1] != stack.shape[2]:
+        raise ValueError(
+            "GPU backend needs a uniform (B, d, d) stack of square matrices; "
+            "ragged batches go through gbskernels_amd.Workspace.")
+    d = int(stack.shape[1])
+    if func == "lhaf" and d % 2:
+        # lhaf(A) == lhaf(A (+) [1]): the added vertex can only loop, weight 1
+        aug = np.zeros((stack.shape[0], d + 1, d + 1), np.complex128)
+        aug[:, :d, :d] = stack
+        aug[:, d, d] = 1.0
+        stack, d = np.ascontiguousarray(aug), d + 1
+    if func == "tor" and d % 2:
+        raise ValueError("torontonian: matrix dimension must be even (2n x 2n)")
+    cap = _max_dim(func, tier)
+    if d > cap:
+        raise ValueError(
+            f"GPU {func} ({tier}): matrix dimension {d} exceeds the kernel's limit {cap}"
+            + (" (an odd loop hafnian is augmented to N+1)" if func == "lhaf" else ""))
+    if func == "tor" and tier in ("dd", "dd_certified") and stack.size \
+            and float(np.max(np.abs(stack.imag))) > 1e-12:
+        raise ValueError("GPU torontonian (precision='dd') is real-domain only; the input "
+                         "has non-negligible imaginary parts")
+    return stack
+
+
+def _run(func: str, tier: str, stack: np.ndarray):
+    ext = _ext()
+    name = func if tier == "fp64" else f"{func}_{tier}"
+    return getattr(ext, name)(stack)
+
+
+# --- auto (reference gbskernels/__init__.py:243-297) ---------------------------
+
+def _rerun_risky(func: str, stack: np.ndarray, values: np.ndarray, risky: np.ndarray) -> str:
+    sub = np.ascontiguousarray(stack[risky])
+    try:
+        sub = _prepare(func, sub, "dd")
+    except ValueError:
+        for i in np.nonzero(risky)[0]:
+            values[i] = _ref.evaluate(func, stack[i])
+        return "ref"
+    values[risky] = _run(func, "dd", sub)
+    return "dd"
+
+
+def _auto_batched(func: str, matrices, return_diagnostics: bool):
+    stack = _prepare(func, matrices, "kappa")
+    values, absnorm = _run(func, "kappa", stack)
+    values = values.astype(np.complex128, copy=True)
+    kappa = absnorm / np.maximum(np.abs(values), 1e-300)
+    risky = kappa >= _AUTO_KAPPA_MAX
+    tier = _rerun_risky(func, stack, values, risky) if risky.any() else None
+    if not return_diagnostics:
+        return values
+    diags = [{"tier": tier if r else "fp64", "cancellation": float(k)} for r, k in zip(risky, kappa)]
+    return values, diags
+
+
+# --- certified (reference gbskernels/__init__.py:350-478) ----------------------
+
+def _check_certified_input(A) -> None:
+    try:
+        finite = bool(np.all(np.isfinite(np.asarray(A))))
+    except (TypeError, ValueError):
+        finite = False
+    if not finite:
+        raise ValueError("certified evaluation requires finite input entries")
+
+
+def _finite_certificate(v: complex, e: float) -> bool:
+    return bool(np.isfinite(v.real) and np.isfinite(v.imag) and np.isfinite(e) and e >= 0.0)
+
+
+def _certificate_rel(v: complex, e: float) -> float:
+    if not _finite_certificate(v, e):
+        return float("inf")
+    mag = abs(v)
+    if mag == 0.0:
+        return 0.0 if e == 0.0 else float("inf")
+    return float(e / mag)
+
+
+def _escalate(func: str, A, bound: float, valid: bool, rel: float, rtol: float):
+    """The proven ladder past certified-fp64: certified-dd, then ref."""
+    try:
+        stack = _prepare(func, np.asarray(A)[None], "dd_certified")
+        v2, b2 = _run(func, "dd_certified", stack)
+        v2c, b2f = complex(v2[0]), float(b2[0])
+        rel2 = _certificate_rel(v2c, b2f)
+        if _finite_certificate(v2c, b2f) and np.isfinite(rel2) and rel2 <= rtol:
+            return v2c, {"tier": "certified-dd", "escalated": True,
+                         "abs_error_bound": b2f, "rel_error_bound": float(rel2)}
+    except (ValueError, NotImplementedError):
+        pass
+    ref = _ref.evaluate(func, A)
+    if not (np.isfinite(ref.real) and np.isfinite(ref.imag)):
+        raise FloatingPointError(f"reference {func} evaluator returned a non-finite value")
+    return ref, {"tier": "ref", "escalated": True,
+                 "abs_error_bound": bound if valid else float("inf"), "rel_error_bound": float(rel)}
+
+
+def _certified_batched(func: str, matrices, rtol):
+    stack = _prepare(func, matrices, "certified")
+    _check_certified_input(stack)
+    values, bounds = _run(func, "certified", stack)
+    out = values.astype(np.complex128, copy=True)
+    diags = []
+    for i in range(len(out)):
+        v, e = complex(out[i]), float(bounds[i])
+        valid = _finite_certificate(v, e)
+        rel = _certificate_rel(v, e)
+        if rtol is not None and (not valid or not np.isfinite(rel) or rel > rtol):
+            out[i], d = _escalate(func, stack[i], e, valid, rel, rtol)
+        elif not valid:
+            raise FloatingPointError(
+                f"certified {func} batch element {i} refused with a non-finite value or bound")
+        else:
+            d = {"tier": "certified-fp64", "abs_error_bound": e, "rel_error_bound": rel}
+            if rtol is not None:
+                d["escalated"] = False
+        diags.append(d)
+    return out, diags
+
+
+# --- dispatch (reference gbskernels/__init__.py:481-514) -----------------------
+
+def _check_args(precision, backend, return_diagnostics, rtol):
+    if precision not in PRECISIONS:
+        raise ValueError(f"unknown precision {precision!r}; expected one of {PRECISIONS}")
+    if backend not in ("cpu", "gpu"):
+        raise ValueError(f"unknown backend {backend!r}; expected 'cpu' or 'gpu'")
+    if return_diagnostics and precision not in ("auto", "certified"):
+        raise ValueError("return_diagnostics=True requires precision='auto' or 'certified'")
+    if precision == "certified" and not return_diagnostics:
+        raise ValueError("precision='certified' returns its error bound via diagnostics; call with "
+                         "return_diagnostics=True (a certificate you cannot see is not a certificate)")
+    if rtol is not None and precision != "certified":
+        raise ValueError("rtol is a certified-tier parameter; it requires precision='certified'")
+    if rtol is not None and (not np.isfinite(rtol) or rtol < 0.0):
+        raise ValueError("rtol must be finite and non-negative")
+    if precision == "ref" and backend == "gpu":
+        raise NotImplementedError("'ref' is the mpmath ground truth -- CPU only (backend='cpu')")
+    if backend == "cpu" and precision != "ref":
+        raise NotImplementedError(
+            "gbskernels_amd is the GPU backend: backend='cpu' serves precision='ref' only "
+            "(the numpy CPU tiers are the reference package's cpu_ref)")
+
+
+def _dispatch_batched(func, matrices, precision, backend, return_diagnostics=False, rtol=None):
+    _check_args(precision, backend, return_diagnostics, rtol)
+    if precision == "ref":
+        batch = matrices if isinstance(matrices, np.ndarray) and matrices.ndim == 3 else list(matrices)
+        return np.array([_ref.evaluate(func, A) for A in batch], dtype=np.complex128)
+    if precision == "auto":
+        return _auto_batched(func, matrices, return_diagnostics)
+    if precision == "certified":
+        return _certified_batched(func, matrices, rtol)
+    return np.asarray(_run(func, precision, _prepare(func, matrices, precision)))
+
+
+def _dispatch_single(func, A, precision, backend, return_diagnostics=False, rtol=None):
+    _check_args(precision, backend, return_diagnostics, rtol)
+    if precision == "ref":
+        return _ref.evaluate(func, A)
+    if precision == "certified":
+        _check_certified_input(A)
+    res = _dispatch_batched(func, np.asarray(A)[None, ...], precision, backend, return_diagnostics, rtol)
+    if return_diagnostics:
+        vals, diags = res
+        return complex(vals[0]), diags[0]
+    return complex(res[0])
+
+
+def perm(A, precision: str = "fp64", backend: str = "gpu", return_diagnostics: bool = False, rtol=None):
+    """Permanent of one square matrix."""
+    return _dispatch_single("perm", A, precision, backend, return_diagnostics, rtol)
+
+
+def perm_batched(matrices, precision: str = "fp64", backend: str = "gpu", return_diagnostics: bool = False, rtol=None):
+    """Permanents of a uniform (B, n, n) stack -> complex128 (B,)."""
+    return _dispatch_batched("perm", matrices, precision, backend, return_diagnostics, rtol)
+
+
+def haf(A, precision: str = "fp64", backend: str = "gpu", return_diagnostics: bool = False, rtol=None):
+    """Hafnian of one symmetric matrix (the diagonal is ignored)."""
+    return _dispatch_single("haf", A, precision, backend, return_diagnostics, rtol)
+
+
+def haf_batched(matrices, precision: str = "fp64", backend: str = "gpu", return_diagnostics: bool = False, rtol=None):
+    """Hafnians of a uniform (B, N, N) stack of symmetric matrices."""
+    return _dispatch_batched("haf", matrices, precision, backend, return_diagnostics, rtol)
+
+
+def lhaf(A, precision: str = "fp64", backend: str = "gpu", return_diagnostics: bool = False, rtol=None):
+    """Loop hafnian of one symmetric matrix (diagonal = loop weights)."""
+    return _dispatch_single("lhaf", A, precision, backend, return_diagnostics, rtol)
+
+
+def lhaf_batched(matrices, precision: str = "fp64", backend: str = "gpu", return_diagnostics: bool = False, rtol=None):
+    """Loop hafnians of a uniform stack; odd N is augmented to N + 1."""
+    return _dispatch_batched("lhaf", matrices, precision, backend, return_diagnostics, rtol)
+
+
+def tor(O, precision: str = "fp64", backend: str = "gpu", return_diagnostics: bool = False, rtol=None):
+    """Torontonian of one 2n x 2n matrix O (xxpp ordering)."""
+    return _dispatch_single("tor", O, precision, backend, return_diagnostics, rtol)
+
+
+def tor_batched(matrices, precision: str = "fp64", backend: str = "gpu", return_diagnostics: bool = False, rtol=None):
+    """Torontonians of a uniform (B, 2n, 2n) stack."""
+    return _dispatch_batched("tor", matrices, precision, backend, return_diagnostics, rtol)
+
+
+# --- Workspace (reference gbskernels/__init__.py:716-908) ----------------------
+
+class Workspace:
+    """Bucketing, device-resident handle for ragged batches.
+
+    Groups a ragged list into one uniform launch per size (first-seen order),
+    reuses one session's grow-only device buffers across calls, and scatters
+    results back to input order; element ``i``'s value is identical to
+    evaluating it alone."""
+
+    def __init__(self, backend: str = "gpu") -> None:
+        if backend not in ("cpu", "gpu"):
+            raise ValueError(f"unknown backend {backend!r}; expected 'cpu' or 'gpu'")
+        self._backend = backend
+        self._session = None
+        self._closed = False
+
+    def __enter__(self) -> "Workspace":
+        if self._backend == "gpu":
+            self._session = _ext().Session()
+        return self
+
+    def __exit__(self, *exc) -> bool:
+        self.close()
+        return False
+
+    def close(self) -> None:
+        if self._closed:
+            return
+        if self._session is not None:
+            self._session.close()
+        self._session = None
+        self._closed = True
+
+    def perm_batched(self, matrices, precision: str = "fp64") -> np.ndarray:
+        return self._ragged("perm", matrices, precision)
+
+    def haf_batched(self, matrices, precision: str = "fp64") -> np.ndarray:
+        return self._ragged("haf", matrices, precision)
+
+    def lhaf_batched(self, matrices, precision: str = "fp64") -> np.ndarray:
+        return self._ragged("lhaf", matrices, precision)
+
+    def tor_batched(self, matrices, precision: str = "fp64") -> np.ndarray:
+        return self._ragged("tor", matrices, precision)
+
+    def _ragged(self, func: str, matrices, precision: str) -> np.ndarray:
+        if self._closed:
+            raise RuntimeError("Workspace is closed")
+        mats = []
+        for i, m in enumerate(matrices):
+            a = np.ascontiguousarray(np.asarray(m, dtype=np.complex128))
+            if a.ndim != 2 or a.shape[0] != a.shape[1]:
+                raise ValueError(f"Workspace: matrix #{i} must be square 2-D; got shape {a.shape}")
+            mats.append(a)
+        out = np.empty(len(mats), dtype=np.complex128)
+        if not mats:
+            return out
+        buckets: dict[int, list[int]] = {}
+        for i, a in enumerate(mats):
+            buckets.setdefault(a.shape[0], []).append(i)
+        for d, idxs in buckets.items():
+            stack = np.ascontiguousarray(np.stack([mats[i] for i in idxs]))
+            try:
+                res = self._bucket(func, stack, precision)
+            except ValueError as e:
+                raise ValueError(f"matrix #{idxs[0]} (dim {d}): {e}") from e
+            out[idxs] = res
+        return out
+
+    def _bucket(self, func: str, stack: np.ndarray, precision: str) -> np.ndarray:
+        if self._backend == "cpu":
+            return _dispatch_batched(func, stack, precision, "cpu")
+        if precision == "auto":
+            return _auto_batched(func, stack, False)
+        if precision not in ("fp64", "dd"):
+            raise NotImplementedError(f"Workspace precision {precision!r}: use 'fp64', 'dd' or 'auto'")
+        stack = _prepare(func, stack, precision)
+        name = func if precision == "fp64" else f"{func}_dd"
+        return np.asarray(getattr(self._session, name)(stack))
+
+    def perm_resident(self, matrices, precision: str = "fp64"):
+        return self._resident("perm", matrices, precision)
+
+    def haf_resident(self, matrices, precision: str = "fp64"):
+        return self._resident("haf", matrices, precision)
+
+    def lhaf_resident(self, matrices, precision: str = "fp64"):
+        return self._resident("lhaf", matrices, precision)
+
+    def tor_resident(self, matrices, precision: str = "fp64"):
+        return self._resident("tor", matrices, precision)
+
+    def _resident(self, func: str, matrices, precision: str):
+        """Uniform batch evaluated with the result left in device memory
+        (a :class:`~gbskernels_amd.ext.gbskernels_ext.DeviceResult`)."""
+        if self._closed:
+            raise RuntimeError("Workspace is closed")
+        if self._backend != "gpu":
+            raise NotImplementedError("device-resident output requires backend='gpu'")
+        if precision not in ("fp64", "dd"):
+            raise NotImplementedError("device-resident output supports 'fp64' and 'dd'")
+        try:
+            stack = np.ascontiguousarray(np.asarray(matrices, dtype=np.complex128))
+            uniform = stack.ndim == 3
+        except (ValueError, TypeError):
+            uniform = False
+        if not uniform:
+            raise ValueError("device-resident output needs a uniform (B, d, d) batch")
+        stack = _prepare(func, stack, precision)
+        name = func if precision == "fp64" else f"{func}_dd"
+        return self._session._resident(name, stack)

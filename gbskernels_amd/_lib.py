@@ -1,0 +1,98 @@
+"""ctypes binding of ``libgbsk.so`` (the C ABI declared in ``include/gbsk.h``).
+
+There is no CPU fallback: if the library is missing or was built without the
+HIP kernels, :func:`lib` raises. ctypes releases the GIL around each foreign
+call, and the library serialises host calls on its own context, so Python
+threads may call concurrently.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("GBSK_LIB", _HERE / "libgbsk.so"))
+
+GBSK_ERR_ARG = 10001
+GBSK_ERR_DIM = 10002
+GBSK_ERR_FUNC = 10003
+GBSK_ERR_WORKSPACE = 10004
+GBSK_ERR_ALLOC = 10005
+
+FUNCS = {"perm": 0, "haf": 1, "lhaf": 2, "tor": 3}
+TIERS = {"fp64": 0, "dd": 1, "kappa": 2, "certified": 3, "dd_certified": 4}
+
+_cp = ctypes.c_void_p
+_i = ctypes.c_int
+_dp = ctypes.POINTER(ctypes.c_double)
+
+# every symbol include/gbsk.h declares, with its ctypes signature
+SIGNATURES = {
+    **{f"gbs_{f}_host": ([_cp, _i, _i, _cp], _i) for f in ("perm", "perm_dd", "haf", "haf_dd", "lhaf", "lhaf_dd", "tor", "tor_dd")},
+    **{f"gbs_{f}_{k}_host": ([_cp, _i, _i, _cp, _cp], _i)
+       for f in ("perm", "haf", "lhaf", "tor") for k in ("kappa", "certified", "dd_certified")},
+    "gbs_session_open": ([ctypes.POINTER(_cp)], _i),
+    "gbs_session_evaluate": ([_cp, _i, _i, _cp, _i, _i, _cp], _i),
+    "gbs_session_close": ([_cp], _i),
+    "gbs_session_reallocs": ([_cp], ctypes.c_size_t),
+    "gbs_session_evaluate_resident": ([_cp, _i, _i, _cp, _i, _i, ctypes.POINTER(_cp)], _i),
+    "gbs_dev_to_host": ([_cp, _cp, _i], _i),
+    "gbs_dev_free": ([_cp], _i),
+    "gbs_workspace_bytes": ([_i, _i, _i, _i], ctypes.c_size_t),
+    "gbs_eval_device": ([_i, _i, _cp, _i, _i, _cp, _cp, _cp, ctypes.c_size_t, _cp], _i),
+    "gbs_max_dim": ([_i, _i], _i),
+    "gbs_error_string": ([_i], ctypes.c_char_p),
+    "gbs_is_device_build": ([], _i),
+}
+
+_lib = None
+
+
+class GbskError(RuntimeError):
+    """A nonzero return code from libgbsk."""
+
+    def __init__(self, rc: int, where: str):
+        self.rc = rc
+        msg = lib().gbs_error_string(rc).decode()
+        super().__init__(f"gbskernels: GPU error {rc} ({msg}) in {where}")
+
+
+def lib():
+    """Load ``libgbsk.so`` once; raise if it is absent (no silent fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"gbskernels_amd: the HIP library {LIB_PATH} is not built; run "
+            "`python -c 'import __graft_entry__ as g; g.build()'` (hipcc, gfx950)")
+    L = ctypes.CDLL(str(LIB_PATH))
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    if L.gbs_is_device_build() != 1:
+        raise RuntimeError("gbskernels_amd: libgbsk.so is not a device build")
+    _lib = L
+    return L
+
+
+def check(rc: int, where: str) -> None:
+    if rc != 0:
+        if rc in (GBSK_ERR_DIM, GBSK_ERR_ARG):
+            raise ValueError(f"gbskernels: {lib().gbs_error_string(rc).decode()} in {where}")
+        if rc == GBSK_ERR_FUNC:
+            raise NotImplementedError(f"gbskernels: {lib().gbs_error_string(rc).decode()} in {where}")
+        raise GbskError(rc, where)
+
+
+def ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def max_dim(func: str, tier: str) -> int:
+    return int(lib().gbs_max_dim(FUNCS[func], TIERS[tier]))

@@ -1,0 +1,285 @@
+// api.hip -- the C ABI (include/gbsk.h): dispatch, one-shot host entry points,
+// sessions and stream-ordered device evaluation.
+//
+// Replaces core/host_api.cu. Differences by design:
+//  * no hipMalloc/hipFree per call: one process-wide context owns a HIP stream
+//    and grow-only device buffers, serialised by a mutex (ctypes drops the GIL,
+//    so concurrent Python threads are safe);
+//  * no device-wide synchronisation: every call waits on its own stream only;
+//  * the torontonian's real-domain test and the recursive/complex dispatch run
+//    on the device, not as an O(B d^2) host scan.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <new>
+
+#include "../../include/gbsk.h"
+#include "common.hpp"
+#include "errors.hpp"
+#include "kernels.hpp"
+
+namespace gbsk {
+
+static_assert(sizeof(gbs_complex) == sizeof(cplx), "gbs_complex layout");
+
+// kernel dimension the launchers take: matrix side for perm/haf/lhaf, modes for tor
+static size_t ws_bytes(int func, int tier, int dim, int B) {
+  switch (func) {
+    case GBSK_PERM: return perm_workspace_bytes(dim, tier == GBSK_DD ? PERM_DD : tier == GBSK_KAPPA ? PERM_KAPPA : PERM_FP64, B);
+    case GBSK_HAF:
+    case GBSK_LHAF: return haf_workspace_bytes(func == GBSK_LHAF, tier, dim, B);
+    case GBSK_TOR: return tor_workspace_bytes(tier, dim, B);
+  }
+  return 0;
+}
+
+static int launch(int func, int tier, const cplx* d_in, int dim, int B, cplx* d_out, double* d_aux, void* ws,
+                  size_t ws_size, hipStream_t st) {
+  if (B <= 0) return 0;
+  if (!d_in || !d_out) return GBSK_ERR_ARG;
+  if (ws_size < ws_bytes(func, tier, dim, B)) return GBSK_ERR_WORKSPACE;
+  switch (func) {
+    case GBSK_PERM:
+      switch (tier) {
+        case GBSK_FP64: return perm_launch(dim, PERM_FP64, d_in, B, d_out, nullptr, ws, st);
+        case GBSK_DD: return perm_launch(dim, PERM_DD, d_in, B, d_out, nullptr, ws, st);
+        case GBSK_KAPPA:
+          if (!d_aux) return GBSK_ERR_ARG;
+          return perm_launch(dim, PERM_KAPPA, d_in, B, d_out, d_aux, ws, st);
+        default: return GBSK_ERR_FUNC;
+      }
+    case GBSK_HAF:
+    case GBSK_LHAF:
+      if ((tier == GBSK_KAPPA || tier >= GBSK_CERTIFIED) && !d_aux) return GBSK_ERR_ARG;
+      return haf_launch(func == GBSK_LHAF, tier, dim, d_in, B, d_out, d_aux, ws, st);
+    case GBSK_TOR:
+      if ((tier == GBSK_KAPPA || tier >= GBSK_CERTIFIED) && !d_aux) return GBSK_ERR_ARG;
+      return tor_launch(tier, dim, d_in, B, d_out, d_aux, ws, st);
+  }
+  return GBSK_ERR_FUNC;
+}
+
+// ------------------------------------------------------------ host context
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int grow(size_t need, size_t* grew = nullptr) {
+    if (need <= cap) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(&p, need) != hipSuccess) {
+      p = nullptr;
+      return (int)hipErrorOutOfMemory;
+    }
+    cap = need;
+    if (grew) ++*grew;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct Ctx {
+  std::mutex mu;
+  hipStream_t stream = nullptr;
+  DevBuf in, out, aux, ws;
+  size_t reallocs = 0;  // times `in` grew: the reuse witness of a session
+  int init() {
+    if (stream) return 0;
+    return (int)hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+  }
+};
+
+static Ctx& global_ctx() {
+  static Ctx* c = new Ctx();  // intentionally leaked: HIP may be torn down first at exit
+  return *c;
+}
+
+// The one-shot path shared by every host entry point. `dim` is the matrix side.
+static int run_host(Ctx& c, int func, int tier, const gbs_complex* h_in, int dim, int B, gbs_complex* h_out,
+                    double* h_aux, bool count_reallocs) {
+  if (B <= 0) return 0;
+  if (!h_in || !h_out || dim < 0) return GBSK_ERR_ARG;
+  const bool want_aux = (tier == GBSK_KAPPA || tier == GBSK_CERTIFIED || tier == GBSK_DD_CERTIFIED);
+  if (want_aux && !h_aux) return GBSK_ERR_ARG;
+  int rc = c.init();
+  const size_t in_bytes = (size_t)B * dim * dim * sizeof(cplx);
+  const size_t wsb = ws_bytes(func, tier, dim, B);
+  if (!rc) rc = c.in.grow(in_bytes > 0 ? in_bytes : 16, count_reallocs ? &c.reallocs : nullptr);
+  if (!rc) rc = c.out.grow((size_t)B * sizeof(cplx));
+  if (!rc) rc = c.aux.grow((size_t)B * sizeof(double));
+  if (!rc && wsb) rc = c.ws.grow(wsb);
+  if (!rc && in_bytes) rc = (int)hipMemcpyAsync(c.in.p, h_in, in_bytes, hipMemcpyHostToDevice, c.stream);
+  if (!rc)
+    rc = launch(func, tier, (const cplx*)c.in.p, dim, B, (cplx*)c.out.p, (double*)c.aux.p, c.ws.p, c.ws.cap,
+                c.stream);
+  if (!rc) rc = (int)hipMemcpyAsync(h_out, c.out.p, (size_t)B * sizeof(cplx), hipMemcpyDeviceToHost, c.stream);
+  if (!rc && want_aux)
+    rc = (int)hipMemcpyAsync(h_aux, c.aux.p, (size_t)B * sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  int rs = (int)hipStreamSynchronize(c.stream);
+  return rc ? rc : rs;
+}
+
+static int host_call(int func, int tier, const gbs_complex* in, int kdim, int B, gbs_complex* out, double* aux) {
+  Ctx& c = global_ctx();
+  std::lock_guard<std::mutex> g(c.mu);
+  const int dim = (func == GBSK_TOR) ? 2 * kdim : kdim;
+  if (kdim < 0) return GBSK_ERR_DIM;
+  return run_host(c, func, tier, in, dim, B, out, aux, false);
+}
+
+}  // namespace gbsk
+
+using namespace gbsk;
+
+struct gbs_session {
+  Ctx ctx;
+};
+
+extern "C" {
+
+int gbs_perm_host(const gbs_complex* m, int n, int b, gbs_complex* o) { return host_call(GBSK_PERM, GBSK_FP64, m, n, b, o, nullptr); }
+int gbs_perm_dd_host(const gbs_complex* m, int n, int b, gbs_complex* o) { return host_call(GBSK_PERM, GBSK_DD, m, n, b, o, nullptr); }
+int gbs_haf_host(const gbs_complex* m, int N, int b, gbs_complex* o) { return host_call(GBSK_HAF, GBSK_FP64, m, N, b, o, nullptr); }
+int gbs_haf_dd_host(const gbs_complex* m, int N, int b, gbs_complex* o) { return host_call(GBSK_HAF, GBSK_DD, m, N, b, o, nullptr); }
+int gbs_lhaf_host(const gbs_complex* m, int N, int b, gbs_complex* o) { return host_call(GBSK_LHAF, GBSK_FP64, m, N, b, o, nullptr); }
+int gbs_lhaf_dd_host(const gbs_complex* m, int N, int b, gbs_complex* o) { return host_call(GBSK_LHAF, GBSK_DD, m, N, b, o, nullptr); }
+int gbs_tor_host(const gbs_complex* m, int n, int b, gbs_complex* o) { return host_call(GBSK_TOR, GBSK_FP64, m, n, b, o, nullptr); }
+int gbs_tor_dd_host(const gbs_complex* m, int n, int b, gbs_complex* o) { return host_call(GBSK_TOR, GBSK_DD, m, n, b, o, nullptr); }
+
+int gbs_perm_kappa_host(const gbs_complex* m, int n, int b, gbs_complex* o, double* a) { return host_call(GBSK_PERM, GBSK_KAPPA, m, n, b, o, a); }
+int gbs_haf_kappa_host(const gbs_complex* m, int N, int b, gbs_complex* o, double* a) { return host_call(GBSK_HAF, GBSK_KAPPA, m, N, b, o, a); }
+int gbs_lhaf_kappa_host(const gbs_complex* m, int N, int b, gbs_complex* o, double* a) { return host_call(GBSK_LHAF, GBSK_KAPPA, m, N, b, o, a); }
+int gbs_tor_kappa_host(const gbs_complex* m, int n, int b, gbs_complex* o, double* a) { return host_call(GBSK_TOR, GBSK_KAPPA, m, n, b, o, a); }
+
+int gbs_perm_certified_host(const gbs_complex* m, int n, int b, gbs_complex* o, double* e) { return host_call(GBSK_PERM, GBSK_CERTIFIED, m, n, b, o, e); }
+int gbs_haf_certified_host(const gbs_complex* m, int N, int b, gbs_complex* o, double* e) { return host_call(GBSK_HAF, GBSK_CERTIFIED, m, N, b, o, e); }
+int gbs_lhaf_certified_host(const gbs_complex* m, int N, int b, gbs_complex* o, double* e) { return host_call(GBSK_LHAF, GBSK_CERTIFIED, m, N, b, o, e); }
+int gbs_tor_certified_host(const gbs_complex* m, int n, int b, gbs_complex* o, double* e) { return host_call(GBSK_TOR, GBSK_CERTIFIED, m, n, b, o, e); }
+int gbs_perm_dd_certified_host(const gbs_complex* m, int n, int b, gbs_complex* o, double* e) { return host_call(GBSK_PERM, GBSK_DD_CERTIFIED, m, n, b, o, e); }
+int gbs_haf_dd_certified_host(const gbs_complex* m, int N, int b, gbs_complex* o, double* e) { return host_call(GBSK_HAF, GBSK_DD_CERTIFIED, m, N, b, o, e); }
+int gbs_lhaf_dd_certified_host(const gbs_complex* m, int N, int b, gbs_complex* o, double* e) { return host_call(GBSK_LHAF, GBSK_DD_CERTIFIED, m, N, b, o, e); }
+int gbs_tor_dd_certified_host(const gbs_complex* m, int n, int b, gbs_complex* o, double* e) { return host_call(GBSK_TOR, GBSK_DD_CERTIFIED, m, n, b, o, e); }
+
+// --- session -----------------------------------------------------------------
+int gbs_session_open(gbs_session** out) {
+  if (!out) return GBSK_ERR_ARG;
+  *out = nullptr;
+  gbs_session* s = new (std::nothrow) gbs_session();
+  if (!s) return GBSK_ERR_ALLOC;
+  int rc = s->ctx.init();
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return 0;
+}
+
+int gbs_session_evaluate(gbs_session* s, int func, int prec, const gbs_complex* in, int dim, int B, gbs_complex* out) {
+  if (!s) return GBSK_ERR_ARG;
+  if (func < GBSK_PERM || func > GBSK_TOR || (prec != GBSK_FP64 && prec != GBSK_DD)) return GBSK_ERR_FUNC;
+  if (func == GBSK_TOR && (dim & 1)) return GBSK_ERR_DIM;
+  std::lock_guard<std::mutex> g(s->ctx.mu);
+  return run_host(s->ctx, func, prec, in, dim, B, out, nullptr, true);
+}
+
+int gbs_session_evaluate_resident(gbs_session* s, int func, int prec, const gbs_complex* in, int dim, int B,
+                                  gbs_complex** d_result) {
+  if (!s || !d_result) return GBSK_ERR_ARG;
+  *d_result = nullptr;
+  if (B <= 0) return 0;
+  if (!in || dim < 0) return GBSK_ERR_ARG;
+  if (func < GBSK_PERM || func > GBSK_TOR || (prec != GBSK_FP64 && prec != GBSK_DD)) return GBSK_ERR_FUNC;
+  if (func == GBSK_TOR && (dim & 1)) return GBSK_ERR_DIM;
+  Ctx& c = s->ctx;
+  std::lock_guard<std::mutex> g(c.mu);
+  const size_t in_bytes = (size_t)B * dim * dim * sizeof(cplx);
+  const size_t wsb = ws_bytes(func, prec, dim, B);
+  int rc = c.in.grow(in_bytes > 0 ? in_bytes : 16, &c.reallocs);
+  if (!rc && wsb) rc = c.ws.grow(wsb);
+  if (!rc) rc = c.aux.grow((size_t)B * sizeof(double));
+  void* res = nullptr;
+  if (!rc) rc = (int)hipMalloc(&res, (size_t)B * sizeof(cplx));
+  if (!rc && in_bytes) rc = (int)hipMemcpyAsync(c.in.p, in, in_bytes, hipMemcpyHostToDevice, c.stream);
+  if (!rc) rc = launch(func, prec, (const cplx*)c.in.p, dim, B, (cplx*)res, (double*)c.aux.p, c.ws.p, c.ws.cap, c.stream);
+  int rs = (int)hipStreamSynchronize(c.stream);
+  if (!rc) rc = rs;
+  if (rc) {
+    if (res) (void)hipFree(res);
+    return rc;
+  }
+  *d_result = (gbs_complex*)res;
+  return 0;
+}
+
+int gbs_session_close(gbs_session* s) {
+  if (!s) return 0;
+  {
+    std::lock_guard<std::mutex> g(s->ctx.mu);
+    if (s->ctx.stream) (void)hipStreamSynchronize(s->ctx.stream);
+    s->ctx.in.release();
+    s->ctx.out.release();
+    s->ctx.aux.release();
+    s->ctx.ws.release();
+    if (s->ctx.stream) (void)hipStreamDestroy(s->ctx.stream);
+    s->ctx.stream = nullptr;
+  }
+  delete s;
+  return 0;
+}
+
+size_t gbs_session_reallocs(const gbs_session* s) { return s ? s->ctx.reallocs : 0; }
+
+int gbs_dev_to_host(const gbs_complex* d, gbs_complex* h, int n) {
+  if (n <= 0) return 0;
+  if (!d || !h) return GBSK_ERR_ARG;
+  return (int)hipMemcpy(h, d, (size_t)n * sizeof(gbs_complex), hipMemcpyDeviceToHost);
+}
+
+int gbs_dev_free(gbs_complex* d) {
+  if (d) (void)hipFree(d);
+  return 0;
+}
+
+// --- stream-ordered device evaluation -----------------------------------------
+size_t gbs_workspace_bytes(int func, int tier, int dim, int B) { return ws_bytes(func, tier, dim, B); }
+
+int gbs_eval_device(int func, int tier, const gbs_complex* d_in, int dim, int B, gbs_complex* d_out, double* d_aux,
+                    void* d_ws, size_t ws_size, void* stream) {
+  if (dim < 0) return GBSK_ERR_DIM;
+  if (func == GBSK_TOR && (dim & 1)) return GBSK_ERR_DIM;
+  if (dim > gbs_max_dim(func, tier)) return GBSK_ERR_DIM;
+  return launch(func, tier, (const cplx*)d_in, dim, B, (cplx*)d_out, d_aux, d_ws, ws_size, (hipStream_t)stream);
+}
+
+int gbs_max_dim(int func, int tier) {
+  switch (func) {
+    case GBSK_PERM: return (tier == GBSK_FP64 || tier == GBSK_DD || tier == GBSK_KAPPA) ? PERM_MAX_N : perm_cert_max_dim(tier);
+    case GBSK_HAF:
+    case GBSK_LHAF: return haf_max_dim(func == GBSK_LHAF, tier);
+    case GBSK_TOR: return tor_max_dim(tier);
+  }
+  return 0;
+}
+
+const char* gbs_error_string(int rc) {
+  switch (rc) {
+    case 0: return "success";
+    case GBSK_ERR_ARG: return "invalid argument";
+    case GBSK_ERR_DIM: return "matrix dimension outside the kernel's supported range";
+    case GBSK_ERR_FUNC: return "unknown or unsupported (function, tier)";
+    case GBSK_ERR_WORKSPACE: return "workspace smaller than gbs_workspace_bytes()";
+    case GBSK_ERR_ALLOC: return "host allocation failed";
+  }
+  return hipGetErrorString((hipError_t)rc);
+}
+
+int gbs_is_device_build(void) { return 1; }
+
+}  // extern "C"

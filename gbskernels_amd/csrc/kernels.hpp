@@ -1,0 +1,27 @@
+// kernels.hpp -- launcher entry points of every kernel family (device pointers,
+// stream-ordered; return 0 or an error code from errors.hpp).
+#pragma once
+
+#include <stddef.h>
+
+#include "common.hpp"
+#include "errors.hpp"
+#include "perm.hpp"
+
+namespace gbsk {
+
+// certified permanent tiers (not in this build yet: 0 = unsupported)
+int perm_cert_max_dim(int tier);
+
+// power-trace hafnian / loop hafnian; tier codes as in include/gbsk.h
+size_t haf_workspace_bytes(bool loop, int tier, int N, int B);
+int haf_launch(bool loop, int tier, int N, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws,
+               hipStream_t st);
+int haf_max_dim(bool loop, int tier);
+
+// torontonian; dim = matrix side 2n
+size_t tor_workspace_bytes(int tier, int dim, int B);
+int tor_launch(int tier, int dim, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st);
+int tor_max_dim(int tier);
+
+}  // namespace gbsk

@@ -1,0 +1,312 @@
+// perm.hip -- Glynn / BB-FG Gray-code permanent on gfx950 (fp64, kappa, dd).
+//
+// Algorithm (reference: core/permanent.cu:30-76, cpu_ref/permanent.py:108-139):
+//   perm(A) = 2^-(n-1) sum_{delta, delta_0=+1} (prod_k delta_k) prod_r sum_c delta_c A[r,c]
+// walked in reflected Gray-code order, so consecutive terms differ in one
+// delta_col and the row sums change by +-2 A[:,col].
+//
+// MI355X mapping (this file's own design, not the reference's one-thread-per-
+// matrix): the 2^(n-1) term space of one matrix is cut into contiguous,
+// 2^M-aligned Gray ranges, one per lane; for n >= 20 the ranges of one matrix
+// also span several workgroups ("chunks"). Because every range starts at a
+// multiple of 2^M, the flipped column ctz(i) of a lane's j-th step is ctz(j) for
+// ALL lanes: the whole wave walks the same column sequence in lockstep. The row
+// sums live in VGPRs (fully unrolled over a compile-time n); columns 1..HOT
+// (flipped every 2nd, 4th, ... step) are cached in VGPRs, the rarer columns are
+// read from an LDS copy of A^T with a wave-uniform address (a broadcast read).
+// Lane partials are combined by a butterfly over the group's lanes, chunk
+// partials by a fixed-order second pass. The chunking depends only on n, so a
+// matrix's value never depends on what else is in the batch.
+#include "common.hpp"
+#include "perm.hpp"
+
+namespace gbsk {
+
+// Per-lane Gray range [start, start + 2^M): initial row sums at gray(start),
+// then 2^M - 1 single-column updates. Returns the signed, unscaled partial.
+// Shared by the fp64 plain and kappa modes (identical value arithmetic).
+template <int N, int HOT, bool KAPPA>
+__device__ inline void glynn_range_fp64(const cplx* __restrict__ AT,  // LDS, AT[c*N + r]
+                                        uint32_t start, int M, cplx& acc_out, double& abs_out) {
+  cplx rs[N];
+  const uint32_t g = start ^ (start >> 1);
+#pragma unroll
+  for (int r = 0; r < N; ++r) rs[r] = AT[r];  // column 0, delta_0 = +1
+#pragma unroll 1
+  for (int c = 1; c < N; ++c) {
+    const double s = ((g >> (c - 1)) & 1u) ? -1.0 : 1.0;
+#pragma unroll
+    for (int r = 0; r < N; ++r) rs[r] = c_axpy(s, AT[c * N + r], rs[r]);
+  }
+  // hot columns: 1..HOT
+  cplx hot[HOT > 0 ? HOT : 1][N];
+#pragma unroll
+  for (int h = 0; h < HOT; ++h)
+#pragma unroll
+    for (int r = 0; r < N; ++r) hot[h][r] = AT[(h + 1) * N + r];
+
+  auto product = [&]() {
+    // four interleaved chains (rows r = q mod 4), then (t0 t1)(t2 t3): fixed order
+    cplx t[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) t[q] = q < N ? rs[q] : cplx{1.0, 0.0};
+#pragma unroll
+    for (int r = 4; r < N; ++r) t[r & 3] = c_mul(t[r & 3], rs[r]);
+    if constexpr (N == 1) return t[0];
+    if constexpr (N == 2) return c_mul(t[0], t[1]);
+    if constexpr (N == 3) return c_mul(c_mul(t[0], t[1]), t[2]);
+    return c_mul(c_mul(t[0], t[1]), c_mul(t[2], t[3]));
+  };
+
+  cplx acc = product();  // term j = 0: start is even, sign +
+  double aacc = KAPPA ? c_abs(acc) : 0.0;
+  const uint32_t nsteps = 1u << M;
+  constexpr int U = HOT;  // static block length 2^U: steps t = 1..2^U-1 use hot columns
+  if constexpr (U == 0) {
+    for (uint32_t j = 1; j < nsteps; ++j) {
+      const int k = __builtin_ctz(j);
+      const uint32_t i = start + j;
+      const double step = ((i >> (k + 1)) & 1u) ? 2.0 : -2.0;
+      const cplx* col = AT + (k + 1) * N;
+#pragma unroll
+      for (int r = 0; r < N; ++r) rs[r] = c_axpy(step, col[r], rs[r]);
+      cplx p = product();
+      acc = (j & 1u) ? c_sub(acc, p) : c_add(acc, p);
+      if (KAPPA) aacc += c_abs(p);
+    }
+  } else {
+    const uint32_t nblocks = nsteps >> U;
+    for (uint32_t jb = 0; jb < nblocks; ++jb) {
+#pragma unroll
+      for (int t = 1; t <= (1 << U); ++t) {
+        const uint32_t j = (jb << U) + (uint32_t)t;
+        if (t == (1 << U) && jb + 1 == nblocks) break;  // j == 2^M: outside the range
+        const uint32_t i = start + j;
+        int k;
+        if (t < (1 << U)) {
+          k = __builtin_ctz((unsigned)t);
+        } else {
+          k = __builtin_ctz(j);
+        }
+        const double step = ((i >> (k + 1)) & 1u) ? 2.0 : -2.0;
+        if (t < (1 << U)) {
+#pragma unroll
+          for (int r = 0; r < N; ++r) rs[r] = c_axpy(step, hot[__builtin_ctz((unsigned)t)][r], rs[r]);
+        } else {
+          const cplx* col = AT + (k + 1) * N;
+#pragma unroll
+          for (int r = 0; r < N; ++r) rs[r] = c_axpy(step, col[r], rs[r]);
+        }
+        cplx p = product();
+        acc = (t & 1) ? c_sub(acc, p) : c_add(acc, p);
+        if (KAPPA) aacc += c_abs(p);
+      }
+    }
+  }
+  acc_out = acc;
+  abs_out = aacc;
+}
+
+// Same walk in complex double-double (reference: core/permanent_dd.cu:34-73,
+// cpu_ref/permanent.py:142-189). The +-2 A entries are exact in binary64, so a
+// row-sum update is one dd + double add per component.
+template <int N>
+__device__ inline void glynn_range_dd(const cplx* __restrict__ AT, uint32_t start, int M, ddc& acc_out) {
+  ddc rs[N];
+  const uint32_t g = start ^ (start >> 1);
+#pragma unroll
+  for (int r = 0; r < N; ++r) rs[r] = ddc_from(AT[r]);
+#pragma unroll 1
+  for (int c = 1; c < N; ++c) {
+    const double s = ((g >> (c - 1)) & 1u) ? -1.0 : 1.0;
+#pragma unroll
+    for (int r = 0; r < N; ++r) rs[r] = ddc_add_scaled(rs[r], s, AT[c * N + r]);
+  }
+  auto product = [&]() {
+    // two interleaved chains (even / odd rows), then their product
+    ddc t0 = rs[0];
+#pragma unroll
+    for (int r = 2; r < N; r += 2) t0 = ddc_mul(t0, rs[r]);
+    if constexpr (N == 1) return t0;
+    ddc t1 = rs[1];
+#pragma unroll
+    for (int r = 3; r < N; r += 2) t1 = ddc_mul(t1, rs[r]);
+    return ddc_mul(t0, t1);
+  };
+  ddc acc = product();
+  const uint32_t nsteps = 1u << M;
+  for (uint32_t j = 1; j < nsteps; ++j) {
+    const int k = __builtin_ctz(j);
+    const uint32_t i = start + j;
+    const double step = ((i >> (k + 1)) & 1u) ? 2.0 : -2.0;
+    const cplx* col = AT + (k + 1) * N;
+#pragma unroll
+    for (int r = 0; r < N; ++r) rs[r] = ddc_add_scaled(rs[r], step, col[r]);
+    ddc p = product();
+    acc = (j & 1u) ? ddc_sub(acc, p) : ddc_add(acc, p);
+  }
+  acc_out = acc;
+}
+
+// One workgroup = one wave = MPW matrices x LPM lanes (or one chunk of one
+// matrix when the matrix is split over W workgroups).
+//   out      : final values (W == 1) or chunk partials [B][W] (W > 1)
+//   aux      : kappa sum|term| (final or partial), may be null in plain mode
+template <int N, int MODE>
+__global__ __launch_bounds__(64) void perm_kernel(const cplx* __restrict__ mats, int B, cplx* __restrict__ out,
+                                                  double* __restrict__ aux) {
+  using C = PermCfg<N, MODE == PERM_DD ? 1 : 0>;
+  constexpr int LPM = C::LPM, MPW = 64 / LPM, W = C::W, M = C::M;
+  __shared__ cplx AT[MPW][N * N];
+  const int lane = threadIdx.x;
+  const int gidx = lane / LPM, sub = lane % LPM;
+  // workgroup -> (first matrix, chunk)
+  const int64_t wg = blockIdx.x;
+  const int64_t mat0 = (wg / W) * MPW;
+  const int chunk = (int)(wg % W);
+  // stage MPW matrices transposed into LDS
+  for (int e = lane; e < MPW * N * N; e += 64) {
+    const int q = e / (N * N), rc = e % (N * N);
+    const int r = rc / N, c = rc % N;
+    const int64_t b = mat0 + q;
+    AT[q][c * N + r] = (b < B) ? mats[b * (N * N) + rc] : cplx{0.0, 0.0};
+  }
+  __syncthreads();
+  const int64_t b = mat0 + gidx;
+  const uint32_t start = ((uint32_t)chunk * LPM + (uint32_t)sub) << M;
+  if constexpr (MODE == PERM_DD) {
+    ddc acc;
+    glynn_range_dd<N>(AT[gidx], start, M, acc);
+    acc = group_sum<LPM>(acc);
+    if (sub == 0 && b < B) {
+      if constexpr (W == 1) {
+        out[b] = ddc_to_c(ddc_mul_d(acc, 1.0 / (double)(1ull << (N - 1))));
+      } else {
+        // chunk partials stay double-double: hi in out, lo in aux-as-cplx
+        out[b * W + chunk] = cplx{acc.re.hi, acc.im.hi};
+        reinterpret_cast<cplx*>(aux)[b * W + chunk] = cplx{acc.re.lo, acc.im.lo};
+      }
+    }
+  } else {
+    constexpr bool KAPPA = (MODE == PERM_KAPPA);
+    cplx acc;
+    double aacc;
+    glynn_range_fp64<N, C::HOT, KAPPA>(AT[gidx], start, M, acc, aacc);
+    acc = group_sum<LPM>(acc);
+    if (KAPPA) aacc = group_sum<LPM>(aacc);
+    if (sub == 0 && b < B) {
+      if constexpr (W == 1) {
+        const double scale = 1.0 / (double)(1ull << (N - 1));
+        out[b] = c_scale(acc, scale);
+        if (KAPPA) aux[b] = aacc * scale;
+      } else {
+        out[b * W + chunk] = acc;
+        if (KAPPA) aux[b * W + chunk] = aacc;
+      }
+    }
+  }
+}
+
+// Fixed-order (chunk 0, 1, ..., W-1) reduction of chunk partials.
+template <int N, int MODE>
+__global__ void perm_reduce_kernel(const cplx* __restrict__ part, const double* __restrict__ part_aux, int B,
+                                   cplx* __restrict__ out, double* __restrict__ aux) {
+  using C = PermCfg<N, MODE == PERM_DD ? 1 : 0>;
+  constexpr int W = C::W;
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const double scale = 1.0 / (double)(1ull << (N - 1));
+  if constexpr (MODE == PERM_DD) {
+    const cplx* lo = reinterpret_cast<const cplx*>(part_aux);
+    ddc s = ddc_zero();
+    for (int w = 0; w < W; ++w) {
+      cplx h = part[b * W + w], l = lo[b * W + w];
+      s = ddc_add(s, ddc{dd{h.re, l.re}, dd{h.im, l.im}});
+    }
+    out[b] = ddc_to_c(ddc_mul_d(s, scale));
+  } else {
+    cplx s{0.0, 0.0};
+    double a = 0.0;
+    for (int w = 0; w < W; ++w) {
+      s = c_add(s, part[b * W + w]);
+      if (MODE == PERM_KAPPA) a += part_aux[b * W + w];
+    }
+    out[b] = c_scale(s, scale);
+    if (MODE == PERM_KAPPA) aux[b] = a * scale;
+  }
+}
+
+template <int N, int MODE>
+static int launch_perm_n(const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
+  using C = PermCfg<N, MODE == PERM_DD ? 1 : 0>;
+  constexpr int MPW = 64 / C::LPM, W = C::W;
+  const int64_t groups = ((int64_t)B + MPW - 1) / MPW;
+  const int64_t grid = groups * W;
+  if (W == 1) {
+    hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, d_in, B, d_out, d_aux);
+  } else {
+    cplx* part = reinterpret_cast<cplx*>(ws);
+    double* part_aux = reinterpret_cast<double*>(part + (size_t)B * W);
+    hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, d_in, B, part, part_aux);
+    hipLaunchKernelGGL((perm_reduce_kernel<N, MODE>), dim3((unsigned)((B + 255) / 256)), dim3(256), 0, st, part,
+                       part_aux, B, d_out, d_aux);
+  }
+  return (int)hipGetLastError();
+}
+
+// n == 0: perm = 1 (empty product); kappa indicator 1
+__global__ void perm_n0_kernel(int B, cplx* out, double* aux) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  out[b] = cplx{1.0, 0.0};
+  if (aux) aux[b] = 1.0;
+}
+
+template <int MODE, int... Ns>
+static int dispatch_perm(int n, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st,
+                         std::integer_sequence<int, Ns...>) {
+  int rc = GBSK_ERR_DIM;
+  ((n == Ns ? (rc = launch_perm_n<Ns, MODE>(d_in, B, d_out, d_aux, ws, st), 0) : 0), ...);
+  return rc;
+}
+
+template <int... Ns>
+static size_t ws_perm(int n, int tier, int B, std::integer_sequence<int, Ns...>) {
+  size_t r = 0;
+  ((n == Ns ? (r = (tier == PERM_DD ? PermCfg<Ns, 1>::W : PermCfg<Ns, 0>::W) > 1
+                       ? (size_t)B * (tier == PERM_DD ? PermCfg<Ns, 1>::W : PermCfg<Ns, 0>::W) * (16 + 16)
+                       : 0,
+               0)
+            : 0),
+   ...);
+  return r;
+}
+
+template <int... Is>
+constexpr std::integer_sequence<int, (Is + 1)...> shift1(std::integer_sequence<int, Is...>) {
+  return {};
+}
+using PermSizes = decltype(shift1(std::make_integer_sequence<int, PERM_MAX_N>{}));
+
+size_t perm_workspace_bytes(int n, int mode, int B) {
+  if (n < 1 || n > PERM_MAX_N || B <= 0) return 0;
+  return ws_perm(n, mode, B, PermSizes{});
+}
+
+int perm_launch(int n, int mode, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
+  if (B <= 0) return 0;
+  if (n < 0 || n > PERM_MAX_N) return GBSK_ERR_DIM;
+  if (n == 0) {
+    hipLaunchKernelGGL(perm_n0_kernel, dim3((B + 255) / 256), dim3(256), 0, st, B, d_out,
+                       mode == PERM_KAPPA ? d_aux : nullptr);
+    return (int)hipGetLastError();
+  }
+  switch (mode) {
+    case PERM_FP64: return dispatch_perm<PERM_FP64>(n, d_in, B, d_out, d_aux, ws, st, PermSizes{});
+    case PERM_KAPPA: return dispatch_perm<PERM_KAPPA>(n, d_in, B, d_out, d_aux, ws, st, PermSizes{});
+    case PERM_DD: return dispatch_perm<PERM_DD>(n, d_in, B, d_out, d_aux, ws, st, PermSizes{});
+  }
+  return GBSK_ERR_ARG;
+}
+
+}  // namespace gbsk

@@ -1,0 +1,179 @@
+"""``gbskernels_ext`` -- drop-in replacement of the reference's compiled module.
+
+The reference's Python package imports a module named ``gbskernels_ext`` and
+calls its attributes by name (reference ``gbskernels/__init__.py:109-136,224``).
+This module provides the same attribute set (reference
+``bindings/gbskernels_ext.cpp:231-391``) over the C ABI of ``libgbsk.so``:
+put this directory on ``sys.path`` (or point ``GBSKERNELS_EXT_DIR`` at it) and
+the reference package runs on the MI355X kernels unchanged.
+
+Marshalling contract (reference ``bindings/gbskernels_ext.cpp:96-141``): input is
+a ``(B, d, d)`` complex128 stack (coerced to C-contiguous complex128 here); the
+kernel dimension is ``d`` for perm/haf/lhaf and ``d // 2`` for tor; a non-square
+stack or an odd torontonian dimension raises ``ValueError``; a nonzero return
+code raises ``RuntimeError``; results are fresh numpy arrays.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+_pkg_root = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+if _pkg_root not in sys.path:
+    sys.path.insert(0, _pkg_root)
+
+from gbskernels_amd import _lib  # noqa: E402
+
+__host_shim__ = False  # a real device build (reference: gpu_backend_kind)
+
+
+def _stack(a, tor: bool) -> np.ndarray:
+    s = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+    if s.ndim != 3 or s.shape[1] != s.shape[2]:
+        raise ValueError("gbskernels: each matrix must be square (B, d, d)")
+    if tor and s.shape[1] % 2 != 0:
+        raise ValueError("torontonian: matrix dimension must be even (2n x 2n)")
+    return s
+
+
+def _eval(name: str, a) -> np.ndarray:
+    tor = name.startswith("tor")
+    s = _stack(a, tor)
+    B, d = s.shape[0], s.shape[1]
+    out = np.empty(max(B, 1), np.complex128)[:B]
+    fn = getattr(_lib.lib(), f"gbs_{name}_host")
+    _lib.check(fn(_lib.ptr(s), d // 2 if tor else d, B, _lib.ptr(out)), name)
+    return out
+
+
+def _eval_aux(name: str, a):
+    tor = name.startswith("tor")
+    s = _stack(a, tor)
+    B, d = s.shape[0], s.shape[1]
+    out = np.empty(max(B, 1), np.complex128)[:B]
+    aux = np.empty(max(B, 1), np.float64)[:B]
+    fn = getattr(_lib.lib(), f"gbs_{name}_host")
+    _lib.check(fn(_lib.ptr(s), d // 2 if tor else d, B, _lib.ptr(out), _lib.ptr(aux)), name)
+    return out, aux
+
+
+def perm(a): return _eval("perm", a)
+def perm_dd(a): return _eval("perm_dd", a)
+def haf(a): return _eval("haf", a)
+def haf_dd(a): return _eval("haf_dd", a)
+def lhaf(a): return _eval("lhaf", a)
+def lhaf_dd(a): return _eval("lhaf_dd", a)
+def tor(a): return _eval("tor", a)
+def tor_dd(a): return _eval("tor_dd", a)
+
+def perm_kappa(a): return _eval_aux("perm_kappa", a)
+def haf_kappa(a): return _eval_aux("haf_kappa", a)
+def lhaf_kappa(a): return _eval_aux("lhaf_kappa", a)
+def tor_kappa(a): return _eval_aux("tor_kappa", a)
+
+def perm_certified(a): return _eval_aux("perm_certified", a)
+def haf_certified(a): return _eval_aux("haf_certified", a)
+def lhaf_certified(a): return _eval_aux("lhaf_certified", a)
+def tor_certified(a): return _eval_aux("tor_certified", a)
+def perm_dd_certified(a): return _eval_aux("perm_dd_certified", a)
+def haf_dd_certified(a): return _eval_aux("haf_dd_certified", a)
+def lhaf_dd_certified(a): return _eval_aux("lhaf_dd_certified", a)
+def tor_dd_certified(a): return _eval_aux("tor_dd_certified", a)
+
+
+class DeviceResult:
+    """A (B,) complex128 result left in device memory (reference
+    ``bindings/gbskernels_ext.cpp:209-228``). Owns the buffer; ``numpy()`` copies
+    it to the host; ``ptr`` is the device address for zero-copy consumers."""
+
+    def __init__(self, ptr: int, n: int):
+        self.ptr = ptr
+        self.shape = (n,)
+        self.dtype = np.dtype(np.complex128)
+
+    def __len__(self) -> int:
+        return self.shape[0]
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(max(self.shape[0], 1), np.complex128)[: self.shape[0]]
+        if self.shape[0]:
+            _lib.check(_lib.lib().gbs_dev_to_host(self.ptr, _lib.ptr(out), self.shape[0]), "dev_to_host")
+        return out
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.numpy()
+        return a if dtype is None else a.astype(dtype)
+
+    def __del__(self):
+        p, self.ptr = getattr(self, "ptr", None), None
+        if p:
+            try:
+                _lib.lib().gbs_dev_free(p)
+            except Exception:
+                pass
+
+
+_SESSION_FUNCS = {"perm": 0, "haf": 1, "lhaf": 2, "tor": 3}
+
+
+class Session:
+    """Device-resident session: grow-only buffers and one HIP stream reused by
+    every call (reference ``core/host_api.cu:474-606``)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().gbs_session_open(ctypes.byref(h)), "session_open")
+        self._h = h
+
+    def _call(self, name: str, a):
+        if self._h is None:
+            raise RuntimeError("gbskernels: session is closed")
+        func, _, prec = name.partition("_")
+        s = _stack(a, func == "tor")
+        B, d = s.shape[0], s.shape[1]
+        out = np.empty(max(B, 1), np.complex128)[:B]
+        _lib.check(_lib.lib().gbs_session_evaluate(self._h, _SESSION_FUNCS[func], 1 if prec == "dd" else 0,
+                                                   _lib.ptr(s), d, B, _lib.ptr(out)), f"Session.{name}")
+        return out
+
+    def _resident(self, name: str, a) -> DeviceResult:
+        if self._h is None:
+            raise RuntimeError("gbskernels: session is closed")
+        func, _, prec = name.partition("_")
+        s = _stack(a, func == "tor")
+        B, d = s.shape[0], s.shape[1]
+        p = ctypes.c_void_p()
+        _lib.check(_lib.lib().gbs_session_evaluate_resident(self._h, _SESSION_FUNCS[func], 1 if prec == "dd" else 0,
+                                                            _lib.ptr(s), d, B, ctypes.byref(p)), f"Session.{name}_resident")
+        return DeviceResult(p.value or 0, B)
+
+    def perm(self, a): return self._call("perm", a)
+    def perm_dd(self, a): return self._call("perm_dd", a)
+    def haf(self, a): return self._call("haf", a)
+    def haf_dd(self, a): return self._call("haf_dd", a)
+    def lhaf(self, a): return self._call("lhaf", a)
+    def lhaf_dd(self, a): return self._call("lhaf_dd", a)
+    def tor(self, a): return self._call("tor", a)
+    def tor_dd(self, a): return self._call("tor_dd", a)
+    def perm_resident(self, a): return self._resident("perm", a)
+    def haf_resident(self, a): return self._resident("haf", a)
+    def lhaf_resident(self, a): return self._resident("lhaf", a)
+    def tor_resident(self, a): return self._resident("tor", a)
+
+    def reallocs(self) -> int:
+        return int(_lib.lib().gbs_session_reallocs(self._h)) if self._h is not None else 0
+
+    def close(self) -> None:
+        if self._h is not None:
+            _lib.lib().gbs_session_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

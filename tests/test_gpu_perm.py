@@ -1,0 +1,121 @@
+"""GPU parity of the Glynn permanent kernels (fp64, kappa, dd) against the oracle,
+closed forms and the reference's published checksums. All calls go through the
+C ABI (via the gbskernels_ext drop-in module)."""
+import math
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import rel_err, rng_complex
+
+sys.path.insert(0, str(Path(__file__).resolve().parent.parent / "gbskernels_amd" / "ext"))
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ext():
+    import gbskernels_ext
+    return gbskernels_ext
+
+
+@pytest.mark.parametrize("n", list(range(1, 17)))
+def test_perm_fp64_vs_oracle(ext, oracle, n):
+    print(f"perm fp64 n={n}", flush=True)
+    A = rng_complex((37, n, n), 100 + n)
+    got, want = ext.perm(A), oracle.perm(A)
+    assert rel_err(got, want).max() < 1e-12
+
+
+@pytest.mark.parametrize("n", [18, 20, 22])
+def test_perm_fp64_large_vs_quad(ext, oracle, n):
+    print(f"perm fp64 large n={n}", flush=True)
+    A = rng_complex((3, n, n), 200 + n, 0.5)
+    got = ext.perm(A)
+    assert rel_err(got, oracle.perm_dd(A)).max() < 1e-11
+
+
+def test_perm_zero_dim_and_empty_batch(ext):
+    assert ext.perm(np.zeros((3, 0, 0)))[0] == 1.0
+    assert ext.perm(np.zeros((0, 5, 5))).shape == (0,)
+
+
+@pytest.mark.parametrize("n", [1, 5, 9, 13, 16, 20])
+def test_perm_closed_forms(ext, n):
+    print(f"perm closed forms n={n}", flush=True)
+    J = np.ones((1, n, n))
+    assert ext.perm(J)[0] == pytest.approx(math.factorial(n), rel=1e-12)
+    assert ext.perm(np.eye(n)[None])[0] == pytest.approx(1.0, abs=1e-12)
+
+
+def test_perm_known_small_values(ext):
+    assert ext.perm(np.array([[[1.0, 2.0], [3.0, 4.0]]]))[0] == pytest.approx(10.0)
+    A = np.array([[[1.0, 2.0, 3.0], [4.0, 5.0, 6.0], [7.0, 8.0, 10.0]]])
+    assert ext.perm(A)[0] == pytest.approx(463.0)
+
+
+@pytest.mark.parametrize("n", [4, 8, 12, 16])
+def test_perm_kappa_value_bitwise_equals_plain(ext, oracle, n):
+    print(f"perm kappa n={n}", flush=True)
+    A = rng_complex((50, n, n), 300 + n)
+    v, a = ext.perm_kappa(A)
+    assert np.array_equal(v, ext.perm(A))
+    assert rel_err(a, oracle.perm_abs(A)).max() < 1e-10
+
+
+@pytest.mark.parametrize("n", [3, 8, 12, 16])
+def test_perm_dd_vs_oracle(ext, oracle, n):
+    print(f"perm dd n={n}", flush=True)
+    A = rng_complex((20, n, n), 400 + n)
+    assert rel_err(ext.perm_dd(A), oracle.perm_q(A)).max() < 1e-15
+
+
+def test_perm_dd_survives_cancellation(ext, oracle):
+    # rank-deficient-ish: two nearly equal rows -> heavy cancellation in Glynn
+    g = np.random.default_rng(9)
+    R = g.standard_normal((10, 10)) + 1j * g.standard_normal((10, 10))
+    A = R.copy()
+    A[1] = A[0] + 1e-9 * R[1]
+    A[:, 1] = -A[:, 0] + 1e-9 * R[:, 1]
+    want = oracle.perm_q(A)[0]
+    dd = ext.perm_dd(A[None])[0]
+    fp = ext.perm(A[None])[0]
+    assert abs(dd - want) / abs(want) < 1e-14
+    assert abs(fp - want) / abs(want) > 1e3 * abs(dd - want) / abs(want)
+
+
+def test_batch_composition_independence(ext):
+    n = 14
+    A = rng_complex((64, n, n), 500)
+    full = ext.perm(A)
+    for idx in ([0], [5, 3], list(range(63, -1, -2))):
+        assert np.array_equal(ext.perm(A[idx]), full[idx])
+    n = 22  # chunked over workgroups
+    A = rng_complex((5, n, n), 501, 0.5)
+    full = ext.perm(A)
+    assert np.array_equal(ext.perm(A[[3]]), full[[3]])
+
+
+def test_reference_published_checksums(ext, oracle, ref_checksums):
+    for row in ref_checksums:
+        if row["func"] not in ("perm", "perm_dd"):
+            continue
+        dim, B = row["matrix_dim"], row["batch"]
+        print(f"checksum {row['func']} dim={dim}", flush=True)
+        a = oracle.gen_bench_complex(dim, B, False, 1.0)
+        got = float(np.sum(np.abs(getattr(ext, row["func"])(a))))
+        assert got == pytest.approx(row["checksum"], rel=1e-12), (row["func"], dim)
+
+
+def test_session_and_resident(ext, oracle):
+    s = ext.Session()
+    A = rng_complex((40, 10, 10), 600)
+    want = ext.perm(A)
+    assert np.array_equal(s.perm(A), want)
+    r0 = s.reallocs()
+    assert np.array_equal(s.perm(A[:7]), want[:7])
+    assert s.reallocs() == r0  # smaller bucket reuses the buffer
+    res = s.perm_resident(A)
+    assert np.array_equal(res.numpy(), want)
+    s.close()
