@@ -1,10 +1,61 @@
-#include "kernels.hpp"
+// haf.hip -- dispatch of the power-trace hafnian / loop hafnian launchers
+// (kernels in haf_impl.hpp, instantiated in haf_*.hip).
+#include "haf_impl.hpp"
+
 namespace gbsk {
-int perm_cert_max_dim(int) { return 0; }
-size_t haf_workspace_bytes(bool, int, int, int) { return 0; }
-int haf_launch(bool, int, int, const cplx*, int, cplx*, double*, void*, hipStream_t) { return GBSK_ERR_FUNC; }
-int haf_max_dim(bool, int) { return 0; }
-size_t tor_workspace_bytes(int, int, int) { return 0; }
-int tor_launch(int, int, const cplx*, int, cplx*, double*, void*, hipStream_t) { return GBSK_ERR_FUNC; }
-int tor_max_dim(int) { return 0; }
+
+// trivial sizes: N == 0 -> 1; odd N -> 0 (haf) or NaN (lhaf, augmented upstream)
+__global__ void haf_trivial_kernel(int N, bool loop, int B, cplx* out, double* aux) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  if (N == 0) {
+    out[b] = cplx{1.0, 0.0};
+    if (aux) aux[b] = 1.0;
+  } else if (!loop) {
+    out[b] = cplx{0.0, 0.0};
+    if (aux) aux[b] = 0.0;
+  } else {
+    out[b] = cplx{NAN, NAN};
+    if (aux) aux[b] = NAN;
+  }
 }
+
+size_t haf_workspace_bytes(bool, int tier, int N, int B) {
+  if (N < 2 || (N & 1) || N > HAF_MAX_N || B <= 0) return 0;
+  const size_t n = N / 2;
+  const size_t elem = (tier == GBSK_DD) ? sizeof(ddc) : sizeof(cplx);
+  return (size_t)B * n * (elem + sizeof(double));
+}
+
+int haf_max_dim(bool, int tier) {
+  switch (tier) {
+    case GBSK_FP64:
+    case GBSK_DD:
+    case GBSK_KAPPA: return HAF_MAX_N;
+  }
+  return 0;
+}
+
+int haf_launch(bool loop, int tier, int N, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws,
+               hipStream_t st) {
+  if (B <= 0) return 0;
+  if (N < 0 || N > HAF_MAX_N) return GBSK_ERR_DIM;
+  if (tier != GBSK_FP64 && tier != GBSK_DD && tier != GBSK_KAPPA) return GBSK_ERR_FUNC;
+  if (N == 0 || (N & 1)) {
+    hipLaunchKernelGGL(haf_trivial_kernel, dim3((B + 255) / 256), dim3(256), 0, st, N, loop, B, d_out,
+                       tier == GBSK_KAPPA ? d_aux : nullptr);
+    return (int)hipGetLastError();
+  }
+  if (!ws) return GBSK_ERR_WORKSPACE;
+  if (tier == GBSK_DD) return loop ? run_haf<ArDD, true, false>(N, d_in, B, d_out, nullptr, ws, st)
+                                   : run_haf<ArDD, false, false>(N, d_in, B, d_out, nullptr, ws, st);
+  if (tier == GBSK_KAPPA) return loop ? run_haf<ArF64, true, true>(N, d_in, B, d_out, d_aux, ws, st)
+                                      : run_haf<ArF64, false, true>(N, d_in, B, d_out, d_aux, ws, st);
+  return loop ? run_haf<ArF64, true, false>(N, d_in, B, d_out, nullptr, ws, st)
+              : run_haf<ArF64, false, false>(N, d_in, B, d_out, nullptr, ws, st);
+}
+
+// placeholders of the families not in this file yet
+int perm_cert_max_dim(int) { return 0; }
+
+}  // namespace gbsk

@@ -1,0 +1,331 @@
+// haf_impl.hpp -- power-trace hafnian and loop hafnian on gfx950 (fp64, kappa, dd).
+//
+// Algorithm (reference: core/hafnian.cu:41-133, core/loop_hafnian.cu:45-139,
+// cpu_ref/hafnian.py:60-129, cpu_ref/loop_hafnian.py:65-108):
+//   haf(A) = sum_{S subset of the n index pairs} (-1)^(n-|S|) [x^n] exp(sum_k g_k x^k)
+// with C = B_S X (B_S the principal submatrix on the pairs of S, X the swap
+// inside each pair), k g_k = tr(C^k)/2 (+ k v_k, v_k = 1/2 d^T X C^(k-1) d for
+// the loop hafnian, d = diag B_S), and the coefficient from the Newton recurrence
+// e_j = (1/j) sum_{k<=j} (k g_k) e_{j-k}. The ordinary hafnian ignores the
+// diagonal, which is zeroed on load (cpu_ref/hafnian.py:115).
+//
+// MI355X design (this file's own):
+//  * one kernel per (subset size s = 2m, tier): all subsets of one size form a
+//    phase; a wave holds one instance's matrix in LDS and walks its C(n, m)
+//    subsets, G = floor(64 / s) at a time, one lane group of s lanes per subset,
+//    lane r owning row r of the current power C^j in VGPRs;
+//  * powers only up to C^h, h = ceil(n/2): since B_S is symmetric,
+//    (C^j)^T = X C^j X, so tr(C^i C^j) = sum_ab (C^j)_ab (C^i)_(a^1)(b^1) pairs
+//    lane a with lane a^1 (a DPP swap) and gives p_{2j-1} and p_{2j} from the
+//    rows of C^(j-1) and C^j: h-1 matrix products instead of n-1;
+//  * the loop term uses the Krylov vectors w_k = C^(k-1) d (one matvec each);
+//  * subset terms are summed per lane group in subset order, groups and phases
+//    in a fixed order by a final pass: the value depends on N alone.
+#pragma once
+
+#include <utility>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace gbsk {
+
+constexpr int HAF_MAX_N = 28;
+constexpr int HAF_MAX_PAIRS = HAF_MAX_N / 2;
+
+// ------------------------------------------------------------ arithmetic policies
+struct ArF64 {
+  using T = cplx;
+  __device__ static T zero() { return cplx{0.0, 0.0}; }
+  __device__ static T one() { return cplx{1.0, 0.0}; }
+  __device__ static T from(cplx z) { return z; }
+  __device__ static T add(T a, T b) { return c_add(a, b); }
+  __device__ static T sub(T a, T b) { return c_sub(a, b); }
+  __device__ static T neg(T a) { return c_neg(a); }
+  __device__ static T mul(T a, T b) { return c_mul(a, b); }
+  // acc + a * z, z a plain complex input entry
+  __device__ static T fma_c(T a, cplx z, T acc) { return c_fma(a, z, acc); }
+  __device__ static T fma(T a, T b, T acc) { return c_fma(a, b, acc); }
+  __device__ static T scale(T a, double s) { return c_scale(a, s); }
+  __device__ static T div_int(T a, int j) { return cplx{a.re / j, a.im / j}; }
+  __device__ static cplx to_c(T a) { return a; }
+  __device__ static T shfl_xor(T a, int m) { return cplx{shfl_xor_d(a.re, m), shfl_xor_d(a.im, m)}; }
+};
+
+__device__ inline dd dd_mul_d_acc(dd a, double b, dd acc) { return dd_add(acc, dd_mul_d(a, b)); }
+
+struct ArDD {
+  using T = ddc;
+  __device__ static T zero() { return ddc_zero(); }
+  __device__ static T one() { return ddc{dd{1.0, 0.0}, dd{0.0, 0.0}}; }
+  __device__ static T from(cplx z) { return ddc_from(z); }
+  __device__ static T add(T a, T b) { return ddc_add(a, b); }
+  __device__ static T sub(T a, T b) { return ddc_sub(a, b); }
+  __device__ static T neg(T a) { return ddc_neg(a); }
+  __device__ static T mul(T a, T b) { return ddc_mul(a, b); }
+  __device__ static T fma_c(T a, cplx z, T acc) {
+    dd re = dd_sub(dd_mul_d(a.re, z.re), dd_mul_d(a.im, z.im));
+    dd im = dd_add(dd_mul_d(a.re, z.im), dd_mul_d(a.im, z.re));
+    return ddc{dd_add(acc.re, re), dd_add(acc.im, im)};
+  }
+  __device__ static T fma(T a, T b, T acc) { return ddc_add(acc, ddc_mul(a, b)); }
+  __device__ static T scale(T a, double s) { return ddc_mul_d(a, s); }
+  __device__ static T div_int(T a, int j) { return ddc_div_d(a, (double)j); }
+  __device__ static cplx to_c(T a) { return ddc_to_c(a); }
+  __device__ static T shfl_xor(T a, int m) {
+    return ddc{dd{shfl_xor_d(a.re.hi, m), shfl_xor_d(a.re.lo, m)}, dd{shfl_xor_d(a.im.hi, m), shfl_xor_d(a.im.lo, m)}};
+  }
+};
+
+// binomial coefficient, n <= 14
+__device__ inline int binom(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  int r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+
+// q-th (lexicographic) m-subset of {0..n-1} -> pair list p[0..m-1]
+template <int M>
+__device__ inline void unrank(int q, int n, int (&p)[M]) {
+  int x = 0;
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    for (;;) {
+      const int c = binom(n - 1 - x, M - 1 - i);
+      if (c <= q) {
+        q -= c;
+        ++x;
+      } else {
+        break;
+      }
+    }
+    p[i] = x++;
+  }
+}
+
+// LDS reduction over the S lanes of each group, fixed order; every lane of the
+// group receives the same sum. All 64 lanes must call it (barriers).
+template <class Ar, int S>
+__device__ inline typename Ar::T group_reduce(typename Ar::T v, typename Ar::T* red) {
+  const int lane = threadIdx.x;
+  red[lane] = v;
+  __syncthreads();
+  const int base = (lane / S < 64 / S) ? (lane / S) * S : 0;
+  typename Ar::T s = red[base];
+#pragma unroll
+  for (int i = 1; i < S; ++i) s = Ar::add(s, red[base + i]);
+  __syncthreads();
+  return s;
+}
+
+// ---------------------------------------------------------------- phase kernel
+// One workgroup (one wave) = one instance; evaluates every subset of size
+// m = S/2 and writes the phase partial (and sum |term| in kappa mode).
+//   part[b * n + (m - 1)]      phase partial (T collapsed per tier below)
+template <class Ar, bool LOOP, int S, bool KAPPA>
+__global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ mats, int N, int B,
+                                                       typename Ar::T* __restrict__ part,
+                                                       double* __restrict__ part_abs) {
+  using T = typename Ar::T;
+  constexpr int M = S / 2;
+  constexpr int G = 64 / S;
+  extern __shared__ cplx As[];  // N * N, dynamic
+  __shared__ T red[64];
+  __shared__ T wbuf[64];
+  __shared__ T kgs[G][HAF_MAX_PAIRS + 1];  // k g_k of each group's current subset
+  __shared__ T es[G][HAF_MAX_PAIRS + 1];   // Newton coefficients e_0..e_n
+  const int lane = threadIdx.x;
+  const int n = N / 2;
+  const int64_t b = blockIdx.x;
+  // stage A (row-major N x N); the ordinary hafnian zeroes the diagonal
+  for (int e = lane; e < N * N; e += 64) {
+    cplx v = mats[b * N * N + e];
+    if (!LOOP && (e / N) == (e % N)) v = cplx{0.0, 0.0};
+    As[e] = v;
+  }
+  __syncthreads();
+
+  const int g = lane / S;
+  const int r = lane % S;
+  const bool lane_ok = g < G;
+  const int count = binom(n, M);
+  const int h = (n + 1) / 2;
+
+  T group_acc = Ar::zero();
+  double group_abs = 0.0;
+
+  for (int q0 = 0; q0 < count; q0 += G) {
+    const int q = q0 + (lane_ok ? g : 0);  // groups past G mirror group 0 (discarded)
+    const bool active = lane_ok && q < count;
+    int p[M];
+    unrank<M>(active ? q : 0, n, p);
+    // lane r owns row r of C = B_S X: C[r][2u] = A[R][2p_u + 1], C[r][2u+1] = A[R][2p_u]
+    const int R = 2 * p[r >> 1] + (r & 1);
+    const int gs = lane_ok ? g : 0;
+    T P[S];
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+      P[2 * u] = Ar::from(As[R * N + 2 * p[u] + 1]);
+      P[2 * u + 1] = Ar::from(As[R * N + 2 * p[u]]);
+    }
+    // p_1 = tr C (C[r][r] = A[R][R^1]) and p_2 = tr(C C) through the partner row
+    {
+      T d1 = Ar::from(As[R * N + (R ^ 1)]);
+      T t2 = Ar::zero();
+#pragma unroll
+      for (int c = 0; c < S; ++c) t2 = Ar::fma(P[c], Ar::shfl_xor(P[c ^ 1], 1), t2);
+      const T s1 = group_reduce<Ar, S>(d1, red);
+      const T s2 = group_reduce<Ar, S>(t2, red);
+      if (r == 0 && lane_ok) {
+        kgs[g][1] = Ar::scale(s1, 0.5);
+        kgs[g][2] = Ar::scale(s2, 0.5);
+      }
+    }
+    // C^j for j = 2..h: one matrix product each, then p_{2j-1}, p_{2j}
+    for (int j = 2; j <= h; ++j) {
+      T acc[S];
+#pragma unroll
+      for (int c = 0; c < S; ++c) acc[c] = Ar::zero();
+#pragma unroll
+      for (int t = 0; t < S; ++t) {
+        const cplx* At = As + (2 * p[t >> 1] + (t & 1)) * N;
+#pragma unroll
+        for (int u = 0; u < M; ++u) {
+          const cplx a1 = At[2 * p[u] + 1], a0 = At[2 * p[u]];
+          acc[2 * u] = Ar::fma_c(P[t], a1, acc[2 * u]);
+          acc[2 * u + 1] = Ar::fma_c(P[t], a0, acc[2 * u + 1]);
+        }
+      }
+      T todd = Ar::zero(), teven = Ar::zero();
+#pragma unroll
+      for (int c = 0; c < S; ++c) {
+        todd = Ar::fma(acc[c], Ar::shfl_xor(P[c ^ 1], 1), todd);
+        teven = Ar::fma(acc[c], Ar::shfl_xor(acc[c ^ 1], 1), teven);
+      }
+      const T so = group_reduce<Ar, S>(todd, red);
+      const T se = group_reduce<Ar, S>(teven, red);
+      if (r == 0 && lane_ok) {
+        kgs[g][2 * j - 1] = Ar::scale(so, 0.5);
+        if (2 * j <= HAF_MAX_PAIRS) kgs[g][2 * j] = Ar::scale(se, 0.5);
+      }
+#pragma unroll
+      for (int c = 0; c < S; ++c) P[c] = acc[c];
+    }
+    if constexpr (LOOP) {
+      // w_1 = d, w_{k+1} = C w_k; v_k = 1/2 sum_a d_(a^1) w_k[a]; k g_k += k v_k
+      const cplx dpartner = As[(R ^ 1) * N + (R ^ 1)];
+      T w = Ar::from(As[R * N + R]);
+      for (int k = 1; k <= n; ++k) {
+        const T vk = group_reduce<Ar, S>(Ar::fma_c(w, dpartner, Ar::zero()), red);
+        if (r == 0 && lane_ok) kgs[g][k] = Ar::add(kgs[g][k], Ar::scale(vk, 0.5 * (double)k));
+        if (k < n) {
+          wbuf[lane] = w;
+          __syncthreads();
+          T nw = Ar::zero();
+#pragma unroll
+          for (int u = 0; u < M; ++u) {
+            nw = Ar::fma_c(wbuf[gs * S + 2 * u], As[R * N + 2 * p[u] + 1], nw);
+            nw = Ar::fma_c(wbuf[gs * S + 2 * u + 1], As[R * N + 2 * p[u]], nw);
+          }
+          __syncthreads();
+          w = nw;
+        }
+      }
+    }
+    __syncthreads();
+    // Newton: e_j = (1/j) sum_{k=1..j} kg_k e_{j-k}, by the first lane of each group
+    T coeff = Ar::zero();
+    if (r == 0 && lane_ok) {
+      T* e = es[g];
+      e[0] = Ar::one();
+      for (int jj = 1; jj <= n; ++jj) {
+        T a = Ar::zero();
+        for (int k = 1; k <= jj; ++k) a = Ar::fma(kgs[g][k], e[jj - k], a);
+        e[jj] = Ar::div_int(a, jj);
+      }
+      coeff = e[n];
+    }
+    __syncthreads();
+    if ((n - M) & 1) coeff = Ar::neg(coeff);
+    if (active && r == 0) {
+      group_acc = Ar::add(group_acc, coeff);
+      if (KAPPA) {
+        const cplx cc = Ar::to_c(coeff);
+        group_abs += c_abs(cc);
+      }
+    }
+  }
+  // combine the G group partials in group order
+  if (r == 0 && lane_ok) red[g] = group_acc;
+  __syncthreads();
+  if (lane == 0) {
+    T s = red[0];
+    for (int i = 1; i < G; ++i) s = Ar::add(s, red[i]);
+    part[b * n + (M - 1)] = s;
+  }
+  if (KAPPA) {
+    __syncthreads();
+    double* rabs = reinterpret_cast<double*>(red);
+    if (r == 0 && lane_ok) rabs[g] = group_abs;
+    __syncthreads();
+    if (lane == 0) {
+      double s = rabs[0];
+      for (int i = 1; i < G; ++i) s += rabs[i];
+      part_abs[b * n + (M - 1)] = s;
+    }
+  }
+}
+
+// out[b] = sum over phases m = 1..n in order (fixed association)
+template <class Ar, bool KAPPA>
+__global__ void haf_reduce_kernel(const typename Ar::T* __restrict__ part, const double* __restrict__ part_abs,
+                                  int n, int B, cplx* __restrict__ out, double* __restrict__ aux) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  typename Ar::T s = part[b * n];
+  double a = KAPPA ? part_abs[b * n] : 0.0;
+  for (int m = 1; m < n; ++m) {
+    s = Ar::add(s, part[b * n + m]);
+    if (KAPPA) a += part_abs[b * n + m];
+  }
+  out[b] = Ar::to_c(s);
+  if (KAPPA) aux[b] = a;
+}
+
+template <class Ar, bool LOOP, bool KAPPA, int S>
+static void launch_phase(const cplx* d_in, int N, int B, typename Ar::T* part, double* part_abs, hipStream_t st) {
+  hipLaunchKernelGGL((haf_phase_kernel<Ar, LOOP, S, KAPPA>), dim3((unsigned)B), dim3(64),
+                     (size_t)N * N * sizeof(cplx), st, d_in, N, B, part, part_abs);
+}
+
+template <class Ar, bool LOOP, bool KAPPA, int... Ss>
+static void launch_phases(const cplx* d_in, int N, int B, typename Ar::T* part, double* part_abs, hipStream_t st,
+                          std::integer_sequence<int, Ss...>) {
+  const int n = N / 2;
+  ((2 * (Ss + 1) <= 2 * n ? (launch_phase<Ar, LOOP, KAPPA, 2 * (Ss + 1)>(d_in, N, B, part, part_abs, st), 0) : 0),
+   ...);
+}
+
+template <class Ar, bool LOOP, bool KAPPA>
+int run_haf(int N, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
+  using T = typename Ar::T;
+  const int n = N / 2;
+  T* part = reinterpret_cast<T*>(ws);
+  double* part_abs = reinterpret_cast<double*>(part + (size_t)B * n);
+  launch_phases<Ar, LOOP, KAPPA>(d_in, N, B, part, part_abs, st, std::make_integer_sequence<int, HAF_MAX_PAIRS>{});
+  hipLaunchKernelGGL((haf_reduce_kernel<Ar, KAPPA>), dim3((B + 255) / 256), dim3(256), 0, st, part, part_abs, n, B,
+                     d_out, d_aux);
+  return (int)hipGetLastError();
+}
+
+// explicit instantiations live in haf_<tier>_<kind>.hip, one translation unit
+// each so that the (heavily unrolled) kernels compile in parallel
+extern template int run_haf<ArF64, false, false>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+extern template int run_haf<ArF64, true, false>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+extern template int run_haf<ArF64, false, true>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+extern template int run_haf<ArF64, true, true>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+extern template int run_haf<ArDD, false, false>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+extern template int run_haf<ArDD, true, false>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+
+}  // namespace gbsk
