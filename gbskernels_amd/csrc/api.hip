@@ -26,7 +26,7 @@ static_assert(sizeof(gbs_complex) == sizeof(cplx), "gbs_complex layout");
 // kernel dimension the launchers take: matrix side for perm/haf/lhaf, modes for tor
 static size_t ws_bytes(int func, int tier, int dim, int B) {
   switch (func) {
-    case GBSK_PERM: return perm_workspace_bytes(dim, tier == GBSK_DD ? PERM_DD : tier == GBSK_KAPPA ? PERM_KAPPA : PERM_FP64, B);
+    case GBSK_PERM: return perm_workspace_bytes(dim, tier == GBSK_DD ? PERM_DD : PERM_FP64, B);
     case GBSK_HAF:
     case GBSK_LHAF: return haf_workspace_bytes(func == GBSK_LHAF, tier, dim, B);
     case GBSK_TOR: return tor_workspace_bytes(tier, dim, B);
@@ -47,6 +47,9 @@ static int launch(int func, int tier, const cplx* d_in, int dim, int B, cplx* d_
         case GBSK_KAPPA:
           if (!d_aux) return GBSK_ERR_ARG;
           return perm_launch(dim, PERM_KAPPA, d_in, B, d_out, d_aux, ws, st);
+        case GBSK_CERTIFIED:
+          if (!d_aux) return GBSK_ERR_ARG;
+          return perm_launch(dim, PERM_CERT, d_in, B, d_out, d_aux, ws, st);
         default: return GBSK_ERR_FUNC;
       }
     case GBSK_HAF:
@@ -260,7 +263,7 @@ int gbs_eval_device(int func, int tier, const gbs_complex* d_in, int dim, int B,
 
 int gbs_max_dim(int func, int tier) {
   switch (func) {
-    case GBSK_PERM: return (tier == GBSK_FP64 || tier == GBSK_DD || tier == GBSK_KAPPA) ? PERM_MAX_N : perm_cert_max_dim(tier);
+    case GBSK_PERM: return (tier == GBSK_DD_CERTIFIED) ? 0 : PERM_MAX_N;
     case GBSK_HAF:
     case GBSK_LHAF: return haf_max_dim(func == GBSK_LHAF, tier);
     case GBSK_TOR: return tor_max_dim(tier);

@@ -55,7 +55,5 @@ int haf_launch(bool loop, int tier, int N, const cplx* d_in, int B, cplx* d_out,
               : run_haf<ArF64, false, false>(N, d_in, B, d_out, nullptr, ws, st);
 }
 
-// placeholders of the families not in this file yet
-int perm_cert_max_dim(int) { return 0; }
 
 }  // namespace gbsk

@@ -10,9 +10,6 @@
 
 namespace gbsk {
 
-// certified permanent tiers (not in this build yet: 0 = unsupported)
-int perm_cert_max_dim(int tier);
-
 // power-trace hafnian / loop hafnian; tier codes as in include/gbsk.h
 size_t haf_workspace_bytes(bool loop, int tier, int N, int B);
 int haf_launch(bool loop, int tier, int N, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws,

@@ -17,26 +17,45 @@
 // Lane partials are combined by a butterfly over the group's lanes, chunk
 // partials by a fixed-order second pass. The chunking depends only on n, so a
 // matrix's value never depends on what else is in the batch.
+#include "cert.hpp"
 #include "common.hpp"
 #include "perm.hpp"
 
 namespace gbsk {
 
+// rounding depth of the certificate arithmetic (cert.hpp): per lane 8 * 2^M + 16 N,
+// plus the butterfly and the chunk sum
+template <int N>
+__device__ constexpr double perm_cert_depth() {
+  using C = PermCfg<N, 0>;
+  return 8.0 * (double)(1u << C::M) + 16.0 * N + 4.0 * C::W + 64.0;
+}
+
 // Per-lane Gray range [start, start + 2^M): initial row sums at gray(start),
 // then 2^M - 1 single-column updates. Returns the signed, unscaled partial.
 // Shared by the fp64 plain and kappa modes (identical value arithmetic).
-template <int N, int HOT, bool KAPPA>
+// CERT: also carry a running bound (1-norm, round-to-nearest with a final
+// inflation, see cert.hpp) of |value - exact|: er[r] for the row sums, ep for a
+// product, E for the lane's partial.
+template <int N, int HOT, bool KAPPA, bool CERT>
 __device__ inline void glynn_range_fp64(const cplx* __restrict__ AT,  // LDS, AT[c*N + r]
                                         uint32_t start, int M, cplx& acc_out, double& abs_out) {
   cplx rs[N];
+  double er[N];
   const uint32_t g = start ^ (start >> 1);
 #pragma unroll
-  for (int r = 0; r < N; ++r) rs[r] = AT[r];  // column 0, delta_0 = +1
+  for (int r = 0; r < N; ++r) {
+    rs[r] = AT[r];  // column 0, delta_0 = +1 (exact)
+    er[r] = 0.0;
+  }
 #pragma unroll 1
   for (int c = 1; c < N; ++c) {
     const double s = ((g >> (c - 1)) & 1u) ? -1.0 : 1.0;
 #pragma unroll
-    for (int r = 0; r < N; ++r) rs[r] = c_axpy(s, AT[c * N + r], rs[r]);
+    for (int r = 0; r < N; ++r) {
+      rs[r] = c_axpy(s, AT[c * N + r], rs[r]);
+      if (CERT) er[r] = cert_round(rs[r], er[r]);
+    }
   }
   // hot columns: 1..HOT
   cplx hot[HOT > 0 ? HOT : 1][N];
@@ -45,21 +64,42 @@ __device__ inline void glynn_range_fp64(const cplx* __restrict__ AT,  // LDS, AT
 #pragma unroll
     for (int r = 0; r < N; ++r) hot[h][r] = AT[(h + 1) * N + r];
 
+  double ep = 0.0;  // bound of the last product (CERT)
   auto product = [&]() {
     // four interleaved chains (rows r = q mod 4), then (t0 t1)(t2 t3): fixed order
     cplx t[4];
+    double et[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) t[q] = q < N ? rs[q] : cplx{1.0, 0.0};
+    for (int q = 0; q < 4; ++q) {
+      t[q] = q < N ? rs[q] : cplx{1.0, 0.0};
+      et[q] = q < N ? er[q] : 0.0;
+    }
 #pragma unroll
-    for (int r = 4; r < N; ++r) t[r & 3] = c_mul(t[r & 3], rs[r]);
-    if constexpr (N == 1) return t[0];
-    if constexpr (N == 2) return c_mul(t[0], t[1]);
-    if constexpr (N == 3) return c_mul(c_mul(t[0], t[1]), t[2]);
-    return c_mul(c_mul(t[0], t[1]), c_mul(t[2], t[3]));
+    for (int r = 4; r < N; ++r) {
+      if (CERT) et[r & 3] = cert_mul(t[r & 3], et[r & 3], rs[r], er[r]);
+      t[r & 3] = c_mul(t[r & 3], rs[r]);
+    }
+    if constexpr (N == 1) {
+      ep = et[0];
+      return t[0];
+    }
+    if constexpr (N == 2) {
+      if (CERT) ep = cert_mul(t[0], et[0], t[1], et[1]);
+      return c_mul(t[0], t[1]);
+    }
+    if constexpr (N == 3) {
+      const cplx t01 = c_mul(t[0], t[1]);
+      if (CERT) ep = cert_mul(t01, cert_mul(t[0], et[0], t[1], et[1]), t[2], et[2]);
+      return c_mul(t01, t[2]);
+    }
+    const cplx t01 = c_mul(t[0], t[1]), t23 = c_mul(t[2], t[3]);
+    if (CERT) ep = cert_mul(t01, cert_mul(t[0], et[0], t[1], et[1]), t23, cert_mul(t[2], et[2], t[3], et[3]));
+    return c_mul(t01, t23);
   };
 
   cplx acc = product();  // term j = 0: start is even, sign +
   double aacc = KAPPA ? c_abs(acc) : 0.0;
+  double E = CERT ? ep : 0.0;
   const uint32_t nsteps = 1u << M;
   constexpr int U = HOT;  // static block length 2^U: steps t = 1..2^U-1 use hot columns
   if constexpr (U == 0) {
@@ -69,10 +109,14 @@ __device__ inline void glynn_range_fp64(const cplx* __restrict__ AT,  // LDS, AT
       const double step = ((i >> (k + 1)) & 1u) ? 2.0 : -2.0;
       const cplx* col = AT + (k + 1) * N;
 #pragma unroll
-      for (int r = 0; r < N; ++r) rs[r] = c_axpy(step, col[r], rs[r]);
+      for (int r = 0; r < N; ++r) {
+        rs[r] = c_axpy(step, col[r], rs[r]);
+        if (CERT) er[r] = cert_round(rs[r], er[r]);
+      }
       cplx p = product();
       acc = (j & 1u) ? c_sub(acc, p) : c_add(acc, p);
       if (KAPPA) aacc += c_abs(p);
+      if (CERT) E = cert_round(acc, E + ep);
     }
   } else {
     const uint32_t nblocks = nsteps >> U;
@@ -97,14 +141,19 @@ __device__ inline void glynn_range_fp64(const cplx* __restrict__ AT,  // LDS, AT
 #pragma unroll
           for (int r = 0; r < N; ++r) rs[r] = c_axpy(step, col[r], rs[r]);
         }
+        if (CERT) {
+#pragma unroll
+          for (int r = 0; r < N; ++r) er[r] = cert_round(rs[r], er[r]);
+        }
         cplx p = product();
         acc = (t & 1) ? c_sub(acc, p) : c_add(acc, p);
         if (KAPPA) aacc += c_abs(p);
+        if (CERT) E = cert_round(acc, E + ep);
       }
     }
   }
   acc_out = acc;
-  abs_out = aacc;
+  abs_out = CERT ? E : aacc;
 }
 
 // Same walk in complex double-double (reference: core/permanent_dd.cu:34-73,
@@ -189,19 +238,25 @@ __global__ __launch_bounds__(64) void perm_kernel(const cplx* __restrict__ mats,
     }
   } else {
     constexpr bool KAPPA = (MODE == PERM_KAPPA);
+    constexpr bool CERT = (MODE == PERM_CERT);
     cplx acc;
     double aacc;
-    glynn_range_fp64<N, C::HOT, KAPPA>(AT[gidx], start, M, acc, aacc);
-    acc = group_sum<LPM>(acc);
-    if (KAPPA) aacc = group_sum<LPM>(aacc);
+    glynn_range_fp64<N, C::HOT, KAPPA, CERT>(AT[gidx], start, M, acc, aacc);
+    if constexpr (CERT) {
+      cert_group_sum<LPM>(acc, aacc);
+    } else {
+      acc = group_sum<LPM>(acc);
+      if (KAPPA) aacc = group_sum<LPM>(aacc);
+    }
     if (sub == 0 && b < B) {
       if constexpr (W == 1) {
         const double scale = 1.0 / (double)(1ull << (N - 1));
         out[b] = c_scale(acc, scale);
         if (KAPPA) aux[b] = aacc * scale;
+        if (CERT) aux[b] = cert_finish(aacc * scale + CERT_UFL, perm_cert_depth<N>());
       } else {
         out[b * W + chunk] = acc;
-        if (KAPPA) aux[b * W + chunk] = aacc;
+        if (KAPPA || CERT) aux[b * W + chunk] = aacc;
       }
     }
   }
@@ -230,9 +285,11 @@ __global__ void perm_reduce_kernel(const cplx* __restrict__ part, const double* 
     for (int w = 0; w < W; ++w) {
       s = c_add(s, part[b * W + w]);
       if (MODE == PERM_KAPPA) a += part_aux[b * W + w];
+      if (MODE == PERM_CERT) a = cert_round(s, a + part_aux[b * W + w]);
     }
     out[b] = c_scale(s, scale);
     if (MODE == PERM_KAPPA) aux[b] = a * scale;
+    if (MODE == PERM_CERT) aux[b] = cert_finish(a * scale + CERT_UFL, perm_cert_depth<N>());
   }
 }
 
@@ -254,13 +311,15 @@ static int launch_perm_n(const cplx* d_in, int B, cplx* d_out, double* d_aux, vo
   return (int)hipGetLastError();
 }
 
-// n == 0: perm = 1 (empty product); kappa indicator 1
-__global__ void perm_n0_kernel(int B, cplx* out, double* aux) {
+// n == 0: perm = 1 (empty product); kappa indicator 1, certificate 0 (exact)
+__global__ void perm_n0_kernel(int B, cplx* out, double* aux, double auxval) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   out[b] = cplx{1.0, 0.0};
-  if (aux) aux[b] = 1.0;
+  if (aux) aux[b] = auxval;
 }
+
+
 
 template <int MODE, int... Ns>
 static int dispatch_perm(int n, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st,
@@ -298,13 +357,14 @@ int perm_launch(int n, int mode, const cplx* d_in, int B, cplx* d_out, double* d
   if (n < 0 || n > PERM_MAX_N) return GBSK_ERR_DIM;
   if (n == 0) {
     hipLaunchKernelGGL(perm_n0_kernel, dim3((B + 255) / 256), dim3(256), 0, st, B, d_out,
-                       mode == PERM_KAPPA ? d_aux : nullptr);
+                       (mode == PERM_KAPPA || mode == PERM_CERT) ? d_aux : nullptr, mode == PERM_KAPPA ? 1.0 : 0.0);
     return (int)hipGetLastError();
   }
   switch (mode) {
     case PERM_FP64: return dispatch_perm<PERM_FP64>(n, d_in, B, d_out, d_aux, ws, st, PermSizes{});
     case PERM_KAPPA: return dispatch_perm<PERM_KAPPA>(n, d_in, B, d_out, d_aux, ws, st, PermSizes{});
     case PERM_DD: return dispatch_perm<PERM_DD>(n, d_in, B, d_out, d_aux, ws, st, PermSizes{});
+    case PERM_CERT: return dispatch_perm<PERM_CERT>(n, d_in, B, d_out, d_aux, ws, st, PermSizes{});
   }
   return GBSK_ERR_ARG;
 }

@@ -11,7 +11,7 @@
 namespace gbsk {
 
 constexpr int PERM_MAX_N = 28;  // 2^27 Gray terms; 32-bit term indices
-enum PermMode { PERM_FP64 = 0, PERM_DD = 1, PERM_KAPPA = 2 };
+enum PermMode { PERM_FP64 = 0, PERM_DD = 1, PERM_KAPPA = 2, PERM_CERT = 3 };
 
 // Work split of one n x n permanent (depends on n and the tier only, never on B):
 //   LPM lanes per matrix (64 / LPM matrices share one wave),

@@ -150,3 +150,45 @@ def test_session_and_resident(ext, oracle):
     res = s.perm_resident(A)
     assert np.array_equal(res.numpy(), want)
     s.close()
+
+
+# ------------------------------------------------------------------ certified
+@pytest.mark.parametrize("n", [1, 2, 5, 8, 11, 14, 16])
+def test_perm_certified_encloses_and_matches_plain(ext, oracle, n):
+    print(f"perm certified n={n}", flush=True)
+    A = rng_complex((24, n, n), 700 + n)
+    v, bound = ext.perm_certified(A)
+    assert np.array_equal(v, ext.perm(A))          # value path is the plain kernel's
+    exact = oracle.perm_q(A)
+    assert np.all(np.isfinite(bound)) and np.all(bound >= 0)
+    assert np.all(np.abs(v - exact) <= bound)       # 100% enclosure
+    # and the bound is informative: within a modest factor of the forward-error scale
+    assert np.all(bound <= 1e3 * glynn_bound(oracle, A) + 1e-300)
+
+
+@pytest.mark.parametrize("n", [20, 22])
+def test_perm_certified_chunked(ext, oracle, n):
+    print(f"perm certified chunked n={n}", flush=True)
+    A = rng_complex((2, n, n), 750 + n, 0.5)
+    v, bound = ext.perm_certified(A)
+    assert np.array_equal(v, ext.perm(A))
+    assert np.all(np.abs(v - oracle.perm_dd(A)) <= bound)
+
+
+def test_perm_certified_adversarial_cancellation(ext, oracle):
+    # perm = tiny against large terms: the bound must still enclose (and be large)
+    g = np.random.default_rng(12)
+    R = g.standard_normal((9, 9)) + 1j * g.standard_normal((9, 9))
+    A = R.copy()
+    A[1] = A[0] + 1e-10 * R[1]
+    A[:, 1] = -A[:, 0] + 1e-10 * R[:, 1]
+    v, bound = ext.perm_certified(A[None])
+    exact = oracle.perm_q(A)[0]
+    assert abs(v[0] - exact) <= bound[0]
+    assert bound[0] / abs(exact) > 1e-8
+
+
+def test_perm_certified_refuses_overflow(ext):
+    A = np.full((1, 12, 12), 1e30 + 0j)
+    v, bound = ext.perm_certified(A)
+    assert not np.isfinite(bound[0]) or not np.isfinite(v[0])
