@@ -4,13 +4,13 @@
 
 namespace gbsk {
 
-// trivial sizes: N == 0 -> 1; odd N -> 0 (haf) or NaN (lhaf, augmented upstream)
-__global__ void haf_trivial_kernel(int N, bool loop, int B, cplx* out, double* aux) {
+// N == 0: value 1 (kappa indicator 1, certificate 0); odd N: haf 0 exactly, lhaf NaN
+__global__ void haf_trivial_kernel(int N, bool loop, int B, cplx* out, double* aux, bool kappa) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   if (N == 0) {
     out[b] = cplx{1.0, 0.0};
-    if (aux) aux[b] = 1.0;
+    if (aux) aux[b] = kappa ? 1.0 : 0.0;
   } else if (!loop) {
     out[b] = cplx{0.0, 0.0};
     if (aux) aux[b] = 0.0;
@@ -31,7 +31,8 @@ int haf_max_dim(bool, int tier) {
   switch (tier) {
     case GBSK_FP64:
     case GBSK_DD:
-    case GBSK_KAPPA: return HAF_MAX_N;
+    case GBSK_KAPPA:
+    case GBSK_CERTIFIED: return HAF_MAX_N;
   }
   return 0;
 }
@@ -40,19 +41,21 @@ int haf_launch(bool loop, int tier, int N, const cplx* d_in, int B, cplx* d_out,
                hipStream_t st) {
   if (B <= 0) return 0;
   if (N < 0 || N > HAF_MAX_N) return GBSK_ERR_DIM;
-  if (tier != GBSK_FP64 && tier != GBSK_DD && tier != GBSK_KAPPA) return GBSK_ERR_FUNC;
+  if (tier != GBSK_FP64 && tier != GBSK_DD && tier != GBSK_KAPPA && tier != GBSK_CERTIFIED) return GBSK_ERR_FUNC;
   if (N == 0 || (N & 1)) {
     hipLaunchKernelGGL(haf_trivial_kernel, dim3((B + 255) / 256), dim3(256), 0, st, N, loop, B, d_out,
-                       tier == GBSK_KAPPA ? d_aux : nullptr);
+                       (tier == GBSK_KAPPA || tier == GBSK_CERTIFIED) ? d_aux : nullptr, tier == GBSK_KAPPA);
     return (int)hipGetLastError();
   }
   if (!ws) return GBSK_ERR_WORKSPACE;
-  if (tier == GBSK_DD) return loop ? run_haf<ArDD, true, false>(N, d_in, B, d_out, nullptr, ws, st)
-                                   : run_haf<ArDD, false, false>(N, d_in, B, d_out, nullptr, ws, st);
-  if (tier == GBSK_KAPPA) return loop ? run_haf<ArF64, true, true>(N, d_in, B, d_out, d_aux, ws, st)
-                                      : run_haf<ArF64, false, true>(N, d_in, B, d_out, d_aux, ws, st);
-  return loop ? run_haf<ArF64, true, false>(N, d_in, B, d_out, nullptr, ws, st)
-              : run_haf<ArF64, false, false>(N, d_in, B, d_out, nullptr, ws, st);
+  if (tier == GBSK_DD) return loop ? run_haf<ArDD, true, HM_PLAIN>(N, d_in, B, d_out, nullptr, ws, st)
+                                   : run_haf<ArDD, false, HM_PLAIN>(N, d_in, B, d_out, nullptr, ws, st);
+  if (tier == GBSK_KAPPA) return loop ? run_haf<ArF64, true, HM_KAPPA>(N, d_in, B, d_out, d_aux, ws, st)
+                                      : run_haf<ArF64, false, HM_KAPPA>(N, d_in, B, d_out, d_aux, ws, st);
+  if (tier == GBSK_CERTIFIED) return loop ? run_haf<ArF64, true, HM_CERT>(N, d_in, B, d_out, d_aux, ws, st)
+                                          : run_haf<ArF64, false, HM_CERT>(N, d_in, B, d_out, d_aux, ws, st);
+  return loop ? run_haf<ArF64, true, HM_PLAIN>(N, d_in, B, d_out, nullptr, ws, st)
+              : run_haf<ArF64, false, HM_PLAIN>(N, d_in, B, d_out, nullptr, ws, st);
 }
 
 

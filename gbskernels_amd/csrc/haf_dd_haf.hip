@@ -1,6 +1,6 @@
-// haf_dd_haf.hip -- instantiation unit: run_haf<ArDD, LOOP=false, KAPPA=false>
+// haf_dd_haf.hip -- instantiation unit: run_haf<ArDD, LOOP=false, HM_PLAIN>
 #include "haf_impl.hpp"
 
 namespace gbsk {
-template int run_haf<ArDD, false, false>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+template int run_haf<ArDD, false, HM_PLAIN>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
 }  // namespace gbsk

@@ -1,6 +1,6 @@
-// haf_dd_lhaf.hip -- instantiation unit: run_haf<ArDD, LOOP=true, KAPPA=false>
+// haf_dd_lhaf.hip -- instantiation unit: run_haf<ArDD, LOOP=true, HM_PLAIN>
 #include "haf_impl.hpp"
 
 namespace gbsk {
-template int run_haf<ArDD, true, false>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+template int run_haf<ArDD, true, HM_PLAIN>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
 }  // namespace gbsk

@@ -25,6 +25,7 @@
 
 #include <utility>
 
+#include "cert.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -119,15 +120,64 @@ __device__ inline typename Ar::T group_reduce(typename Ar::T v, typename Ar::T* 
   return s;
 }
 
+// ------------------------------------------------------------ certified parts
+// (fp64 only; cert.hpp states the error model). Bounds are 1-norm bounds of the
+// error of the computed complex value relative to the exact value of the same
+// formula on the exact inputs.
+
+enum HafMode { HM_PLAIN = 0, HM_KAPPA = 1, HM_CERT = 2 };
+
+// gamma_k = k u / (1 - k u) <= k u (1 + 1e-6) for k u < 1e-7 (k <= 2 HAF_MAX_N here)
+__device__ inline double cert_gamma(int k) { return (double)k * CERT_U * 1.000001; }
+
+// bound of r = c_fma(x, y, a): propagated input errors plus the four roundings,
+// each at most u times the magnitude it rounds (see common.hpp c_fma)
+__device__ inline double cert_fma(cplx x, double ex, cplx y, double ey, cplx a, double ea, cplx r) {
+  const double X = cert_norm1(x), Y = cert_norm1(y);
+  double t = fma(X, ey, ea + CERT_UFL);
+  t = fma(Y, ex, t);
+  t = fma(ex, ey, t);
+  return fma(CERT_U, cert_norm1(a) + fma(X, Y, cert_norm1(r)), t);
+}
+
+// LDS reduction of (value, bound) over the S lanes of each group, same order as
+// group_reduce (so the value is bit-identical to the plain kernel's)
+template <int S>
+__device__ inline void cert_group_reduce(cplx& v, double& e, cplx* red, double* rede) {
+  const int lane = threadIdx.x;
+  red[lane] = v;
+  rede[lane] = e;
+  __syncthreads();
+  const int base = (lane / S < 64 / S) ? (lane / S) * S : 0;
+  cplx s = red[base];
+  double E = rede[base];
+#pragma unroll
+  for (int i = 1; i < S; ++i) {
+    s = c_add(s, red[base + i]);
+    E = cert_round(s, E + rede[base + i]);
+  }
+  __syncthreads();
+  v = s;
+  e = E;
+}
+
+// rounding depth of a phase's bound arithmetic (generous; see cert_finish)
+__device__ inline double haf_cert_depth(int S, int n, int count) {
+  return 64.0 * (n + 1) * (S + 8) + 8.0 * count + 16.0 * n * n + 256.0;
+}
+
 // ---------------------------------------------------------------- phase kernel
 // One workgroup (one wave) = one instance; evaluates every subset of size
-// m = S/2 and writes the phase partial (and sum |term| in kappa mode).
-//   part[b * n + (m - 1)]      phase partial (T collapsed per tier below)
-template <class Ar, bool LOOP, int S, bool KAPPA>
+// m = S/2 and writes the phase partial (and sum |term| in kappa mode, the
+// bound in certified mode).
+//   part[b * n + (m - 1)]      phase partial
+template <class Ar, bool LOOP, int S, int MODE>
 __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ mats, int N, int B,
                                                        typename Ar::T* __restrict__ part,
                                                        double* __restrict__ part_abs) {
   using T = typename Ar::T;
+  constexpr bool KAPPA = (MODE == HM_KAPPA);
+  constexpr bool CERT = (MODE == HM_CERT);
   constexpr int M = S / 2;
   constexpr int G = 64 / S;
   extern __shared__ cplx As[];  // N * N, dynamic
@@ -135,6 +185,10 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
   __shared__ T wbuf[64];
   __shared__ T kgs[G][HAF_MAX_PAIRS + 1];  // k g_k of each group's current subset
   __shared__ T es[G][HAF_MAX_PAIRS + 1];   // Newton coefficients e_0..e_n
+  __shared__ double rede[CERT ? 64 : 1];
+  __shared__ double wbufe[CERT ? 64 : 1];
+  __shared__ double kge[CERT ? G : 1][HAF_MAX_PAIRS + 1];
+  __shared__ double ese[CERT ? G : 1][HAF_MAX_PAIRS + 1];
   const int lane = threadIdx.x;
   const int n = N / 2;
   const int64_t b = blockIdx.x;
@@ -151,9 +205,10 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
   const bool lane_ok = g < G;
   const int count = binom(n, M);
   const int h = (n + 1) / 2;
+  const double gam = CERT ? cert_gamma(2 * S) : 0.0;
 
   T group_acc = Ar::zero();
-  double group_abs = 0.0;
+  double group_abs = 0.0;  // kappa: sum |term|; certified: bound of group_acc
 
   for (int q0 = 0; q0 < count; q0 += G) {
     const int q = q0 + (lane_ok ? g : 0);  // groups past G mirror group 0 (discarded)
@@ -164,50 +219,109 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
     const int R = 2 * p[r >> 1] + (r & 1);
     const int gs = lane_ok ? g : 0;
     T P[S];
+    double eP[CERT ? S : 1];
 #pragma unroll
     for (int u = 0; u < M; ++u) {
       P[2 * u] = Ar::from(As[R * N + 2 * p[u] + 1]);
       P[2 * u + 1] = Ar::from(As[R * N + 2 * p[u]]);
     }
+    if (CERT) {
+#pragma unroll
+      for (int c = 0; c < S; ++c) eP[c] = 0.0;  // C is exact
+    }
     // p_1 = tr C (C[r][r] = A[R][R^1]) and p_2 = tr(C C) through the partner row
     {
       T d1 = Ar::from(As[R * N + (R ^ 1)]);
       T t2 = Ar::zero();
+      double e2 = 0.0;
 #pragma unroll
-      for (int c = 0; c < S; ++c) t2 = Ar::fma(P[c], Ar::shfl_xor(P[c ^ 1], 1), t2);
-      const T s1 = group_reduce<Ar, S>(d1, red);
-      const T s2 = group_reduce<Ar, S>(t2, red);
-      if (r == 0 && lane_ok) {
-        kgs[g][1] = Ar::scale(s1, 0.5);
-        kgs[g][2] = Ar::scale(s2, 0.5);
+      for (int c = 0; c < S; ++c) {
+        const T y = Ar::shfl_xor(P[c ^ 1], 1);
+        const T t2n = Ar::fma(P[c], y, t2);
+        if constexpr (CERT) e2 = cert_fma(P[c], 0.0, y, 0.0, t2, e2, t2n);
+        t2 = t2n;
+      }
+      if constexpr (CERT) {
+        double e1 = 0.0;
+        cert_group_reduce<S>(d1, e1, red, rede);
+        cert_group_reduce<S>(t2, e2, red, rede);
+        if (r == 0 && lane_ok) {
+          kgs[g][1] = Ar::scale(d1, 0.5);
+          kgs[g][2] = Ar::scale(t2, 0.5);
+          kge[g][1] = fma(0.5, e1, CERT_UFL);
+          kge[g][2] = fma(0.5, e2, CERT_UFL);
+        }
+      } else {
+        const T s1 = group_reduce<Ar, S>(d1, red);
+        const T s2 = group_reduce<Ar, S>(t2, red);
+        if (r == 0 && lane_ok) {
+          kgs[g][1] = Ar::scale(s1, 0.5);
+          kgs[g][2] = Ar::scale(s2, 0.5);
+        }
       }
     }
     // C^j for j = 2..h: one matrix product each, then p_{2j-1}, p_{2j}
     for (int j = 2; j <= h; ++j) {
       T acc[S];
+      double eA[CERT ? S : 1];
 #pragma unroll
       for (int c = 0; c < S; ++c) acc[c] = Ar::zero();
+      if (CERT) {
+#pragma unroll
+        for (int c = 0; c < S; ++c) eA[c] = (double)S * CERT_UFL;
+      }
 #pragma unroll
       for (int t = 0; t < S; ++t) {
         const cplx* At = As + (2 * p[t >> 1] + (t & 1)) * N;
+        // a-priori bound of the fma-chain dot product: sum_t (eP_t + gamma_2S |P_t|) |C_tc|
+        const double wt = CERT ? fma(gam, cert_norm1(Ar::to_c(P[t])), eP[t]) : 0.0;
 #pragma unroll
         for (int u = 0; u < M; ++u) {
           const cplx a1 = At[2 * p[u] + 1], a0 = At[2 * p[u]];
           acc[2 * u] = Ar::fma_c(P[t], a1, acc[2 * u]);
           acc[2 * u + 1] = Ar::fma_c(P[t], a0, acc[2 * u + 1]);
+          if (CERT) {
+            eA[2 * u] = fma(wt, cert_norm1(a1), eA[2 * u]);
+            eA[2 * u + 1] = fma(wt, cert_norm1(a0), eA[2 * u + 1]);
+          }
         }
       }
       T todd = Ar::zero(), teven = Ar::zero();
+      double eo = 0.0, ee = 0.0;
 #pragma unroll
       for (int c = 0; c < S; ++c) {
-        todd = Ar::fma(acc[c], Ar::shfl_xor(P[c ^ 1], 1), todd);
-        teven = Ar::fma(acc[c], Ar::shfl_xor(acc[c ^ 1], 1), teven);
+        const T yo = Ar::shfl_xor(P[c ^ 1], 1);
+        const T ye = Ar::shfl_xor(acc[c ^ 1], 1);
+        const T ton = Ar::fma(acc[c], yo, todd);
+        const T ten = Ar::fma(acc[c], ye, teven);
+        if constexpr (CERT) {
+          const double eyo = shfl_xor_d(eP[c ^ 1], 1), eye = shfl_xor_d(eA[c ^ 1], 1);
+          eo = cert_fma(acc[c], eA[c], yo, eyo, todd, eo, ton);
+          ee = cert_fma(acc[c], eA[c], ye, eye, teven, ee, ten);
+        }
+        todd = ton;
+        teven = ten;
       }
-      const T so = group_reduce<Ar, S>(todd, red);
-      const T se = group_reduce<Ar, S>(teven, red);
-      if (r == 0 && lane_ok) {
-        kgs[g][2 * j - 1] = Ar::scale(so, 0.5);
-        if (2 * j <= HAF_MAX_PAIRS) kgs[g][2 * j] = Ar::scale(se, 0.5);
+      if constexpr (CERT) {
+        cert_group_reduce<S>(todd, eo, red, rede);
+        cert_group_reduce<S>(teven, ee, red, rede);
+        if (r == 0 && lane_ok) {
+          kgs[g][2 * j - 1] = Ar::scale(todd, 0.5);
+          kge[g][2 * j - 1] = fma(0.5, eo, CERT_UFL);
+          if (2 * j <= HAF_MAX_PAIRS) {
+            kgs[g][2 * j] = Ar::scale(teven, 0.5);
+            kge[g][2 * j] = fma(0.5, ee, CERT_UFL);
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < S; ++c) eP[c] = eA[c];
+      } else {
+        const T so = group_reduce<Ar, S>(todd, red);
+        const T se = group_reduce<Ar, S>(teven, red);
+        if (r == 0 && lane_ok) {
+          kgs[g][2 * j - 1] = Ar::scale(so, 0.5);
+          if (2 * j <= HAF_MAX_PAIRS) kgs[g][2 * j] = Ar::scale(se, 0.5);
+        }
       }
 #pragma unroll
       for (int c = 0; c < S; ++c) P[c] = acc[c];
@@ -216,35 +330,69 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
       // w_1 = d, w_{k+1} = C w_k; v_k = 1/2 sum_a d_(a^1) w_k[a]; k g_k += k v_k
       const cplx dpartner = As[(R ^ 1) * N + (R ^ 1)];
       T w = Ar::from(As[R * N + R]);
+      double ew = 0.0;
       for (int k = 1; k <= n; ++k) {
-        const T vk = group_reduce<Ar, S>(Ar::fma_c(w, dpartner, Ar::zero()), red);
-        if (r == 0 && lane_ok) kgs[g][k] = Ar::add(kgs[g][k], Ar::scale(vk, 0.5 * (double)k));
+        T vpart = Ar::fma_c(w, dpartner, Ar::zero());
+        if constexpr (CERT) {
+          double ev = cert_fma(Ar::to_c(w), ew, dpartner, 0.0, cplx{0.0, 0.0}, 0.0, vpart);
+          cert_group_reduce<S>(vpart, ev, red, rede);
+          if (r == 0 && lane_ok) {
+            // kg_k + fl(fl(v * 0.5 k)): two roundings
+            const T kv = Ar::scale(vpart, 0.5 * (double)k);
+            const T sum = Ar::add(kgs[g][k], kv);
+            const double ekv = cert_round(kv, fma(0.5 * (double)k, ev, 0.0));
+            kge[g][k] = cert_round(sum, kge[g][k] + ekv);
+            kgs[g][k] = sum;
+          }
+        } else {
+          const T vk = group_reduce<Ar, S>(vpart, red);
+          if (r == 0 && lane_ok) kgs[g][k] = Ar::add(kgs[g][k], Ar::scale(vk, 0.5 * (double)k));
+        }
         if (k < n) {
           wbuf[lane] = w;
+          if (CERT) wbufe[lane] = ew;
           __syncthreads();
           T nw = Ar::zero();
+          double enw = CERT ? (double)S * CERT_UFL : 0.0;
 #pragma unroll
           for (int u = 0; u < M; ++u) {
-            nw = Ar::fma_c(wbuf[gs * S + 2 * u], As[R * N + 2 * p[u] + 1], nw);
-            nw = Ar::fma_c(wbuf[gs * S + 2 * u + 1], As[R * N + 2 * p[u]], nw);
+            const T w1 = wbuf[gs * S + 2 * u], w0 = wbuf[gs * S + 2 * u + 1];
+            const cplx a1 = As[R * N + 2 * p[u] + 1], a0 = As[R * N + 2 * p[u]];
+            nw = Ar::fma_c(w1, a1, nw);
+            nw = Ar::fma_c(w0, a0, nw);
+            if (CERT) {
+              enw = fma(fma(gam, cert_norm1(Ar::to_c(w1)), wbufe[gs * S + 2 * u]), cert_norm1(a1), enw);
+              enw = fma(fma(gam, cert_norm1(Ar::to_c(w0)), wbufe[gs * S + 2 * u + 1]), cert_norm1(a0), enw);
+            }
           }
           __syncthreads();
           w = nw;
+          ew = enw;
         }
       }
     }
     __syncthreads();
     // Newton: e_j = (1/j) sum_{k=1..j} kg_k e_{j-k}, by the first lane of each group
     T coeff = Ar::zero();
+    double ecoeff = 0.0;
     if (r == 0 && lane_ok) {
       T* e = es[g];
       e[0] = Ar::one();
+      if (CERT) ese[g][0] = 0.0;
       for (int jj = 1; jj <= n; ++jj) {
         T a = Ar::zero();
-        for (int k = 1; k <= jj; ++k) a = Ar::fma(kgs[g][k], e[jj - k], a);
+        double ea = 0.0;
+        for (int k = 1; k <= jj; ++k) {
+          const T an = Ar::fma(kgs[g][k], e[jj - k], a);
+          if (CERT) ea = cert_fma(Ar::to_c(kgs[g][k]), kge[g][k], Ar::to_c(e[jj - k]), ese[g][jj - k], Ar::to_c(a), ea,
+                                  Ar::to_c(an));
+          a = an;
+        }
         e[jj] = Ar::div_int(a, jj);
+        if (CERT) ese[g][jj] = cert_round(Ar::to_c(e[jj]), ea / (double)jj);
       }
       coeff = e[n];
+      if (CERT) ecoeff = ese[g][n];
     }
     __syncthreads();
     if ((n - M) & 1) coeff = Ar::neg(coeff);
@@ -254,15 +402,22 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
         const cplx cc = Ar::to_c(coeff);
         group_abs += c_abs(cc);
       }
+      if (CERT) group_abs = cert_round(Ar::to_c(group_acc), group_abs + ecoeff);
     }
   }
   // combine the G group partials in group order
   if (r == 0 && lane_ok) red[g] = group_acc;
+  if (CERT && r == 0 && lane_ok) rede[g] = group_abs;
   __syncthreads();
   if (lane == 0) {
     T s = red[0];
-    for (int i = 1; i < G; ++i) s = Ar::add(s, red[i]);
+    double E = CERT ? rede[0] : 0.0;
+    for (int i = 1; i < G; ++i) {
+      s = Ar::add(s, red[i]);
+      if (CERT) E = cert_round(Ar::to_c(s), E + rede[i]);
+    }
     part[b * n + (M - 1)] = s;
+    if (CERT) part_abs[b * n + (M - 1)] = cert_finish(E, haf_cert_depth(S, n, count));
   }
   if (KAPPA) {
     __syncthreads();
@@ -278,54 +433,58 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
 }
 
 // out[b] = sum over phases m = 1..n in order (fixed association)
-template <class Ar, bool KAPPA>
+template <class Ar, int MODE>
 __global__ void haf_reduce_kernel(const typename Ar::T* __restrict__ part, const double* __restrict__ part_abs,
                                   int n, int B, cplx* __restrict__ out, double* __restrict__ aux) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   typename Ar::T s = part[b * n];
-  double a = KAPPA ? part_abs[b * n] : 0.0;
+  double a = MODE != HM_PLAIN ? part_abs[b * n] : 0.0;
   for (int m = 1; m < n; ++m) {
     s = Ar::add(s, part[b * n + m]);
-    if (KAPPA) a += part_abs[b * n + m];
+    if (MODE == HM_KAPPA) a += part_abs[b * n + m];
+    if (MODE == HM_CERT) a = cert_round(Ar::to_c(s), a + part_abs[b * n + m]);
   }
   out[b] = Ar::to_c(s);
-  if (KAPPA) aux[b] = a;
+  if (MODE == HM_KAPPA) aux[b] = a;
+  if (MODE == HM_CERT) aux[b] = cert_finish(a, 4.0 * n + 8.0);
 }
 
-template <class Ar, bool LOOP, bool KAPPA, int S>
+template <class Ar, bool LOOP, int MODE, int S>
 static void launch_phase(const cplx* d_in, int N, int B, typename Ar::T* part, double* part_abs, hipStream_t st) {
-  hipLaunchKernelGGL((haf_phase_kernel<Ar, LOOP, S, KAPPA>), dim3((unsigned)B), dim3(64),
+  hipLaunchKernelGGL((haf_phase_kernel<Ar, LOOP, S, MODE>), dim3((unsigned)B), dim3(64),
                      (size_t)N * N * sizeof(cplx), st, d_in, N, B, part, part_abs);
 }
 
-template <class Ar, bool LOOP, bool KAPPA, int... Ss>
+template <class Ar, bool LOOP, int MODE, int... Ss>
 static void launch_phases(const cplx* d_in, int N, int B, typename Ar::T* part, double* part_abs, hipStream_t st,
                           std::integer_sequence<int, Ss...>) {
   const int n = N / 2;
-  ((2 * (Ss + 1) <= 2 * n ? (launch_phase<Ar, LOOP, KAPPA, 2 * (Ss + 1)>(d_in, N, B, part, part_abs, st), 0) : 0),
+  ((2 * (Ss + 1) <= 2 * n ? (launch_phase<Ar, LOOP, MODE, 2 * (Ss + 1)>(d_in, N, B, part, part_abs, st), 0) : 0),
    ...);
 }
 
-template <class Ar, bool LOOP, bool KAPPA>
+template <class Ar, bool LOOP, int MODE>
 int run_haf(int N, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
   using T = typename Ar::T;
   const int n = N / 2;
   T* part = reinterpret_cast<T*>(ws);
   double* part_abs = reinterpret_cast<double*>(part + (size_t)B * n);
-  launch_phases<Ar, LOOP, KAPPA>(d_in, N, B, part, part_abs, st, std::make_integer_sequence<int, HAF_MAX_PAIRS>{});
-  hipLaunchKernelGGL((haf_reduce_kernel<Ar, KAPPA>), dim3((B + 255) / 256), dim3(256), 0, st, part, part_abs, n, B,
+  launch_phases<Ar, LOOP, MODE>(d_in, N, B, part, part_abs, st, std::make_integer_sequence<int, HAF_MAX_PAIRS>{});
+  hipLaunchKernelGGL((haf_reduce_kernel<Ar, MODE>), dim3((B + 255) / 256), dim3(256), 0, st, part, part_abs, n, B,
                      d_out, d_aux);
   return (int)hipGetLastError();
 }
 
 // explicit instantiations live in haf_<tier>_<kind>.hip, one translation unit
 // each so that the (heavily unrolled) kernels compile in parallel
-extern template int run_haf<ArF64, false, false>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
-extern template int run_haf<ArF64, true, false>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
-extern template int run_haf<ArF64, false, true>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
-extern template int run_haf<ArF64, true, true>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
-extern template int run_haf<ArDD, false, false>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
-extern template int run_haf<ArDD, true, false>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+extern template int run_haf<ArF64, false, HM_PLAIN>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+extern template int run_haf<ArF64, true, HM_PLAIN>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+extern template int run_haf<ArF64, false, HM_KAPPA>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+extern template int run_haf<ArF64, true, HM_KAPPA>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+extern template int run_haf<ArF64, false, HM_CERT>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+extern template int run_haf<ArF64, true, HM_CERT>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+extern template int run_haf<ArDD, false, HM_PLAIN>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+extern template int run_haf<ArDD, true, HM_PLAIN>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
 
 }  // namespace gbsk

@@ -134,3 +134,35 @@ def test_reference_published_checksums(ext, oracle, ref_checksums):
         a = oracle.gen_bench_complex(dim, B, True, 1.0)
         got = float(np.sum(np.abs(getattr(ext, row["func"])(a))))
         assert got == pytest.approx(row["checksum"], rel=1e-11), (row["func"], dim)
+
+
+# ------------------------------------------------------------------ certified
+@pytest.mark.parametrize("N", [2, 4, 8, 12, 16])
+@pytest.mark.parametrize("func", ["haf", "lhaf"])
+def test_certified_encloses_and_matches_plain(ext, oracle, func, N):
+    print(f"{func} certified N={N}", flush=True)
+    A = rng_symmetric(8 if N <= 12 else 3, N, 40 * N + (func == "lhaf"))
+    v, bound = getattr(ext, f"{func}_certified")(A)
+    assert np.array_equal(v, getattr(ext, func)(A))
+    exact = getattr(oracle, f"{func}_q")(A)
+    assert np.all(np.isfinite(bound)) and np.all(bound >= 0)
+    assert np.all(np.abs(v - exact) <= bound)
+    # informative: within a modest factor of the power-trace error scale
+    assert np.all(bound <= 1e4 * pt_bound(getattr(oracle, f"{func}_abs")(A), N))
+
+
+@pytest.mark.parametrize("func", ["haf", "lhaf"])
+def test_certified_large_encloses_dd_oracle(ext, oracle, func):
+    A = rng_symmetric(2, 20, 55 + (func == "lhaf"), 0.5)
+    v, bound = getattr(ext, f"{func}_certified")(A)
+    assert np.array_equal(v, getattr(ext, func)(A))
+    assert np.all(np.abs(v - getattr(oracle, f"{func}_dd")(A)) <= bound)
+
+
+def test_certified_config1_relative_bound(ext, oracle):
+    G = np.random.default_rng(0).standard_normal((512, 8, 8))
+    A = G + G.transpose(0, 2, 1)
+    v, bound = ext.haf_certified(A)
+    exact = oracle.haf_q(A)
+    assert np.all(np.abs(v - exact) <= bound)
+    assert np.median(bound / np.abs(exact)) < 1e-11
