@@ -176,16 +176,20 @@ def test_perm_certified_chunked(ext, oracle, n):
 
 
 def test_perm_certified_adversarial_cancellation(ext, oracle):
-    # perm = tiny against large terms: the bound must still enclose (and be large)
+    """A (+) R with A = [[1, 1], [1, -1 + d]]: the Gray walk drives one row sum
+    between d and 2 - d, so the fp64 value loses ~u / d relative accuracy even
+    though sum|term| / |perm| is small. The certificate must see it."""
     g = np.random.default_rng(12)
-    R = g.standard_normal((9, 9)) + 1j * g.standard_normal((9, 9))
-    A = R.copy()
-    A[1] = A[0] + 1e-10 * R[1]
-    A[:, 1] = -A[:, 0] + 1e-10 * R[:, 1]
+    R = g.standard_normal((7, 7)) + 1j * g.standard_normal((7, 7))
+    A = np.zeros((9, 9), complex)
+    A[:2, :2] = [[1, 1], [1, -1 + 1e-8]]
+    A[2:, 2:] = R
     v, bound = ext.perm_certified(A[None])
     exact = oracle.perm_q(A)[0]
     assert abs(v[0] - exact) <= bound[0]
-    assert bound[0] / abs(exact) > 1e-8
+    assert abs(v[0] - exact) / abs(exact) > 1e-10      # fp64 really is off here
+    assert bound[0] / abs(exact) > 1e-9                 # and the bound says so
+    assert abs(ext.perm_dd(A[None])[0] - exact) / abs(exact) < 1e-15
 
 
 def test_perm_certified_refuses_overflow(ext):
