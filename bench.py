@@ -70,6 +70,9 @@ def main() -> int:
     ap.add_argument("--dd-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (a known-size kernel that "
+                         "calibrates hardware byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
@@ -81,6 +84,13 @@ def main() -> int:
 
     L = _lib.lib()
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy * (1 << 20) // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": src.numel() * 8}))
+        return 0
     stream = torch.cuda.current_stream(dev)
     sp = stream.cuda_stream
 
@@ -129,10 +139,12 @@ def main() -> int:
     out16 = dev_out[16].cpu().numpy().view(np.complex128).copy()
     out24 = dev_out[24].cpu().numpy().view(np.complex128).copy()
 
-    print(f"bench: dd steps={args.dd_steps}", file=sys.stderr, flush=True)
-    dd_wall, dd_kern_ms = run_steps(1, args.dd_steps, 1)
-    dd16 = dev_out[16].cpu().numpy().view(np.complex128).copy()
-    dd24 = dev_out[24].cpu().numpy().view(np.complex128).copy()
+    dd_wall, dd_kern_ms, dd16, dd24 = None, None, None, None
+    if args.dd_steps > 0:
+        print(f"bench: dd steps={args.dd_steps}", file=sys.stderr, flush=True)
+        dd_wall, dd_kern_ms = run_steps(1, args.dd_steps, 1)
+        dd16 = dev_out[16].cpu().numpy().view(np.complex128).copy()
+        dd24 = dev_out[24].cpu().numpy().view(np.complex128).copy()
 
     # parity gate on a sample (oracle = CPU restatement of cpu_ref's Glynn walk)
     from oracle import oracle as O
@@ -141,8 +153,10 @@ def main() -> int:
     ref24 = O.perm_dd(m24[:2])
     rel = lambda a, b: float(np.max(np.abs(a - b) / np.abs(b)))
     parity = {"n16_fp64_vs_oracle_dd": rel(out16[s16], ref16), "n24_fp64_vs_oracle_dd": rel(out24[:2], ref24),
-              "n16_dd_vs_oracle_dd": rel(dd16[s16], ref16), "n24_dd_vs_oracle_dd": rel(dd24[:2], ref24),
               "samples": [len(s16), 2]}
+    if dd16 is not None:
+        parity["n16_dd_vs_oracle_dd"] = rel(dd16[s16], ref16)
+        parity["n24_dd_vs_oracle_dd"] = rel(dd24[:2], ref24)
     if max(parity["n16_fp64_vs_oracle_dd"], parity["n24_fp64_vs_oracle_dd"]) > 1e-10:
         print(f"bench: PARITY FAILURE {parity}", file=sys.stderr)
         return 1
@@ -163,8 +177,9 @@ def main() -> int:
         per_kernel[f"perm_fp64_n{n}"] = {"batch": m.shape[0], "avg_ms": kern_ms[n],
                                          "evals_per_s": m.shape[0] / (kern_ms[n] * 1e-3),
                                          "tflops": f / (kern_ms[n] * 1e-3) / 1e12}
-        per_kernel[f"perm_dd_n{n}"] = {"batch": m.shape[0], "avg_ms": dd_kern_ms[n],
-                                       "evals_per_s": m.shape[0] / (dd_kern_ms[n] * 1e-3)}
+        if dd_kern_ms:
+            per_kernel[f"perm_dd_n{n}"] = {"batch": m.shape[0], "avg_ms": dd_kern_ms[n],
+                                           "evals_per_s": m.shape[0] / (dd_kern_ms[n] * 1e-3)}
 
     cpu = None
     if not args.no_cpu:
@@ -179,8 +194,8 @@ def main() -> int:
                    "global_batch": evals_per_step, "tier": "fp64"},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "dd": {"evals_per_s": evals_per_step * args.dd_steps / dd_wall,
-               "ms_per_step": dd_wall / args.dd_steps * 1e3, "steps": args.dd_steps},
+        "dd": ({"evals_per_s": evals_per_step * args.dd_steps / dd_wall,
+                "ms_per_step": dd_wall / args.dd_steps * 1e3, "steps": args.dd_steps} if dd_wall else None),
         "per_kernel": per_kernel,
         "parity": parity,
     }
