@@ -1,4 +1,4 @@
-// tor.hip -- torontonian of real, physical-domain O on gfx950 (fp64, kappa).
+// tor.hip -- torontonian of real, physical-domain O on gfx950 (fp64, kappa, dd).
 //
 // tor(O) = sum_{S subset modes} (-1)^(n-|S|) / sqrt(det(I - O)_S), xxpp order
 // (reference: cpu_ref/torontonian.py:48-64, cpu_ref/tor_recursive.py:46-101,
@@ -27,19 +27,115 @@
 namespace gbsk {
 
 constexpr int TOR_MAX_MODES = 20;  // dim 40
-constexpr int TOR_R = 5;           // per-lane trailing modes
 constexpr int TOR_T2 = 6;          // cooperative middle modes (64 tasks)
+
+// real arithmetic policies: binary64 and double-double (common.hpp)
+struct RF64 {
+  using T = double;
+  static constexpr int RMAX = 5;  // per-lane trailing modes
+  __device__ static T zero() { return 0.0; }
+  __device__ static T one() { return 1.0; }
+  __device__ static T from(double x) { return x; }
+  __device__ static T one_minus(cplx o) { return 1.0 - o.re; }
+  __device__ static T neg(cplx o) { return 0.0 - o.re; }
+  __device__ static bool domain(cplx o) { return o.im == 0.0; }  // real O only
+  __device__ static double mag(T a) { return a; }
+  __device__ static T add(T a, T b) { return a + b; }
+  __device__ static T sub(T a, T b) { return a - b; }
+  __device__ static T mul(T a, T b) { return a * b; }
+  __device__ static T div(T a, T b) { return a / b; }
+  __device__ static T sqrt_(T a) { return sqrt(a); }
+  __device__ static T fms(T a, T b, T c) { return fma(-a, b, c); }  // c - a b
+  __device__ static bool pos(T a) { return a > 0.0; }
+  __device__ static double to_d(T a) { return a; }
+  __device__ static T sum64(T v) { return group_sum<64>(v); }
+  using P = cplx;  // partials: complex, so the complex fallback can overwrite them
+  __device__ static P part(T a) { return cplx{a, 0.0}; }
+  __device__ static P part_nan() { return cplx{NAN, NAN}; }
+};
+struct RDD {
+  using T = dd;
+  static constexpr int RMAX = 3;
+  __device__ static T zero() { return dd{0.0, 0.0}; }
+  __device__ static T one() { return dd{1.0, 0.0}; }
+  __device__ static T from(double x) { return dd{x, 0.0}; }
+  __device__ static T one_minus(cplx o) { return two_sum(1.0, -o.re); }  // exact
+  __device__ static T neg(cplx o) { return dd{-o.re, 0.0}; }
+  __device__ static bool domain(cplx o) { return o.im == 0.0; }
+  __device__ static double mag(T a) { return dd_to_d(a); }
+  __device__ static T add(T a, T b) { return dd_add(a, b); }
+  __device__ static T sub(T a, T b) { return dd_sub(a, b); }
+  __device__ static T mul(T a, T b) { return dd_mul(a, b); }
+  __device__ static T div(T a, T b) { return dd_div(a, b); }
+  __device__ static T sqrt_(T a) { return dd_sqrt(a); }
+  __device__ static T fms(T a, T b, T c) { return dd_sub(c, dd_mul(a, b)); }
+  __device__ static bool pos(T a) { return a.hi > 0.0; }
+  __device__ static double to_d(T a) { return dd_to_d(a); }
+  using P = dd;
+  __device__ static P part(T a) { return a; }
+  __device__ static P part_nan() { return dd{NAN, NAN}; }
+  __device__ static T sum64(T v) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+      dd o{shfl_xor_d(v.hi, m), shfl_xor_d(v.lo, m)};
+      v = ((threadIdx.x & m) == 0) ? dd_add(v, o) : dd_add(o, v);
+    }
+    return v;
+  }
+};
+
+// complex symmetric O (no conjugation): L L^T factorisation without pivoting,
+// principal square root of each subset determinant (the np.sqrt convention of
+// cpu_ref/torontonian.py:60); a zero pivot is a breakdown (NaN)
+__device__ inline cplx c_div(cplx a, cplx b) {  // Smith's algorithm
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double r = b.im / b.re, den = fma(b.im, r, b.re);
+    return cplx{fma(a.im, r, a.re) / den, fma(-a.re, r, a.im) / den};
+  }
+  const double r = b.re / b.im, den = fma(b.re, r, b.im);
+  return cplx{fma(a.re, r, a.im) / den, fma(a.im, r, -a.re) / den};
+}
+__device__ inline cplx c_sqrt(cplx z) {  // principal branch
+  if (z.re == 0.0 && z.im == 0.0) return cplx{0.0, z.im};
+  const double r = hypot(z.re, z.im);
+  const double t = sqrt(0.5 * (r + fabs(z.re)));
+  if (z.re >= 0.0) return cplx{t, z.im / (2.0 * t)};
+  return cplx{fabs(z.im) / (2.0 * t), copysign(t, z.im)};
+}
+struct RC64 {
+  using T = cplx;
+  static constexpr int RMAX = 3;
+  __device__ static T zero() { return cplx{0.0, 0.0}; }
+  __device__ static T one() { return cplx{1.0, 0.0}; }
+  __device__ static T from(double x) { return cplx{x, x}; }
+  __device__ static T one_minus(cplx o) { return cplx{1.0 - o.re, 0.0 - o.im}; }
+  __device__ static T neg(cplx o) { return cplx{0.0 - o.re, 0.0 - o.im}; }
+  __device__ static bool domain(cplx) { return true; }
+  __device__ static double mag(T a) { return hypot(a.re, a.im); }
+  __device__ static T add(T a, T b) { return c_add(a, b); }
+  __device__ static T sub(T a, T b) { return c_sub(a, b); }
+  __device__ static T mul(T a, T b) { return c_mul(a, b); }
+  __device__ static T div(T a, T b) { return c_div(a, b); }
+  __device__ static T sqrt_(T a) { return c_sqrt(a); }
+  __device__ static T fms(T a, T b, T c) { return c_sub(c, c_mul(a, b)); }
+  __device__ static bool pos(T a) { return a.re != 0.0 || a.im != 0.0; }
+  __device__ static double to_d(T a) { return a.re; }
+  __device__ static T sum64(T v) { return group_sum<64>(v); }
+  using P = cplx;
+  __device__ static P part(T a) { return a; }
+  __device__ static P part_nan() { return cplx{NAN, NAN}; }
+};
 
 struct TorSplit {
   int R, T2, T1;
 };
 // lanes first (T2 middle modes -> up to 64 tasks), then per-lane depth R, then
 // leading patterns as separate workgroups
-__host__ __device__ inline TorSplit tor_split(int nm) {
+__host__ __device__ inline TorSplit tor_split(int nm, int rmax) {
   TorSplit s;
   s.T2 = nm < TOR_T2 ? nm : TOR_T2;
   const int rest = nm - s.T2;
-  s.R = rest < TOR_R ? rest : TOR_R;
+  s.R = rest < rmax ? rest : rmax;
   s.T1 = rest - s.R;
   return s;
 }
@@ -50,12 +146,13 @@ __device__ constexpr int tri(int r, int c) { return r * (r + 1) / 2 + c; }
 // ----------------------------------------------------------- per-lane bottom
 // Prefix-Cholesky walk over the R trailing modes of one task. M is the task's
 // 2R x 2R Schur complement (packed lower, mode-major), L the factor prefix.
-template <int R, int D, unsigned MASK, int C>
+template <class Ar, int R, int D, unsigned MASK, int C>
 struct Bottom {
-  static __device__ __forceinline__ void run(const double* __restrict__ M, double (&L)[2 * R][2 * R], double det,
-                                             int count, int nm, bool& bad, double& acc, double& aacc) {
+  using T = typename Ar::T;
+  static __device__ __forceinline__ void run(const T* __restrict__ M, T (&L)[2 * R][2 * R], T det, int count, int nm,
+                                             bool& bad, T& acc, double& aacc) {
     // exclude mode D
-    Bottom<R, D + 1, MASK, C>::run(M, L, det, count, nm, bad, acc, aacc);
+    Bottom<Ar, R, D + 1, MASK, C>::run(M, L, det, count, nm, bad, acc, aacc);
     // include mode D: append rows 2C (x) and 2C+1 (p)
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
@@ -72,60 +169,69 @@ struct Bottom {
             ++k;
           }
         const int gc = 2 * mode + (c & 1);
-        double s = gr >= gc ? M[tri(gr, gc)] : M[tri(gc, gr)];
+        T s = gr >= gc ? M[tri(gr, gc)] : M[tri(gc, gr)];
 #pragma unroll
-        for (int u = 0; u < c; ++u) s = fma(-L[r][u], L[c][u], s);
+        for (int u = 0; u < c; ++u) s = Ar::fms(L[r][u], L[c][u], s);
         if (c < r) {
-          L[r][c] = s / L[c][c];
+          L[r][c] = Ar::div(s, L[c][c]);
         } else {
-          bad |= !(s > 0.0);
-          L[r][r] = sqrt(s);
+          bad |= !Ar::pos(s);
+          L[r][r] = Ar::sqrt_(s);
         }
       }
     }
-    const double q = L[2 * C][2 * C] * L[2 * C + 1][2 * C + 1];
-    Bottom<R, D + 1, MASK | (1u << D), C + 1>::run(M, L, det * (q * q), count + 1, nm, bad, acc, aacc);
-  }
-};
-template <int R, unsigned MASK, int C>
-struct Bottom<R, R, MASK, C> {
-  static __device__ __forceinline__ void run(const double* __restrict__, double (&)[2 * R][2 * R], double det,
-                                             int count, int nm, bool&, double& acc, double& aacc) {
-    const double t = 1.0 / sqrt(det);
-    acc += ((nm - count) & 1) ? -t : t;
-    aacc += t;
+    const T q = Ar::mul(L[2 * C][2 * C], L[2 * C + 1][2 * C + 1]);
+    Bottom<Ar, R, D + 1, MASK | (1u << D), C + 1>::run(M, L, Ar::mul(det, Ar::mul(q, q)), count + 1, nm, bad, acc,
+                                                       aacc);
   }
 };
 
-template <int R>
-__device__ inline void bottom_walk(const double* M, double det, int count, int nm, bool& bad, double& acc,
-                                   double& aacc) {
-  double L[2 * R > 0 ? 2 * R : 1][2 * R > 0 ? 2 * R : 1];
+// leaf: (-1)^(nm - count) / sqrt(det)
+template <class Ar>
+__device__ __forceinline__ void tor_leaf(typename Ar::T det, int count, int nm, typename Ar::T& acc, double& aacc) {
+  const typename Ar::T t = Ar::div(Ar::one(), Ar::sqrt_(det));
+  acc = ((nm - count) & 1) ? Ar::sub(acc, t) : Ar::add(acc, t);
+  aacc += Ar::mag(t);
+}
+
+template <class Ar, int R, unsigned MASK, int C>
+struct Bottom<Ar, R, R, MASK, C> {
+  using T = typename Ar::T;
+  static __device__ __forceinline__ void run(const T* __restrict__, T (&)[2 * R][2 * R], T det, int count, int nm,
+                                             bool&, T& acc, double& aacc) {
+    tor_leaf<Ar>(det, count, nm, acc, aacc);
+  }
+};
+
+template <class Ar, int R>
+__device__ inline void bottom_walk(const typename Ar::T* M, typename Ar::T det, int count, int nm, bool& bad,
+                                   typename Ar::T& acc, double& aacc) {
+  typename Ar::T L[2 * R > 0 ? 2 * R : 1][2 * R > 0 ? 2 * R : 1];
   if constexpr (R == 0) {
-    const double t = 1.0 / sqrt(det);
-    acc += ((nm - count) & 1) ? -t : t;
-    aacc += t;
+    tor_leaf<Ar>(det, count, nm, acc, aacc);
   } else {
-    Bottom<R, 0, 0u, 0>::run(M, L, det, count, nm, bad, acc, aacc);
+    Bottom<Ar, R, 0, 0u, 0>::run(M, L, det, count, nm, bad, acc, aacc);
   }
 }
 
 // ----------------------------------------------------- cooperative helpers
 // Eliminate pivot row p of the D x D symmetric matrix X (full storage, LDS) in
 // place: X -= X[:,p] X[p,:] / X[p][p] on rows/cols != p. Returns the pivot.
-__device__ inline double eliminate(double* X, int D, int p) {
-  const double piv = X[p * D + p];
+template <class Ar>
+__device__ inline typename Ar::T eliminate(typename Ar::T* X, int D, int p) {
+  const typename Ar::T piv = X[p * D + p];
   __syncthreads();
   for (int e = threadIdx.x; e < D * D; e += 64) {
     const int i = e / D, j = e % D;
-    if (i != p && j != p) X[e] = fma(-X[i * D + p] / piv, X[p * D + j], X[e]);
+    if (i != p && j != p) X[e] = Ar::fms(Ar::div(X[i * D + p], piv), X[p * D + j], X[e]);
   }
   __syncthreads();
   return piv;
 }
 
 // Y (Dy x Dy) = X (Dx x Dx) with mode `md` (rows 2md, 2md+1) removed.
-__device__ inline void drop_mode(const double* X, int Dx, int md, double* Y) {
+template <class T>
+__device__ inline void drop_mode(const T* X, int Dx, int md, T* Y) {
   const int Dy = Dx - 2;
   for (int e = threadIdx.x; e < Dy * Dy; e += 64) {
     int i = e / Dy, j = e % Dy;
@@ -136,21 +242,30 @@ __device__ inline void drop_mode(const double* X, int Dx, int md, double* Y) {
 }
 
 // ---------------------------------------------------------------- kernel
-// workgroup = (instance b, leading pattern p1); out partials [B][2^T1]
-template <int R, bool KAPPA>
+// workgroup = (instance b, leading pattern p1); out partials [B][2^T1] (T)
+// RERUN: only workgroups whose partial is NaN (a real-path breakdown or a
+// complex O) run, overwriting it (the complex fallback of the fp64 tier).
+template <class Ar, int R, bool KAPPA, bool RERUN>
 __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ mats, int nm, int B,
-                                                      double* __restrict__ part, double* __restrict__ part_abs) {
-  extern __shared__ double lds[];
-  const TorSplit sp = tor_split(nm);
+                                                      typename Ar::P* __restrict__ part,
+                                                      double* __restrict__ part_abs) {
+  using T = typename Ar::T;
+  extern __shared__ double lds_raw[];
+  T* lds = reinterpret_cast<T*>(lds_raw);
+  const TorSplit sp = tor_split(nm, Ar::RMAX);
   const int T1 = sp.T1, T2 = sp.T2;
   const int dim = 2 * nm;
   const int lane = threadIdx.x;
   const int64_t wg = blockIdx.x;
   const int64_t b = wg >> T1;
   const unsigned p1 = (unsigned)(wg & ((1 << T1) - 1));
-  // LDS layout: work matrix (dim^2), level stack (T2+1 levels), task slots
-  double* W = lds;
-  double* lev = W + dim * dim;
+  if constexpr (RERUN) {
+    const typename Ar::P prev = part[(size_t)b * ((size_t)1 << T1) + p1];
+    if (!(prev.re != prev.re)) return;  // this workgroup's real partial is valid
+  }
+  // LDS layout: level stack (T2+1 levels), then one region used first by the
+  // work matrix (dim^2, dead once level 0 is extracted) and then by the task slots
+  T* lev = lds;
   int levoff[TOR_T2 + 2];
   {
     int o = 0;
@@ -162,8 +277,9 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
     levoff[T2 + 1] = o;
   }
   constexpr int SLOT = (2 * R) * (2 * R + 1) / 2;
-  double* slots = lev + levoff[T2 + 1];
-  double* slot_det = slots + 64 * SLOT;
+  T* W = lev + levoff[T2 + 1];
+  T* slots = W;
+  T* slot_det = slots + 64 * SLOT;
   int* slot_cnt = reinterpret_cast<int*>(slot_det + 64);
   __shared__ int bad_flag;
   if (lane == 0) bad_flag = 0;
@@ -175,21 +291,21 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
     const int r = e / dim, c = e % dim;
     const int gr = (r >> 1) + (r & 1) * nm, gc = (c >> 1) + (c & 1) * nm;
     const cplx o = O[gr * dim + gc];
-    W[e] = (r == c ? 1.0 : 0.0) - o.re;
-    if (o.im != 0.0) bad_flag = 1;  // complex O: not this kernel's domain
+    W[e] = (r == c) ? Ar::one_minus(o) : Ar::neg(o);
+    if (!Ar::domain(o)) bad_flag = 1;  // complex O: not the real kernels' domain
   }
   __syncthreads();
 
   // leading modes: eliminate the included ones (mode order, x then p pivot)
-  double det = 1.0;
+  T det = Ar::one();
   int count = 0;
   bool bad = false;
   for (int j = 0; j < T1; ++j) {
     if ((p1 >> j) & 1u) {
-      const double a = eliminate(W, dim, 2 * j);
-      const double c = eliminate(W, dim, 2 * j + 1);
-      bad |= !(a > 0.0) || !(c > 0.0);
-      det *= a * c;
+      const T a = eliminate<Ar>(W, dim, 2 * j);
+      const T c = eliminate<Ar>(W, dim, 2 * j + 1);
+      bad |= !Ar::pos(a) || !Ar::pos(c);
+      det = Ar::mul(det, Ar::mul(a, c));
       ++count;
     }
   }
@@ -204,7 +320,7 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
   }
 
   // cooperative walk over the T2 middle modes
-  double dets[TOR_T2 + 1];
+  T dets[TOR_T2 + 1];
   int cnts[TOR_T2 + 1];
   int state[TOR_T2 + 1];
   dets[0] = det;
@@ -216,7 +332,7 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
     if (d == T2) {
       // leaf: the task is this level's matrix (the trailing R modes)
       const int Dd = 2 * R;
-      const double* X = lev + levoff[d];
+      const T* X = lev + levoff[d];
       for (int e = lane; e < SLOT; e += 64) {
         int r = 0;
         while (tri(r + 1, 0) <= e) ++r;
@@ -233,8 +349,8 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
       continue;
     }
     const int Dd = 2 * (nm - T1 - d);
-    double* X = lev + levoff[d];
-    double* Y = lev + levoff[d + 1];
+    T* X = lev + levoff[d];
+    T* Y = lev + levoff[d + 1];
     if (state[d] == 0) {  // exclude middle mode d: drop its rows
       state[d] = 1;
       drop_mode(X, Dd, 0, Y);
@@ -244,25 +360,24 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
       ++d;
       continue;
     }
-    if (state[d] == 1) {  // include: eliminate rows 0, 1 into a copy
+    if (state[d] == 1) {  // include: two pivots (x then p), Schur complement into Y
       state[d] = 2;
-      // two pivots (x then p) of mode d, Schur complement formed directly in Y
-      const double a = X[0];
-      const double b01 = X[1];
-      const double c = X[Dd + 1] - b01 * b01 / a;
+      const T a = X[0];
+      const T b01 = X[1];
+      const T c = Ar::sub(X[Dd + 1], Ar::div(Ar::mul(b01, b01), a));
       const int Dy = Dd - 2;
       for (int e = lane; e < Dy * Dy; e += 64) {
         const int i = e / Dy + 2, j = e % Dy + 2;
-        const double xi0 = X[i * Dd], x0j = X[j];
-        double v = fma(-xi0 / a, x0j, X[i * Dd + j]);
-        const double xi1 = fma(-xi0 / a, b01, X[i * Dd + 1]);
-        const double x1j = fma(-b01 / a, x0j, X[Dd + j]);
-        v = fma(-xi1 / c, x1j, v);
+        const T xi0 = X[i * Dd], x0j = X[j];
+        T v = Ar::fms(Ar::div(xi0, a), x0j, X[i * Dd + j]);
+        const T xi1 = Ar::fms(Ar::div(xi0, a), b01, X[i * Dd + 1]);
+        const T x1j = Ar::fms(Ar::div(b01, a), x0j, X[Dd + j]);
+        v = Ar::fms(Ar::div(xi1, c), x1j, v);
         Y[e] = v;
       }
       __syncthreads();
-      bad |= !(a > 0.0) || !(c > 0.0);
-      dets[d + 1] = dets[d] * (a * c);
+      bad |= !Ar::pos(a) || !Ar::pos(c);
+      dets[d + 1] = Ar::mul(dets[d], Ar::mul(a, c));
       cnts[d + 1] = cnts[d] + 1;
       state[d + 1] = 0;
       ++d;
@@ -274,32 +389,69 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
   __syncthreads();
 
   // per-lane trailing walks
-  double acc = 0.0, aacc = 0.0;
+  T acc = Ar::zero();
+  double aacc = 0.0;
   if (lane < ntask) {
-    bottom_walk<R>(slots + lane * SLOT, slot_det[lane], slot_cnt[lane], nm, bad, acc, aacc);
+    bottom_walk<Ar, R>(slots + lane * SLOT, slot_det[lane], slot_cnt[lane], nm, bad, acc, aacc);
   }
   if (bad) bad_flag = 1;
-  acc = group_sum<64>(acc);
+  acc = Ar::sum64(acc);
   if (KAPPA) aacc = group_sum<64>(aacc);
   __syncthreads();
   if (lane == 0) {
     const size_t np = (size_t)1 << T1;
-    part[b * np + p1] = bad_flag ? NAN : acc;
+    part[b * np + p1] = bad_flag ? Ar::part_nan() : Ar::part(acc);
     if (KAPPA) part_abs[b * np + p1] = bad_flag ? NAN : aacc;
   }
 }
 
-template <bool KAPPA>
-__global__ void tor_reduce_kernel(const double* __restrict__ part, const double* __restrict__ part_abs, int np, int B,
-                                  cplx* __restrict__ out, double* __restrict__ aux) {
+template <bool DD, bool KAPPA>
+__global__ void tor_reduce_kernel(const void* __restrict__ partv, const double* __restrict__ part_abs, int np,
+                                  int B, cplx* __restrict__ out, double* __restrict__ aux) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
-  double s = 0.0, a = 0.0;
-  for (int i = 0; i < np; ++i) {
-    s += part[b * np + i];
-    if (KAPPA) a += part_abs[b * np + i];
+  double a = 0.0;
+  if constexpr (DD) {
+    const dd* part = reinterpret_cast<const dd*>(partv);
+    dd s{0.0, 0.0};
+    for (int i = 0; i < np; ++i) s = dd_add(s, part[b * np + i]);
+    out[b] = cplx{dd_to_d(s), 0.0};
+  } else {
+    const cplx* part = reinterpret_cast<const cplx*>(partv);
+    cplx s{0.0, 0.0};
+    for (int i = 0; i < np; ++i) {
+      s = c_add(s, part[b * np + i]);
+      if (KAPPA) a += part_abs[b * np + i];
+    }
+    out[b] = s;
   }
-  out[b] = cplx{s, 0.0};
+  if (KAPPA) aux[b] = a;
+}
+
+// complex fallback bookkeeping: every partial of an instance whose real-kernel
+// value is NaN is marked NaN (to be recomputed), the others 0 (skipped)
+__global__ void tor_mark_kernel(const cplx* __restrict__ out, int np, int B, cplx* __restrict__ cpart) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)B * np) return;
+  const cplx v = out[e / np];
+  cpart[e] = (v.re != v.re) ? cplx{NAN, NAN} : cplx{0.0, 0.0};
+}
+
+// instances the complex kernel recomputed take its fixed-order sum
+template <bool KAPPA>
+__global__ void tor_merge_kernel(const cplx* __restrict__ cpart, const double* __restrict__ cpart_abs, int np, int B,
+                                 cplx* __restrict__ out, double* __restrict__ aux) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const cplx v = out[b];
+  if (!(v.re != v.re)) return;
+  cplx s{0.0, 0.0};
+  double a = 0.0;
+  for (int i = 0; i < np; ++i) {
+    s = c_add(s, cpart[b * np + i]);
+    if (KAPPA) a += cpart_abs[b * np + i];
+  }
+  out[b] = s;
   if (KAPPA) aux[b] = a;
 }
 
@@ -310,8 +462,9 @@ __global__ void tor_n0_kernel(int B, cplx* out, double* aux) {
   if (aux) aux[b] = 1.0;
 }
 
+template <class Ar>
 static size_t tor_lds_bytes(int nm) {
-  const TorSplit sp = tor_split(nm);
+  const TorSplit sp = tor_split(nm, Ar::RMAX);
   const int dim = 2 * nm;
   size_t lev = 0;
   for (int d = 0; d <= sp.T2; ++d) {
@@ -319,21 +472,24 @@ static size_t tor_lds_bytes(int nm) {
     lev += Dd * Dd;
   }
   const size_t slot = (size_t)(2 * sp.R) * (2 * sp.R + 1) / 2;
-  return (size_t)dim * dim * 8 + lev * 8 + 64 * slot * 8 + 64 * 8 + 64 * 4;
+  const size_t T = sizeof(typename Ar::T);
+  const size_t task_area = 64 * slot * T + 64 * T + 64 * 4;
+  const size_t work = (size_t)dim * dim * T;
+  return lev * T + (work > task_area ? work : task_area);
 }
 
-template <bool KAPPA>
-static int run_tor_real(int nm, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
-  const TorSplit sp = tor_split(nm);
+template <class Ar, bool KAPPA, bool RERUN>
+static int launch_tor(int nm, const cplx* d_in, int B, typename Ar::P* part, double* part_abs, hipStream_t st) {
+  const TorSplit sp = tor_split(nm, Ar::RMAX);
   const int np = 1 << sp.T1;
-  double* part = reinterpret_cast<double*>(ws);
-  double* part_abs = part + (size_t)B * np;
-  const size_t lds = tor_lds_bytes(nm);
+  const size_t lds = tor_lds_bytes<Ar>(nm);
   const dim3 grid((unsigned)((size_t)B * np));
   switch (sp.R) {
-#define TOR_CASE(RR)                                                                                            \
-  case RR:                                                                                                      \
-    hipLaunchKernelGGL((tor_real_kernel<RR, KAPPA>), grid, dim3(64), lds, st, d_in, nm, B, part, part_abs); \
+#define TOR_CASE(RR)                                                                                        \
+  case RR:                                                                                                  \
+    if constexpr (RR <= Ar::RMAX)                                                                           \
+      hipLaunchKernelGGL((tor_real_kernel<Ar, RR, KAPPA, RERUN>), grid, dim3(64), lds, st, d_in, nm, B, part, \
+                         part_abs);                                                                         \
     break;
     TOR_CASE(0)
     TOR_CASE(1)
@@ -344,21 +500,64 @@ static int run_tor_real(int nm, const cplx* d_in, int B, cplx* d_out, double* d_
 #undef TOR_CASE
     default: return GBSK_ERR_DIM;
   }
-  hipLaunchKernelGGL((tor_reduce_kernel<KAPPA>), dim3((B + 255) / 256), dim3(256), 0, st, part, part_abs, np, B, d_out,
-                     d_aux);
+  return 0;
+}
+
+// fp64 tier: real-domain kernel; workgroups that found a complex entry or a
+// non-positive pivot are recomputed by the complex-symmetric kernel, whose split
+// is chosen so that both kernels partition the instance identically (same T1).
+template <bool KAPPA>
+static int run_tor_fp64(int nm, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
+  const TorSplit sr = tor_split(nm, RF64::RMAX), sc = tor_split(nm, RC64::RMAX);
+  const int np = 1 << (sr.T1 > sc.T1 ? sr.T1 : sc.T1);
+  (void)np;
+  cplx* part = reinterpret_cast<cplx*>(ws);
+  // the two kernels' leading splits differ (R differs); the complex rerun is
+  // therefore done per instance: the real kernel's partials are reduced first,
+  // and the complex kernel recomputes whole instances whose sum is NaN
+  const int npr = 1 << sr.T1, npc = 1 << sc.T1;
+  double* part_abs = reinterpret_cast<double*>(part + (size_t)B * npr);
+  int rc = launch_tor<RF64, KAPPA, false>(nm, d_in, B, part, part_abs, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL((tor_reduce_kernel<false, KAPPA>), dim3((B + 255) / 256), dim3(256), 0, st, part, part_abs, npr,
+                     B, d_out, d_aux);
+  cplx* cpart = reinterpret_cast<cplx*>(part_abs + (size_t)B * npr);
+  double* cpart_abs = reinterpret_cast<double*>(cpart + (size_t)B * npc);
+  hipLaunchKernelGGL(tor_mark_kernel, dim3((unsigned)(((size_t)B * npc + 255) / 256)), dim3(256), 0, st, d_out, npc,
+                     B, cpart);
+  rc = launch_tor<RC64, KAPPA, true>(nm, d_in, B, cpart, cpart_abs, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL((tor_merge_kernel<KAPPA>), dim3((B + 255) / 256), dim3(256), 0, st, cpart, cpart_abs, npc, B,
+                     d_out, d_aux);
+  return (int)hipGetLastError();
+}
+
+static int run_tor_dd(int nm, const cplx* d_in, int B, cplx* d_out, void* ws, hipStream_t st) {
+  const TorSplit sp = tor_split(nm, RDD::RMAX);
+  const int np = 1 << sp.T1;
+  dd* part = reinterpret_cast<dd*>(ws);
+  int rc = launch_tor<RDD, false, false>(nm, d_in, B, part, nullptr, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL((tor_reduce_kernel<true, false>), dim3((B + 255) / 256), dim3(256), 0, st, part, nullptr, np, B,
+                     d_out, nullptr);
   return (int)hipGetLastError();
 }
 
 size_t tor_workspace_bytes(int tier, int dim, int B) {
   if (dim < 2 || (dim & 1) || dim > 2 * TOR_MAX_MODES || B <= 0) return 0;
+  if (tier == GBSK_DD) {
+    const TorSplit sp = tor_split(dim / 2, RDD::RMAX);
+    return (size_t)B * ((size_t)1 << sp.T1) * (sizeof(dd) + sizeof(double));
+  }
   if (tier != GBSK_FP64 && tier != GBSK_KAPPA) return 0;
-  const TorSplit sp = tor_split(dim / 2);
-  return (size_t)B * ((size_t)1 << sp.T1) * 2 * sizeof(double);
+  const TorSplit sr = tor_split(dim / 2, RF64::RMAX), sc = tor_split(dim / 2, RC64::RMAX);
+  return (size_t)B * (((size_t)1 << sr.T1) + ((size_t)1 << sc.T1)) * (sizeof(cplx) + sizeof(double));
 }
 
 int tor_max_dim(int tier) {
   switch (tier) {
     case GBSK_FP64:
+    case GBSK_DD:
     case GBSK_KAPPA: return 2 * TOR_MAX_MODES;
   }
   return 0;
@@ -367,7 +566,7 @@ int tor_max_dim(int tier) {
 int tor_launch(int tier, int dim, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
   if (B <= 0) return 0;
   if (dim < 0 || (dim & 1) || dim > 2 * TOR_MAX_MODES) return GBSK_ERR_DIM;
-  if (tier != GBSK_FP64 && tier != GBSK_KAPPA) return GBSK_ERR_FUNC;
+  if (tier != GBSK_FP64 && tier != GBSK_KAPPA && tier != GBSK_DD) return GBSK_ERR_FUNC;
   const int nm = dim / 2;
   if (nm == 0) {
     hipLaunchKernelGGL(tor_n0_kernel, dim3((B + 255) / 256), dim3(256), 0, st, B, d_out,
@@ -375,8 +574,9 @@ int tor_launch(int tier, int dim, const cplx* d_in, int B, cplx* d_out, double* 
     return (int)hipGetLastError();
   }
   if (!ws) return GBSK_ERR_WORKSPACE;
-  return tier == GBSK_KAPPA ? run_tor_real<true>(nm, d_in, B, d_out, d_aux, ws, st)
-                            : run_tor_real<false>(nm, d_in, B, d_out, nullptr, ws, st);
+  if (tier == GBSK_DD) return run_tor_dd(nm, d_in, B, d_out, ws, st);
+  return tier == GBSK_KAPPA ? run_tor_fp64<true>(nm, d_in, B, d_out, d_aux, ws, st)
+                            : run_tor_fp64<false>(nm, d_in, B, d_out, nullptr, ws, st);
 }
 
 }  // namespace gbsk

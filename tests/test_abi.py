@@ -44,6 +44,7 @@ def test_pure_queries():
     assert L.gbs_is_device_build() == 1
     assert _lib.max_dim("perm", "fp64") == 28
     assert _lib.max_dim("perm", "dd") == 28
+    assert _lib.max_dim("tor", "dd") == 40
     assert L.gbs_error_string(10002).decode().startswith("matrix dimension")
     # chunked permanents need workspace for partials, unchunked ones none
     assert L.gbs_workspace_bytes(0, 0, 16, 1000) == 0

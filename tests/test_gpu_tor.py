@@ -63,10 +63,32 @@ def test_tor_zero_modes(ext):
     assert ext.tor(np.zeros((3, 0, 0)))[0] == 1.0
 
 
-def test_tor_off_domain_is_nan(ext):
-    got = ext.tor(np.stack([1.5 * np.eye(8), 0.2 * np.eye(8)]))
-    assert np.isnan(got[0].real)
+def test_tor_off_domain_falls_back_to_complex(ext, oracle):
+    """Real O with I - O_S not SPD: the real kernel refuses (non-positive pivot)
+    and the complex-symmetric kernel recomputes the instance (the reference's
+    recursive -> complex-LU fallback, core/host_api.cu:177-228)."""
+    O = np.stack([1.5 * np.eye(8), 0.2 * np.eye(8)]).astype(complex)
+    got = ext.tor(O)
+    assert got[0] == pytest.approx(oracle.tor(O[:1])[0], rel=1e-12)   # = (2 - 1)^4 = 1
+    assert got[0].real == pytest.approx(1.0, rel=1e-12)
     assert got[1].real == pytest.approx(0.25 ** 4, rel=1e-12)
+    # mixed-sign determinants take the principal branch
+    X = np.diag([1.5, 0.3, 0.3, 0.3]).astype(complex)  # mode 0: det(I - O) = -0.35
+    assert ext.tor(X[None])[0] == pytest.approx(oracle.tor(X[None])[0], rel=1e-12)
+
+
+@pytest.mark.parametrize("nm", [1, 3, 5, 7, 9])
+def test_tor_complex_symmetric_vs_oracle(ext, oracle, nm):
+    print(f"tor complex nm={nm}", flush=True)
+    g = np.random.default_rng(800 + nm)
+    X = 0.08 * (g.standard_normal((4, 2 * nm, 2 * nm)) + 1j * g.standard_normal((4, 2 * nm, 2 * nm)))
+    O = X + X.transpose(0, 2, 1)
+    got = ext.tor(O)
+    want, absn = oracle.tor(O, want_abs=True)
+    assert np.all(np.abs(got - want) <= 2 * tor_bound(absn, nm))
+    v, a = ext.tor_kappa(O)
+    assert np.array_equal(v, got)
+    assert rel_err(a, absn).max() < 1e-10
 
 
 @pytest.mark.parametrize("nm", [3, 8, 12])
@@ -99,3 +121,32 @@ def test_reference_published_checksums_recursive(ext, oracle, ref_checksums):
         got256 = float(np.sum(np.abs(ext.tor(a[:256]).real)))
         assert got256 == pytest.approx(want_dd, rel=1e-9 if dim < 16 else 1e-6)
         assert got == pytest.approx(row["checksum"], rel=1e-9 if dim < 16 else 1e-7), dim
+
+
+# ------------------------------------------------------------------- dd tier
+@pytest.mark.parametrize("nm", [1, 2, 4, 6, 8, 10, 12])
+def test_tor_dd_vs_dd_oracle(ext, oracle, nm):
+    print(f"tor dd nm={nm}", flush=True)
+    O = physical_stack(nm, 4, 500 + nm)
+    got = ext.tor_dd(O)
+    exact = oracle.tor_dd(O)  # DD subset determinants (core/torontonian_dd.cu algorithm)
+    # both are double-double evaluations of an alternating sum: agree to ~u^2 * sum|term|
+    assert np.all(np.abs(got - exact) <= 1e-13 * np.abs(exact) + 1e-27 * oracle.tor_abs(O))
+
+
+@pytest.mark.parametrize("nm", [14, 20])
+def test_tor_dd_large_vs_fp64(ext, oracle, nm):
+    O = physical_stack(nm, 2, 600 + nm)
+    got = ext.tor_dd(O).real
+    fp = ext.tor(O).real
+    _, absn = ext.tor_kappa(O)
+    assert np.all(np.abs(got - fp) <= tor_bound(absn, nm))
+
+
+def test_tor_dd_closed_form_cancellation(ext):
+    # single mode, O = a I: tor = a / (1 - a); fp64 cancels as a -> 0, dd does not
+    for a in (1e-3, 1e-8, 1e-12):
+        O = np.array([[[a, 0.0], [0.0, a]]], complex)
+        want = a / (1 - a)
+        assert ext.tor_dd(O)[0].real == pytest.approx(want, rel=1e-15)
+    assert np.isnan(ext.tor_dd(np.array([1.5 * np.eye(4)], complex))[0].real)
