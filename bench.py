@@ -35,20 +35,13 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-METRIC = "evals/s perm/haf/lhaf n=16,24 fp64+dd; served p50/p99 latency; rel err vs mpmath"
+METRIC = "evals/s perm/haf n=16,24 fp64+dd; served p50/p99 latency; rel err vs mpmath"
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
 
 
-def haar_unitary(m: int, rng: np.random.Generator) -> np.ndarray:
-    z = (rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))) / np.sqrt(2.0)
-    q, r = np.linalg.qr(z)
-    ph = np.diagonal(r).copy()
-    ph /= np.abs(ph)
-    return q * ph
-
-
 def workload():
+    from gbskernels_amd.workloads import haar_unitary
     rng = np.random.default_rng(1)
     U = haar_unitary(64, rng)
     rows16 = rng.integers(0, 64, (65536, 16))
@@ -62,6 +55,18 @@ def perm_flops(n: int) -> float:
     return float(2 ** (n - 1) * (8 * n - 4))
 
 
+def haf_flops(N: int, loop: bool = False) -> float:
+    """canonical power-trace flops of one N x N (loop) hafnian"""
+    from math import comb
+    n, f = N // 2, 0.0
+    for m in range(1, n + 1):
+        s = 2.0 * m
+        f += comb(n, m) * ((n - 1) * 8 * s ** 3 + n * 2 * s + 4 * n * (n + 1) + 4 * n + 2)
+        if loop:
+            f += comb(n, m) * n * (8 * s * s + 8 * s + 6)
+    return f
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +75,10 @@ def main() -> int:
     ap.add_argument("--dd-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--haf", type=int, default=1, help="also time haf/lhaf buckets at N=16,24 (0 skips)")
+    ap.add_argument("--served-requests", type=int, default=40000,
+                    help="requests of the served mixed-size workload (0 skips)")
+    ap.add_argument("--served-batch", type=int, default=2048, help="requests per batch / window")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (a known-size kernel that "
                          "calibrates hardware byte counters) and exit")
@@ -181,6 +190,13 @@ def main() -> int:
             per_kernel[f"perm_dd_n{n}"] = {"batch": m.shape[0], "avg_ms": dd_kern_ms[n],
                                            "evals_per_s": m.shape[0] / (dd_kern_ms[n] * 1e-3)}
 
+    if args.haf:
+        print("bench: haf/lhaf n=16,24", file=sys.stderr, flush=True)
+        per_kernel.update(haf_kernels(L, torch, dev, stream))
+    served = None
+    if args.served_requests > 0:
+        served = served_bench(args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(O, m16, m24, args.cpu_seconds)
@@ -198,9 +214,144 @@ def main() -> int:
                 "ms_per_step": dd_wall / args.dd_steps * 1e3, "steps": args.dd_steps} if dd_wall else None),
         "per_kernel": per_kernel,
         "parity": parity,
+        "served": served,
     }
     print(json.dumps(line))
     return 0
+
+
+def haf_kernels(L, torch, dev, stream):
+    """Uniform haf / lhaf buckets at N = 16 (B = 4096) and N = 24 (B = 256),
+    config-3 style complex symmetric submatrices, fp64 and dd, kernel time by
+    HIP events on the launch stream."""
+    from gbskernels_amd.workloads import gbs_adjacency, haf_subset
+    A32 = gbs_adjacency(np.random.default_rng(3))
+    rng = np.random.default_rng(7)
+    out = {}
+    for func, code in (("haf", 1), ("lhaf", 2)):
+        for N, B in ((16, 4096), (24, 256)):
+            m = np.stack([haf_subset(A32, N, rng, func == "lhaf") for _ in range(B)])
+            d_in = torch.from_numpy(m.view(np.float64)).to(dev)
+            d_out = torch.empty(2 * B, dtype=torch.float64, device=dev)
+            for tier, tname, reps in ((0, "fp64", 3), (1, "dd", 1)):
+                wsb = int(L.gbs_workspace_bytes(code, tier, N, B))
+                ws = torch.empty(max(wsb, 256), dtype=torch.uint8, device=dev)
+
+                def go():
+                    rc = L.gbs_eval_device(code, tier, d_in.data_ptr(), N, B, d_out.data_ptr(), None,
+                                           ws.data_ptr(), ws.numel(), stream.cuda_stream)
+                    if rc:
+                        raise RuntimeError(f"gbs_eval_device rc={rc}")
+                go()
+                torch.cuda.synchronize()
+                ms = []
+                for _ in range(reps):
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    go()
+                    e1.record(stream)
+                    torch.cuda.synchronize()
+                    ms.append(e0.elapsed_time(e1))
+                t = float(np.mean(ms)) * 1e-3
+                f = haf_flops(N, func == "lhaf") * B
+                out[f"{func}_{tname}_n{N}"] = {"batch": B, "avg_ms": t * 1e3, "evals_per_s": B / t,
+                                               "tflops_canonical": f / t / 1e12 * (10 if tier else 1)}
+    return out
+
+
+def served_bench(args):
+    """Served mixed-size perm/haf/lhaf (n <= 24, auto tier: fp64 with dd
+    re-evaluation when kappa >= 1e8) three ways on the same request set:
+
+    * plain: the per-size launch path -- windows of K requests in arrival
+      order, one synchronous call per (func, dim) bucket, then one per bucket of
+      escalated items (what Workspace does per bucket), inputs pre-gathered per
+      bucket outside the timed region;
+    * served, saturated: every request offered at t = 0 to the continuous-
+      batching server (max K per batch, 3 slots x 4 streams);
+    * served, Poisson 50k req/s: the open-loop stream; p50 / p99 of
+      completion - arrival.
+    Values must agree bit for bit between the paths."""
+    from gbskernels_amd import _lib
+    from gbskernels_amd.serving import Batch, Server, pack
+    from gbskernels_amd.workloads import served_mix
+
+    n_req, K = args.served_requests, args.served_batch
+    print(f"bench: served mix {n_req} requests", file=sys.stderr, flush=True)
+    items, arrivals = served_mix(n_req, seed=5)
+    func, tier, dim, flat, off = pack(items)
+    KAPPA, DD = _lib.TIERS["kappa"], _lib.TIERS["dd"]
+
+    def sub(ix, t):
+        ix = np.asarray(ix)
+        d = dim[ix]
+        m2 = d.astype(np.int64) ** 2
+        o = np.concatenate([[0], np.cumsum(m2)[:-1]]).astype(np.int64)
+        fl = np.concatenate([flat[off[i]:off[i] + m2[j]] for j, i in enumerate(ix)])
+        return (np.ascontiguousarray(func[ix]), np.full(len(ix), t, np.int32), np.ascontiguousarray(d),
+                np.ascontiguousarray(fl), o)
+
+    # plain per-size path
+    windows = []
+    for w0 in range(0, n_req, K):
+        groups = {}
+        for i in range(w0, min(n_req, w0 + K)):
+            groups.setdefault((int(func[i]), int(dim[i])), []).append(i)
+        windows.append([(np.array(ix), sub(ix, KAPPA)) for ix in groups.values()])
+    plain = np.zeros(n_req, np.complex128)
+    dd_cap = {f: _lib.lib().gbs_max_dim(f, DD) for f in range(4)}
+    escal = 0
+    with Batch(nstreams=1, use_graphs=False) as b:
+        b.run_packed(*windows[0][0][1])  # warm
+        t0 = time.perf_counter()
+        for groups in windows:
+            for ix, packed in groups:
+                v, a = b.run_packed(*packed)
+                kap = a / np.maximum(np.abs(v), 1e-300)
+                risky = (kap >= 1e8) & (dim[ix] <= dd_cap[int(func[ix[0]])])
+                if risky.any():
+                    v2, _ = b.run_packed(*sub(ix[risky], DD))
+                    v = v.copy()
+                    v[risky] = v2
+                    escal += int(risky.sum())
+                plain[ix] = v
+        t_plain = time.perf_counter() - t0
+    nb = sum(len(g) for g in windows)
+
+    auto = np.full(n_req, _lib.GBSK_AUTO, np.int32)
+
+    def serve(arr_s):
+        m = len(arr_s)
+        with Server(slots=3, streams_per_slot=4, max_batch=K) as s:
+            vals, aux, infos = s.replay_packed(func[:m], auto[:m], dim[:m], flat, off[:m], arr_s)
+            st = s.stats()
+        t_arr = np.array([i.arrival_ns for i in infos], np.int64)
+        t_done = np.array([i.done_ns for i in infos], np.int64)
+        t0 = t_arr[0] - int(round(arr_s[0] * 1e9))
+        lat = (t_done - t_arr) * 1e-9
+        return vals, {"evals_per_s": m / ((t_done.max() - t0) * 1e-9),
+                      "p50_ms": float(np.percentile(lat, 50)) * 1e3, "p99_ms": float(np.percentile(lat, 99)) * 1e3,
+                      "batches": st["batches"], "escalated": st["escalated"], "max_in_flight": st["max_in_flight"]}
+
+    serve(np.zeros(min(n_req, 2 * K)))  # warm the slots' buffers
+    sat_vals, sat = serve(np.zeros(n_req))
+    poi_vals, poi = serve(arrivals)
+    same = lambda x, y: bool(np.array_equal(x.view(np.uint64), y.view(np.uint64)))
+    plain_rate = n_req / t_plain
+    sizes = np.bincount(dim, minlength=25)
+    return {
+        "workload": f"{n_req} requests, func uniform perm/haf/lhaf, perm n = 1 + Geom(0.15) on 2..24 (rows of a "
+                    "Haar 64x64 unitary), haf/lhaf N = 2k, k ~ Geom(0.15) on 1..12 (submatrices of "
+                    "U diag(tanh r) U^T, U Haar 32x32; lhaf diagonal CN(0, 0.3^2)); tier auto (fp64, dd when "
+                    f"kappa >= 1e8); windows / max batch {K}",
+        "plain_per_size": {"evals_per_s": plain_rate, "bucket_calls": nb, "escalated": escal},
+        "served_saturated": sat,
+        "served_poisson_50k": poi,
+        "speedup_saturated_vs_plain": sat["evals_per_s"] / plain_rate,
+        "identical_values": same(sat_vals, plain) and same(poi_vals, plain),
+        "size_histogram": {int(k): int(v) for k, v in enumerate(sizes) if v},
+    }
 
 
 def cpu_baseline(O, m16, m24, budget_s):

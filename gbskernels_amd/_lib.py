@@ -25,6 +25,26 @@ GBSK_ERR_ALLOC = 10005
 
 FUNCS = {"perm": 0, "haf": 1, "lhaf": 2, "tor": 3}
 TIERS = {"fp64": 0, "dd": 1, "kappa": 2, "certified": 3, "dd_certified": 4}
+GBSK_AUTO = 5  # server-only tier: kappa, re-evaluated in dd when ill-conditioned
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_ulonglong) for k in ("batches", "items", "buckets", "graphs_captured", "graph_replays")]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("budget_bytes", ctypes.c_size_t), ("slots", ctypes.c_int), ("streams_per_slot", ctypes.c_int),
+                ("max_batch", ctypes.c_int), ("kappa_max", ctypes.c_double)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("arrival_ns", ctypes.c_longlong), ("admitted_ns", ctypes.c_longlong), ("done_ns", ctypes.c_longlong),
+                ("tier_used", ctypes.c_int), ("status", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_ulonglong)
+                for k in ("submitted", "completed", "batches", "buckets", "escalated", "max_in_flight")]
 
 _cp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -47,6 +67,17 @@ SIGNATURES = {
     "gbs_max_dim": ([_i, _i], _i),
     "gbs_error_string": ([_i], ctypes.c_char_p),
     "gbs_is_device_build": ([], _i),
+    "gbs_batch_create": ([ctypes.POINTER(_cp), ctypes.c_size_t, _i, _i], _i),
+    "gbs_batch_run": ([_cp, _i, _cp, _cp, _cp, _cp, _cp, _cp, _cp], _i),
+    "gbs_batch_get_stats": ([_cp, ctypes.POINTER(BatchStats)], _i),
+    "gbs_batch_destroy": ([_cp], _i),
+    "gbs_now_ns": ([], ctypes.c_longlong),
+    "gbs_server_create": ([ctypes.POINTER(_cp), ctypes.POINTER(ServerConfig)], _i),
+    "gbs_server_submit": ([_cp, _i, _i, _i, _cp, ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)], _i),
+    "gbs_server_wait": ([_cp, ctypes.c_longlong, _cp, _cp, _cp], _i),
+    "gbs_server_replay": ([_cp, _i, _cp, _cp, _cp, _cp, _cp, _cp, _cp, _cp, _cp], _i),
+    "gbs_server_get_stats": ([_cp, ctypes.POINTER(ServerStats)], _i),
+    "gbs_server_destroy": ([_cp], _i),
 }
 
 _lib = None

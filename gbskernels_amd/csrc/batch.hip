@@ -1,0 +1,816 @@
+// batch.hip -- heterogeneous batches: many instances of mixed function, tier
+// and size evaluated by one call, and the continuous-batching server on top.
+//
+// The reference evaluates a ragged batch one uniform bucket at a time, each a
+// synchronous host round trip (gbskernels/__init__.py:813-852 over
+// core/host_api.cu:537-590). Here a batch is planned once:
+//  * items are grouped into (function, tier, dim) buckets, longest (canonical
+//    flops) first;
+//  * inputs are packed bucket-contiguous into pinned host memory and copied to
+//    the device in one transfer;
+//  * buckets are launched on a pool of streams (event fork / join), so small and
+//    large sizes run concurrently and fill the device;
+//  * results come back in one transfer and are scattered to item order;
+//  * optionally the device part (copy in, launches, copy out) is captured into a
+//    hipGraph, cached by bucket signature, and replayed.
+// Every bucket runs the same kernels as a uniform call, so an item's value is
+// bit-identical alone, in a bucket, or in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "../../include/gbsk.h"
+#include "batch.hpp"
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace gbsk {
+
+// canonical flops of one evaluation (reference algorithm as written; the cost
+// model that orders buckets longest first)
+double canonical_flops(int func, int tier, int dim) {
+  double f = 0.0;
+  if (func == GBSK_PERM) {
+    f = dim > 0 ? std::ldexp(1.0, dim - 1) * (8.0 * dim - 4.0) : 1.0;
+  } else if (func == GBSK_HAF || func == GBSK_LHAF) {
+    const int n = dim / 2;
+    double c = 1.0;  // C(n, m)
+    for (int m = 1; m <= n; ++m) {
+      c = c * (n - m + 1) / m;
+      const double s = 2.0 * m;
+      f += c * ((n - 1) * 8.0 * s * s * s + n * 2.0 * s + 4.0 * n * (n + 1) + 4.0 * n + 2.0);
+      if (func == GBSK_LHAF) f += c * n * (8.0 * s * s + 8.0 * s + 6.0);
+    }
+  } else {
+    const int nm = dim / 2;
+    double c = 1.0;
+    for (int k = 1; k <= nm; ++k) {
+      c = c * (nm - k + 1) / k;
+      f += c * (8.0 * (2.0 * k) * (2.0 * k) + 4.0);
+    }
+    f += 3.0 * std::ldexp(1.0, nm);
+  }
+  if (tier == GBSK_DD || tier == GBSK_DD_CERTIFIED) f *= 10.0;
+  if (tier == GBSK_CERTIFIED) f *= 2.0;
+  return f;
+}
+
+struct Bucket {
+  int func, tier, dim;
+  std::vector<int> items;
+  size_t in_off = 0, out_off = 0, ws_off = 0, ws_bytes = 0;
+  double cost = 0.0;
+  int stream = 0;     // index into the owner's stream pool
+  bool done = false;  // per-bucket completion (server slots)
+};
+
+struct GraphEntry {
+  std::vector<long long> sig;
+  hipGraphExec_t exec = nullptr;
+  unsigned long long gen = 0;
+};
+
+}  // namespace gbsk
+
+using namespace gbsk;
+
+struct gbs_batch {
+  std::mutex mu;
+  int nstreams = 4;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> joins;
+  hipEvent_t ev_in = nullptr;
+  void *d_in = nullptr, *d_out = nullptr, *d_aux = nullptr, *d_ws = nullptr;
+  size_t cap_in = 0, cap_out = 0, cap_aux = 0, cap_ws = 0;
+  cplx* h_in = nullptr;
+  cplx* h_out = nullptr;
+  double* h_aux = nullptr;
+  size_t hcap_in = 0, hcap_out = 0, hcap_aux = 0;
+  size_t budget = 0;
+  bool use_graphs = false;
+  std::vector<GraphEntry> graphs;  // small LRU keyed by bucket signature
+  unsigned long long gen = 0;
+  unsigned long long grows = 0;     // device buffer reallocations (invalidate graphs)
+  gbs_batch_stats stats{};
+  hipEvent_t ev_done = nullptr;  // completion of the submitted batch
+  std::vector<Bucket> cur;       // plan of the submitted batch
+  int cur_n = 0;
+  bool per_bucket = false;              // server slot: per-bucket copies back and events
+  std::vector<hipEvent_t> bucket_ev;    // grow-only pool, one per bucket of the plan
+};
+
+namespace gbsk {
+
+static int grow_dev(void** p, size_t* cap, size_t need, unsigned long long* grows) {
+  if (need <= *cap) return 0;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (hipMalloc(p, need) != hipSuccess) {
+    *p = nullptr;
+    return (int)hipErrorOutOfMemory;
+  }
+  *cap = need;
+  ++*grows;
+  return 0;
+}
+
+template <class T>
+static int grow_pinned(T** p, size_t* cap, size_t need_elems, unsigned long long* grows) {
+  if (need_elems <= *cap) return 0;
+  if (*p) (void)hipHostFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (hipHostMalloc((void**)p, need_elems * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+    *p = nullptr;
+    return GBSK_ERR_ALLOC;
+  }
+  *cap = need_elems;
+  ++*grows;
+  return 0;
+}
+
+static void drop_graphs(gbs_batch* b) {
+  for (auto& g : b->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  b->graphs.clear();
+}
+
+// plan: buckets (longest first) with packed offsets; returns bytes needed
+// Buckets are ordered longest first (lpt: the makespan of one batch) or shortest
+// first (!lpt: mean completion time of a served batch) and each is given to the
+// least-loaded of nstreams streams.
+static int plan(int n, const int* func, const int* tier, const int* dim, std::vector<Bucket>& out, size_t* in_elems,
+                size_t* ws_total, bool lpt, int nstreams) {
+  std::map<long long, int> index;
+  out.clear();
+  for (int i = 0; i < n; ++i) {
+    if (func[i] < GBSK_PERM || func[i] > GBSK_TOR || tier[i] < GBSK_FP64 || tier[i] > GBSK_DD_CERTIFIED)
+      return GBSK_ERR_FUNC;
+    const int cap = gbs_max_dim(func[i], tier[i]);
+    if (cap == 0) return GBSK_ERR_FUNC;  // tier not built for this function
+    if (dim[i] < 0 || dim[i] > cap || (func[i] == GBSK_TOR && (dim[i] & 1))) return GBSK_ERR_DIM;
+    const long long key = ((long long)func[i] << 40) | ((long long)tier[i] << 32) | (long long)dim[i];
+    auto it = index.find(key);
+    if (it == index.end()) {
+      index.emplace(key, (int)out.size());
+      Bucket bk;
+      bk.func = func[i];
+      bk.tier = tier[i];
+      bk.dim = dim[i];
+      out.push_back(bk);
+      out.back().items.push_back(i);
+    } else {
+      out[it->second].items.push_back(i);
+    }
+  }
+  for (auto& bk : out) bk.cost = canonical_flops(bk.func, bk.tier, bk.dim) * (double)bk.items.size();
+  if (lpt)
+    std::stable_sort(out.begin(), out.end(), [](const Bucket& a, const Bucket& c) { return a.cost > c.cost; });
+  else
+    std::stable_sort(out.begin(), out.end(), [](const Bucket& a, const Bucket& c) { return a.cost < c.cost; });
+  std::vector<double> load((size_t)(nstreams < 1 ? 1 : nstreams), 0.0);
+  size_t ie = 0, oe = 0, ws = 0;
+  for (auto& bk : out) {
+    const size_t k = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+    bk.stream = (int)k;
+    load[k] += bk.cost;
+    bk.in_off = ie;
+    bk.out_off = oe;
+    ie += bk.items.size() * (size_t)bk.dim * bk.dim;
+    oe += bk.items.size();
+    bk.ws_bytes = gbs_workspace_bytes(bk.func, bk.tier, bk.dim, (int)bk.items.size());
+    bk.ws_off = ws;
+    ws += (bk.ws_bytes + 255) & ~(size_t)255;
+  }
+  *in_elems = ie;
+  *ws_total = ws;
+  return 0;
+}
+
+// tiers whose kernels write a per-element double (kappa sum|term| or bound)
+static bool tier_has_aux(int tier) { return tier == GBSK_KAPPA || tier == GBSK_CERTIFIED || tier == GBSK_DD_CERTIFIED; }
+
+// device bytes a batch occupies (inputs, outputs, auxiliaries, workspace)
+static size_t batch_bytes(size_t in_elems, size_t n, size_t ws_total) {
+  return in_elems * sizeof(cplx) + n * (sizeof(cplx) + sizeof(double)) + ws_total;
+}
+
+static int enqueue_device(gbs_batch* b, const std::vector<Bucket>& bks, size_t in_elems, size_t n) {
+  hipStream_t s0 = b->streams[0];
+  int rc = 0;
+  if (in_elems) rc = (int)hipMemcpyAsync(b->d_in, b->h_in, in_elems * sizeof(cplx), hipMemcpyHostToDevice, s0);
+  if (!rc) rc = (int)hipEventRecord(b->ev_in, s0);
+  std::vector<char> waited((size_t)b->nstreams, 0);
+  waited[0] = 1;
+  for (size_t k = 0; !rc && k < bks.size(); ++k) {
+    const Bucket& bk = bks[k];
+    hipStream_t s = b->streams[bk.stream];
+    if (!waited[bk.stream]) {
+      rc = (int)hipStreamWaitEvent(s, b->ev_in, 0);
+      waited[bk.stream] = 1;
+    }
+    if (!rc)
+      rc = gbs_eval_device(bk.func, bk.tier, (const gbs_complex*)((cplx*)b->d_in + bk.in_off), bk.dim,
+                           (int)bk.items.size(), (gbs_complex*)((cplx*)b->d_out + bk.out_off),
+                           (double*)b->d_aux + bk.out_off, (char*)b->d_ws + bk.ws_off, bk.ws_bytes, (void*)s);
+    if (!rc && b->per_bucket) {
+      // this bucket's results come back on its own stream, then its event fires
+      const size_t cnt = bk.items.size();
+      rc = (int)hipMemcpyAsync(b->h_out + bk.out_off, (cplx*)b->d_out + bk.out_off, cnt * sizeof(cplx),
+                               hipMemcpyDeviceToHost, s);
+      if (!rc)
+        rc = (int)hipMemcpyAsync(b->h_aux + bk.out_off, (double*)b->d_aux + bk.out_off, cnt * sizeof(double),
+                                 hipMemcpyDeviceToHost, s);
+      if (!rc) rc = (int)hipEventRecord(b->bucket_ev[k], s);
+    }
+  }
+  if (b->per_bucket) return rc;
+  // join only the streams this batch forked (under capture an unforked stream
+  // is not part of the graph)
+  for (int k = 1; !rc && k < b->nstreams; ++k) {
+    if (!waited[k]) continue;
+    rc = (int)hipEventRecord(b->joins[k], b->streams[k]);
+    if (!rc) rc = (int)hipStreamWaitEvent(s0, b->joins[k], 0);
+  }
+  if (!rc) rc = (int)hipMemcpyAsync(b->h_out, b->d_out, n * sizeof(cplx), hipMemcpyDeviceToHost, s0);
+  if (!rc) rc = (int)hipMemcpyAsync(b->h_aux, b->d_aux, n * sizeof(double), hipMemcpyDeviceToHost, s0);
+  return rc;
+}
+
+// Plan, pack and queue one batch on b's streams; completion is b->ev_done.
+// ptrs[i] is item i's matrix (host memory, dim[i]^2 entries).
+static int batch_submit(gbs_batch* b, int n, const int* func, const int* tier, const int* dim,
+                        const cplx* const* ptrs) {
+  b->cur.clear();
+  b->cur_n = 0;
+  if (n <= 0) return 0;
+  size_t in_elems = 0, ws_total = 0;
+  int rc = plan(n, func, tier, dim, b->cur, &in_elems, &ws_total, !b->per_bucket, b->nstreams);
+  if (rc) return rc;
+  const size_t need = batch_bytes(in_elems, (size_t)n, ws_total);
+  if (b->budget && need > b->budget) return GBSK_ERR_WORKSPACE;
+  // grow-only buffers; any reallocation changes addresses a captured graph
+  // refers to, so cached graphs are dropped
+  const unsigned long long g0 = b->grows;
+  rc = grow_dev(&b->d_in, &b->cap_in, in_elems * sizeof(cplx) + 16, &b->grows);
+  if (!rc) rc = grow_dev(&b->d_out, &b->cap_out, (size_t)n * sizeof(cplx), &b->grows);
+  if (!rc) rc = grow_dev(&b->d_aux, &b->cap_aux, (size_t)n * sizeof(double), &b->grows);
+  if (!rc) rc = grow_dev(&b->d_ws, &b->cap_ws, ws_total + 256, &b->grows);
+  if (!rc) rc = grow_pinned(&b->h_in, &b->hcap_in, in_elems + 1, &b->grows);
+  if (!rc) rc = grow_pinned(&b->h_out, &b->hcap_out, (size_t)n, &b->grows);
+  if (!rc) rc = grow_pinned(&b->h_aux, &b->hcap_aux, (size_t)n, &b->grows);
+  if (rc) return rc;
+  if (b->grows != g0) drop_graphs(b);
+  const std::vector<Bucket>& bks = b->cur;
+  if (b->per_bucket)
+    while (b->bucket_ev.size() < bks.size()) {
+      hipEvent_t e = nullptr;
+      rc = (int)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+      if (rc) return rc;
+      b->bucket_ev.push_back(e);
+    }
+  // pack bucket-contiguous
+  for (const Bucket& bk : bks) {
+    const size_t m2 = (size_t)bk.dim * bk.dim;
+    cplx* dst = b->h_in + bk.in_off;
+    for (size_t j = 0; j < bk.items.size(); ++j) std::memcpy(dst + j * m2, ptrs[bk.items[j]], m2 * sizeof(cplx));
+  }
+  // device part: replay a captured graph of the same signature, else launch
+  bool done = false;
+  if (b->use_graphs && !b->per_bucket) {
+    std::vector<long long> sig;
+    sig.reserve(bks.size() * 4 + 1);
+    sig.push_back((long long)in_elems);
+    for (const Bucket& bk : bks) {
+      sig.push_back(bk.func);
+      sig.push_back(bk.tier);
+      sig.push_back(bk.dim);
+      sig.push_back((long long)bk.items.size());
+    }
+    GraphEntry* hit = nullptr;
+    for (auto& g : b->graphs)
+      if (g.sig == sig) hit = &g;
+    if (!hit) {
+      hipGraph_t graph = nullptr;
+      rc = (int)hipStreamBeginCapture(b->streams[0], hipStreamCaptureModeThreadLocal);
+      if (!rc) {
+        int rc2 = enqueue_device(b, bks, in_elems, (size_t)n);
+        rc = (int)hipStreamEndCapture(b->streams[0], &graph);
+        if (rc2) rc = rc2;
+      }
+      GraphEntry ge;
+      ge.sig = sig;
+      if (!rc) rc = (int)hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
+      if (graph) (void)hipGraphDestroy(graph);
+      if (rc) return rc;
+      if (b->graphs.size() >= 16) {
+        auto oldest = std::min_element(b->graphs.begin(), b->graphs.end(),
+                                       [](const GraphEntry& x, const GraphEntry& y) { return x.gen < y.gen; });
+        (void)hipGraphExecDestroy(oldest->exec);
+        b->graphs.erase(oldest);
+      }
+      b->graphs.push_back(ge);
+      hit = &b->graphs.back();
+      ++b->stats.graphs_captured;
+    } else {
+      ++b->stats.graph_replays;
+    }
+    hit->gen = ++b->gen;
+    rc = (int)hipGraphLaunch(hit->exec, b->streams[0]);
+    done = true;
+  }
+  if (!done && !rc) rc = enqueue_device(b, bks, in_elems, (size_t)n);
+  if (!rc && !b->per_bucket) rc = (int)hipEventRecord(b->ev_done, b->streams[0]);
+  if (!rc) b->cur_n = n;
+  return rc;
+}
+
+// Wait for the submitted batch and scatter its results to item order.
+static int batch_complete(gbs_batch* b, gbs_complex* out, double* aux) {
+  if (b->cur_n == 0) return 0;
+  int rc = (int)hipEventSynchronize(b->ev_done);
+  if (rc) return rc;
+  for (const Bucket& bk : b->cur) {
+    const bool has_aux = tier_has_aux(bk.tier);
+    for (size_t j = 0; j < bk.items.size(); ++j) {
+      out[bk.items[j]] = ((gbs_complex*)b->h_out)[bk.out_off + j];
+      if (aux) aux[bk.items[j]] = has_aux ? b->h_aux[bk.out_off + j] : 0.0;
+    }
+  }
+  ++b->stats.batches;
+  b->stats.items += (unsigned long long)b->cur_n;
+  b->stats.buckets += (unsigned long long)b->cur.size();
+  b->cur_n = 0;
+  return 0;
+}
+
+static int run_batch(gbs_batch* b, int n, const int* func, const int* tier, const int* dim, const gbs_complex* in,
+                     const long long* off, gbs_complex* out, double* aux) {
+  if (n <= 0) return 0;
+  if (!func || !tier || !dim || !in || !off || !out) return GBSK_ERR_ARG;
+  std::vector<const cplx*> ptrs((size_t)n);
+  for (int i = 0; i < n; ++i) ptrs[i] = (const cplx*)in + off[i];
+  int rc = batch_submit(b, n, func, tier, dim, ptrs.data());
+  if (!rc) rc = batch_complete(b, out, aux);
+  if (rc) (void)hipStreamSynchronize(b->streams[0]);
+  return rc;
+}
+
+}  // namespace gbsk
+
+// ---------------------------------------------------------------------------
+// continuous-batching server
+
+namespace gbsk {
+
+static long long now_ns() {
+  return (long long)std::chrono::duration_cast<std::chrono::nanoseconds>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+struct Req {
+  int func = 0, tier = 0, dim = 0;
+  int run_tier = 0;  // tier of the pass being evaluated
+  std::vector<cplx> a;
+  bool real_input = true;
+  long long arrival = 0, admitted = 0, done = 0;
+  gbs_complex out{0.0, 0.0};
+  double aux = 0.0;
+  int tier_used = 0, status = 0;
+  bool finished = false;
+};
+
+struct Slot {
+  gbs_batch* b = nullptr;
+  std::vector<Req*> reqs;  // local item index -> request
+  size_t reserved = 0;     // admitted device bytes
+  size_t remaining = 0;    // buckets not yet complete
+  bool busy = false;
+};
+
+// device bytes one more item of (func, tier, dim) adds to a bucket of c items
+static size_t item_bytes(int func, int tier, int dim, int c) {
+  const size_t w1 = gbs_workspace_bytes(func, tier, dim, c + 1), w0 = c ? gbs_workspace_bytes(func, tier, dim, c) : 0;
+  return (size_t)dim * dim * sizeof(cplx) + sizeof(cplx) + sizeof(double) + (w1 > w0 ? w1 - w0 : 0) + (c ? 0 : 256);
+}
+
+}  // namespace gbsk
+
+struct gbs_server {
+  gbs_server_config cfg{};
+  int device = 0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Req*> pending;
+  std::map<long long, Req*> table;
+  long long next_ticket = 1;
+  std::vector<Slot> slots;
+  size_t in_use = 0;
+  bool stop = false;
+  std::thread th;
+  gbs_server_stats stats{};
+  unsigned long long in_flight = 0;
+};
+
+namespace gbsk {
+
+static void finish_req(Req* r, long long t) {
+  r->done = t;
+  r->finished = true;
+}
+
+// Completes bucket k of a slot: results to the requests, or an auto pass that
+// needs double-double back to the queue (front). Called with s->mu held.
+static void complete_bucket(gbs_server* s, Slot& sl, Bucket& bk, int err, std::vector<Req*>& requeue) {
+  const long long t = now_ns();
+  for (size_t j = 0; j < bk.items.size(); ++j) {
+    Req* r = sl.reqs[bk.items[j]];
+    if (err) {
+      r->status = err;
+      finish_req(r, t);
+      continue;
+    }
+    const gbs_complex v = ((gbs_complex*)sl.b->h_out)[bk.out_off + j];
+    const double ax = sl.b->h_aux[bk.out_off + j];
+    if (r->tier == GBSK_AUTO && r->run_tier == GBSK_KAPPA) {
+      r->out = v;
+      r->aux = ax;
+      r->tier_used = GBSK_KAPPA;
+      // kappa = sum|term| / max(|value|, 1e-300) >= kappa_max, as the auto tier
+      const double kappa = ax / std::max(std::hypot(v.re, v.im), 1e-300);
+      const bool risky = kappa >= s->cfg.kappa_max;
+      const bool dd_ok = r->dim <= gbs_max_dim(r->func, GBSK_DD) && (r->func != GBSK_TOR || r->real_input);
+      if (risky && dd_ok) {
+        r->run_tier = GBSK_DD;
+        requeue.push_back(r);
+        ++s->stats.escalated;
+        continue;
+      }
+      finish_req(r, t);
+      continue;
+    }
+    r->out = v;
+    if (r->tier != GBSK_AUTO) {
+      r->aux = tier_has_aux(r->tier) ? ax : 0.0;
+      r->tier_used = r->tier;
+    } else {
+      r->tier_used = GBSK_DD;  // aux keeps the first pass's sum|term|
+    }
+    finish_req(r, t);
+  }
+  bk.done = true;
+}
+
+static void dispatcher(gbs_server* s) {
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    bool progress = false;
+    bool any_busy = false;
+    // 1. completions
+    for (Slot& sl : s->slots) {
+      if (!sl.busy) continue;
+      std::vector<Req*> requeue;
+      bool fin = false;
+      for (size_t k = 0; k < sl.b->cur.size(); ++k) {
+        Bucket& bk = sl.b->cur[k];
+        if (bk.done) continue;
+        const hipError_t q = hipEventQuery(sl.b->bucket_ev[k]);
+        if (q == hipErrorNotReady) continue;
+        complete_bucket(s, sl, bk, q == hipSuccess ? 0 : (int)q, requeue);
+        fin = true;
+        --sl.remaining;
+      }
+      for (auto it = requeue.rbegin(); it != requeue.rend(); ++it) s->pending.push_front(*it);
+      if (fin) {
+        progress = true;
+        s->cv_done.notify_all();
+      }
+      if (sl.remaining == 0) {
+        sl.busy = false;
+        s->in_use -= sl.reserved;
+        sl.reserved = 0;
+        sl.reqs.clear();
+      } else {
+        any_busy = true;
+      }
+    }
+    // 2. admission: everything pending that fits, into each free slot
+    for (Slot& sl : s->slots) {
+      if (sl.busy || s->pending.empty()) continue;
+      std::vector<Req*> take;
+      std::map<long long, int> counts;
+      size_t bytes = 0;
+      const size_t avail = s->cfg.budget_bytes ? s->cfg.budget_bytes - s->in_use : (size_t)-1;
+      while (!s->pending.empty() && (s->cfg.max_batch <= 0 || (int)take.size() < s->cfg.max_batch)) {
+        Req* r = s->pending.front();
+        const long long key = ((long long)r->func << 40) | ((long long)r->run_tier << 32) | (long long)r->dim;
+        int& c = counts[key];
+        const size_t add = item_bytes(r->func, r->run_tier, r->dim, c);
+        if (bytes + add > avail) {
+          if (take.empty() && s->in_use == 0) {  // can never fit
+            s->pending.pop_front();
+            r->status = GBSK_ERR_WORKSPACE;
+            finish_req(r, now_ns());
+            s->cv_done.notify_all();
+            progress = true;
+            continue;
+          }
+          break;
+        }
+        bytes += add;
+        ++c;
+        take.push_back(r);
+        s->pending.pop_front();
+      }
+      if (take.empty()) continue;
+      const int n = (int)take.size();
+      std::vector<int> fn(n), tr(n), dm(n);
+      std::vector<const cplx*> ptrs(n);
+      const long long t = now_ns();
+      for (int i = 0; i < n; ++i) {
+        fn[i] = take[i]->func;
+        tr[i] = take[i]->run_tier;
+        dm[i] = take[i]->dim;
+        ptrs[i] = take[i]->a.data();
+        if (!take[i]->admitted) take[i]->admitted = t;
+      }
+      sl.reqs = take;
+      sl.reserved = bytes;
+      s->in_use += bytes;
+      s->in_flight += (unsigned long long)n;
+      lk.unlock();  // pack and launch without blocking submitters
+      const int rc = batch_submit(sl.b, n, fn.data(), tr.data(), dm.data(), ptrs.data());
+      lk.lock();
+      progress = true;
+      any_busy = true;
+      sl.busy = true;
+      ++s->stats.batches;
+      if (rc) {
+        // nothing of a failed submission is awaited: fail its requests
+        for (hipStream_t st : sl.b->streams) (void)hipStreamSynchronize(st);
+        for (Req* r : take) {
+          r->status = rc;
+          finish_req(r, now_ns());
+        }
+        for (auto& bk : sl.b->cur) bk.done = true;
+        sl.remaining = 0;
+        s->cv_done.notify_all();
+        sl.busy = false;
+        s->in_use -= sl.reserved;
+        sl.reserved = 0;
+        sl.reqs.clear();
+      } else {
+        sl.remaining = sl.b->cur.size();
+        s->stats.buckets += (unsigned long long)sl.remaining;
+      }
+      unsigned long long inf = 0;
+      for (const Slot& o : s->slots)
+        if (o.busy) inf += (unsigned long long)o.reqs.size();
+      if (inf > s->stats.max_in_flight) s->stats.max_in_flight = inf;
+    }
+    if (s->stop && s->pending.empty() && !any_busy) break;
+    if (!progress) {
+      if (!any_busy) {
+        s->cv_work.wait(lk, [s] { return s->stop || !s->pending.empty(); });
+      } else {
+        lk.unlock();
+        std::this_thread::yield();
+        lk.lock();
+      }
+    }
+  }
+}
+
+}  // namespace gbsk
+
+extern "C" {
+
+int gbs_batch_create(gbs_batch** out, size_t budget_bytes, int nstreams, int use_graphs) {
+  if (!out) return GBSK_ERR_ARG;
+  *out = nullptr;
+  gbs_batch* b = new (std::nothrow) gbs_batch();
+  if (!b) return GBSK_ERR_ALLOC;
+  b->nstreams = nstreams < 1 ? 1 : (nstreams > 16 ? 16 : nstreams);
+  b->budget = budget_bytes;
+  b->use_graphs = use_graphs != 0;
+  b->streams.resize(b->nstreams, nullptr);
+  b->joins.resize(b->nstreams, nullptr);
+  int rc = 0;
+  for (int k = 0; !rc && k < b->nstreams; ++k) {
+    rc = (int)hipStreamCreateWithFlags(&b->streams[k], hipStreamNonBlocking);
+    if (!rc) rc = (int)hipEventCreateWithFlags(&b->joins[k], hipEventDisableTiming);
+  }
+  if (!rc) rc = (int)hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming);
+  if (!rc) rc = (int)hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming);
+  if (rc) {
+    gbs_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+int gbs_batch_run(gbs_batch* b, int n, const int* func, const int* tier, const int* dim, const gbs_complex* in,
+                  const long long* off, gbs_complex* out, double* aux) {
+  if (!b) return GBSK_ERR_ARG;
+  std::lock_guard<std::mutex> g(b->mu);
+  return run_batch(b, n, func, tier, dim, in, off, out, aux);
+}
+
+int gbs_batch_get_stats(gbs_batch* b, gbs_batch_stats* s) {
+  if (!b || !s) return GBSK_ERR_ARG;
+  std::lock_guard<std::mutex> g(b->mu);
+  *s = b->stats;
+  return 0;
+}
+
+int gbs_batch_destroy(gbs_batch* b) {
+  if (!b) return 0;
+  {
+    std::lock_guard<std::mutex> g(b->mu);
+    for (auto s : b->streams)
+      if (s) (void)hipStreamSynchronize(s);
+    drop_graphs(b);
+    if (b->d_in) (void)hipFree(b->d_in);
+    if (b->d_out) (void)hipFree(b->d_out);
+    if (b->d_aux) (void)hipFree(b->d_aux);
+    if (b->d_ws) (void)hipFree(b->d_ws);
+    if (b->h_in) (void)hipHostFree(b->h_in);
+    if (b->h_out) (void)hipHostFree(b->h_out);
+    if (b->h_aux) (void)hipHostFree(b->h_aux);
+    for (auto e : b->joins)
+      if (e) (void)hipEventDestroy(e);
+    if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+    if (b->ev_done) (void)hipEventDestroy(b->ev_done);
+    for (auto e : b->bucket_ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto s : b->streams)
+      if (s) (void)hipStreamDestroy(s);
+  }
+  delete b;
+  return 0;
+}
+
+long long gbs_now_ns(void) { return now_ns(); }
+
+int gbs_server_create(gbs_server** out, const gbs_server_config* cfg) {
+  if (!out) return GBSK_ERR_ARG;
+  *out = nullptr;
+  gbs_server_config c{};
+  if (cfg) c = *cfg;
+  if (c.slots < 1) c.slots = 1;
+  if (c.slots > 8) c.slots = 8;
+  if (c.streams_per_slot < 1) c.streams_per_slot = 1;
+  if (c.streams_per_slot > 16) c.streams_per_slot = 16;
+  if (!(c.kappa_max > 0.0)) c.kappa_max = 1e8;
+  gbs_server* s = new (std::nothrow) gbs_server();
+  if (!s) return GBSK_ERR_ALLOC;
+  s->cfg = c;
+  int rc = (int)hipGetDevice(&s->device);
+  s->slots.resize((size_t)c.slots);
+  for (size_t k = 0; !rc && k < s->slots.size(); ++k) {
+    rc = gbs_batch_create(&s->slots[k].b, 0, c.streams_per_slot, 0);
+    if (!rc) s->slots[k].b->per_bucket = true;
+  }
+  if (!rc) {
+    try {
+      s->th = std::thread(dispatcher, s);
+    } catch (...) {
+      rc = GBSK_ERR_ALLOC;
+    }
+  }
+  if (rc) {
+    for (auto& sl : s->slots)
+      if (sl.b) gbs_batch_destroy(sl.b);
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return 0;
+}
+
+int gbs_server_submit(gbs_server* s, int func, int tier, int dim, const gbs_complex* in, long long arrival_ns,
+                      long long* ticket) {
+  if (!s || !ticket || (!in && dim > 0)) return GBSK_ERR_ARG;
+  if (func < GBSK_PERM || func > GBSK_TOR || tier < GBSK_FP64 || tier > GBSK_AUTO) return GBSK_ERR_FUNC;
+  const int run = tier == GBSK_AUTO ? GBSK_KAPPA : tier;
+  const int cap = gbs_max_dim(func, run);
+  if (cap == 0) return GBSK_ERR_FUNC;
+  if (dim < 0 || dim > cap || (func == GBSK_TOR && (dim & 1))) return GBSK_ERR_DIM;
+  Req* r = new (std::nothrow) Req();
+  if (!r) return GBSK_ERR_ALLOC;
+  r->func = func;
+  r->tier = tier;
+  r->run_tier = run;
+  r->dim = dim;
+  r->arrival = arrival_ns > 0 ? arrival_ns : now_ns();
+  const size_t m2 = (size_t)dim * dim;
+  try {
+    r->a.resize(m2);
+  } catch (...) {
+    delete r;
+    return GBSK_ERR_ALLOC;
+  }
+  if (m2) std::memcpy(r->a.data(), in, m2 * sizeof(cplx));
+  for (size_t e = 0; e < m2 && r->real_input; ++e) r->real_input = r->a[e].im == 0.0;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    if (s->stop) {
+      delete r;
+      return GBSK_ERR_ARG;
+    }
+    *ticket = s->next_ticket++;
+    s->table.emplace(*ticket, r);
+    s->pending.push_back(r);
+    ++s->stats.submitted;
+  }
+  s->cv_work.notify_one();
+  return 0;
+}
+
+int gbs_server_wait(gbs_server* s, long long ticket, gbs_complex* out, double* aux, gbs_request_info* info) {
+  if (!s || !out) return GBSK_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->table.find(ticket);
+  if (it == s->table.end()) return GBSK_ERR_ARG;
+  Req* r = it->second;
+  s->cv_done.wait(lk, [r] { return r->finished; });
+  *out = r->out;
+  if (aux) *aux = r->aux;
+  if (info) {
+    info->arrival_ns = r->arrival;
+    info->admitted_ns = r->admitted;
+    info->done_ns = r->done;
+    info->tier_used = r->tier_used;
+    info->status = r->status;
+  }
+  const int st = r->status;
+  s->table.erase(it);
+  ++s->stats.completed;
+  delete r;
+  return st;
+}
+
+int gbs_server_replay(gbs_server* s, int n, const int* func, const int* tier, const int* dim, const gbs_complex* in,
+                      const long long* off, const long long* arrival_offset_ns, gbs_complex* out, double* aux,
+                      gbs_request_info* info) {
+  if (!s) return GBSK_ERR_ARG;
+  if (n <= 0) return 0;
+  if (!func || !tier || !dim || !in || !off || !arrival_offset_ns || !out) return GBSK_ERR_ARG;
+  std::vector<long long> tickets((size_t)n, 0);
+  int first = 0;
+  const long long t0 = now_ns();
+  int i = 0;
+  for (; i < n; ++i) {
+    const long long due = t0 + arrival_offset_ns[i];
+    while (now_ns() < due) std::this_thread::yield();
+    const int rc = gbs_server_submit(s, func[i], tier[i], dim[i], in + off[i], due, &tickets[i]);
+    if (rc) {
+      first = rc;
+      break;
+    }
+  }
+  for (int j = 0; j < i; ++j) {
+    const int rc = gbs_server_wait(s, tickets[j], out + j, aux ? aux + j : nullptr, info ? info + j : nullptr);
+    if (rc && !first) first = rc;
+  }
+  return first;
+}
+
+int gbs_server_get_stats(gbs_server* s, gbs_server_stats* st) {
+  if (!s || !st) return GBSK_ERR_ARG;
+  std::lock_guard<std::mutex> g(s->mu);
+  *st = s->stats;
+  return 0;
+}
+
+int gbs_server_destroy(gbs_server* s) {
+  if (!s) return 0;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->th.joinable()) s->th.join();
+  for (auto& sl : s->slots)
+    if (sl.b) gbs_batch_destroy(sl.b);
+  for (auto& kv : s->table) delete kv.second;
+  delete s;
+  return 0;
+}
+
+}  // extern "C"
+

@@ -209,10 +209,22 @@ __global__ __launch_bounds__(64) void perm_kernel(const cplx* __restrict__ mats,
   __shared__ cplx AT[MPW][N * N];
   const int lane = threadIdx.x;
   const int gidx = lane / LPM, sub = lane % LPM;
-  // workgroup -> (first matrix, chunk)
+  // workgroup -> (matrix group, chunk). Workgroups are dealt to the 8 XCDs in
+  // id order, so for a split matrix (W > 1) the id is decoded such that all W
+  // chunks of one group share an XCD (its L2 then fetches the matrix once).
   const int64_t wg = blockIdx.x;
-  const int64_t mat0 = (wg / W) * MPW;
-  const int chunk = (int)(wg % W);
+  int64_t grp;
+  int chunk;
+  if constexpr (W == 1) {
+    grp = wg;
+    chunk = 0;
+  } else {
+    const int64_t k = wg >> 3;
+    grp = (k / W) * 8 + (wg & 7);
+    chunk = (int)(k % W);
+  }
+  const int64_t mat0 = grp * MPW;
+  if (mat0 >= B) return;  // padding workgroup (uniform exit, before any barrier)
   // stage MPW matrices transposed into LDS
   for (int e = lane; e < MPW * N * N; e += 64) {
     const int q = e / (N * N), rc = e % (N * N);
@@ -298,7 +310,7 @@ static int launch_perm_n(const cplx* d_in, int B, cplx* d_out, double* d_aux, vo
   using C = PermCfg<N, MODE == PERM_DD ? 1 : 0>;
   constexpr int MPW = 64 / C::LPM, W = C::W;
   const int64_t groups = ((int64_t)B + MPW - 1) / MPW;
-  const int64_t grid = groups * W;
+  const int64_t grid = (W == 1) ? groups : ((groups + 7) / 8) * 8 * W;
   if (W == 1) {
     hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, d_in, B, d_out, d_aux);
   } else {

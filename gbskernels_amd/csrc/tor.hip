@@ -21,6 +21,7 @@
 //    fully unrolled prefix-Cholesky (reference algorithm) whose factor lives in
 //    VGPRs.
 // Partials: lanes -> butterfly -> one partial per workgroup -> fixed-order sum.
+#include "cert.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -33,6 +34,7 @@ constexpr int TOR_T2 = 6;          // cooperative middle modes (64 tasks)
 struct RF64 {
   using T = double;
   static constexpr int RMAX = 5;  // per-lane trailing modes
+  static constexpr bool CERT = false;
   __device__ static T zero() { return 0.0; }
   __device__ static T one() { return 1.0; }
   __device__ static T from(double x) { return x; }
@@ -56,6 +58,7 @@ struct RF64 {
 struct RDD {
   using T = dd;
   static constexpr int RMAX = 3;
+  static constexpr bool CERT = false;
   __device__ static T zero() { return dd{0.0, 0.0}; }
   __device__ static T one() { return dd{1.0, 0.0}; }
   __device__ static T from(double x) { return dd{x, 0.0}; }
@@ -105,6 +108,7 @@ __device__ inline cplx c_sqrt(cplx z) {  // principal branch
 struct RC64 {
   using T = cplx;
   static constexpr int RMAX = 3;
+  static constexpr bool CERT = false;
   __device__ static T zero() { return cplx{0.0, 0.0}; }
   __device__ static T one() { return cplx{1.0, 0.0}; }
   __device__ static T from(double x) { return cplx{x, x}; }
@@ -124,6 +128,88 @@ struct RC64 {
   using P = cplx;
   __device__ static P part(T a) { return a; }
   __device__ static P part_nan() { return cplx{NAN, NAN}; }
+};
+
+// Certified fp64: the value of every operation is exactly RF64's (same split,
+// same rounding sequence, so values are bit-identical to the fp64 tier), and a
+// running bound e >= |value - exact| travels with it. Rules, u = 2^-53, each
+// bound expression evaluated in round-to-nearest and inflated once at the end
+// (cert_finish, rounding depth D):
+//   a +- b      e = ea + eb + u|v|
+//   a * b       e = |a| eb + |b| ea + ea eb + u|v| + UFL
+//   c - a b     (one fma) e = ec + |a| eb + |b| ea + ea eb + u|v| + UFL
+//   a / b       needs |b| >= 4 eb (then |b_exact| >= |b| / 2):
+//               e = 2 (ea + (1 + u)|v| eb) / |b| + u|v| + UFL
+//   sqrt(a)     needs a >= 4 ea (so a_exact > 0):
+//               e = (1 + u) ea / v + u v + UFL   (|sqrt x - sqrt y| <= |x - y| / sqrt y)
+// A failed requirement makes the bound +inf (refusal); pivots are divided by
+// or square-rooted, so a certified bound also certifies that every exact
+// pivot is positive (the real formula applies).
+struct cv {
+  double v, e;
+};
+struct RCERT {
+  using T = cv;
+  static constexpr int RMAX = RF64::RMAX;  // the fp64 tier's split, for identical values
+  static constexpr bool CERT = true;
+  __device__ static T zero() { return cv{0.0, 0.0}; }
+  __device__ static T one() { return cv{1.0, 0.0}; }
+  __device__ static T one_minus(cplx o) {
+    const double v = 1.0 - o.re;
+    return cv{v, CERT_U * fabs(v)};
+  }
+  __device__ static T neg(cplx o) { return cv{0.0 - o.re, 0.0}; }
+  __device__ static bool domain(cplx o) { return o.im == 0.0; }
+  __device__ static double mag(T a) { return a.v; }
+  __device__ static T add(T a, T b) {
+    const double v = a.v + b.v;
+    return cv{v, fma(CERT_U, fabs(v), a.e + b.e)};
+  }
+  __device__ static T sub(T a, T b) {
+    const double v = a.v - b.v;
+    return cv{v, fma(CERT_U, fabs(v), a.e + b.e)};
+  }
+  __device__ static double prod_err(double a, double ea, double b, double eb, double v) {
+    double t = fma(fabs(a), eb, CERT_UFL);
+    t = fma(fabs(b), ea, t);
+    t = fma(ea, eb, t);
+    return fma(CERT_U, fabs(v), t);
+  }
+  __device__ static T mul(T a, T b) {
+    const double v = a.v * b.v;
+    return cv{v, prod_err(a.v, a.e, b.v, b.e, v)};
+  }
+  __device__ static T fms(T a, T b, T c) {
+    const double v = fma(-a.v, b.v, c.v);
+    return cv{v, prod_err(a.v, a.e, b.v, b.e, v) + c.e};
+  }
+  __device__ static T div(T a, T b) {
+    const double v = a.v / b.v;
+    const double bb = fabs(b.v);
+    if (!(bb >= 4.0 * b.e)) return cv{v, INFINITY};
+    const double t = fma((1.0 + 2.0 * CERT_U) * fabs(v), b.e, a.e);
+    return cv{v, fma(CERT_U, fabs(v), fma(2.0, t / bb, CERT_UFL))};
+  }
+  __device__ static T sqrt_(T a) {
+    const double v = sqrt(a.v);
+    if (a.e == 0.0) return cv{v, fma(CERT_U, v, CERT_UFL)};
+    if (!(a.v >= 4.0 * a.e)) return cv{v, INFINITY};
+    return cv{v, fma(CERT_U, v, fma(1.0 + 2.0 * CERT_U, a.e / v, CERT_UFL))};
+  }
+  __device__ static bool pos(T a) { return a.v > 0.0; }
+  __device__ static T sum64(T x) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+      const double ov = shfl_xor_d(x.v, m), oe = shfl_xor_d(x.e, m);
+      x.v += ov;  // the same butterfly as group_sum<64> of the fp64 tier
+      x.e = fma(CERT_U, fabs(x.v), x.e + oe);
+    }
+    return x;
+  }
+  using P = cplx;
+  __device__ static P part(T a) { return cplx{a.v, 0.0}; }
+  __device__ static P part_nan() { return cplx{NAN, NAN}; }
+  __device__ static double bound(T a) { return a.e; }
 };
 
 struct TorSplit {
@@ -401,7 +487,11 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
   if (lane == 0) {
     const size_t np = (size_t)1 << T1;
     part[b * np + p1] = bad_flag ? Ar::part_nan() : Ar::part(acc);
-    if (KAPPA) part_abs[b * np + p1] = bad_flag ? NAN : aacc;
+    if constexpr (Ar::CERT) {
+      part_abs[b * np + p1] = bad_flag ? INFINITY : Ar::bound(acc);
+    } else if (KAPPA) {
+      part_abs[b * np + p1] = bad_flag ? NAN : aacc;
+    }
   }
 }
 
@@ -428,6 +518,23 @@ __global__ void tor_reduce_kernel(const void* __restrict__ partv, const double* 
   if (KAPPA) aux[b] = a;
 }
 
+// certified reduce: the fp64 tier's fixed-order sum of the partials (same
+// bits) with the partials' bounds plus one rounding charge per addition
+__global__ void tor_reduce_cert_kernel(const cplx* __restrict__ part, const double* __restrict__ part_e, int np,
+                                       int B, double depth, cplx* __restrict__ out, double* __restrict__ aux) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  cplx s{0.0, 0.0};
+  double e = 0.0;
+  for (int i = 0; i < np; ++i) {
+    s = c_add(s, part[b * np + i]);
+    e = cert_round(s, e + part_e[b * np + i]);
+  }
+  out[b] = s;
+  const double f = cert_finish(e, depth);
+  aux[b] = (f == f) ? f : INFINITY;
+}
+
 // complex fallback bookkeeping: every partial of an instance whose real-kernel
 // value is NaN is marked NaN (to be recomputed), the others 0 (skipped)
 __global__ void tor_mark_kernel(const cplx* __restrict__ out, int np, int B, cplx* __restrict__ cpart) {
@@ -438,7 +545,9 @@ __global__ void tor_mark_kernel(const cplx* __restrict__ out, int np, int B, cpl
 }
 
 // instances the complex kernel recomputed take its fixed-order sum
-template <bool KAPPA>
+// (certified tier: such an instance's bound is +inf -- the complex path is
+// not certified)
+template <bool KAPPA, bool CERT>
 __global__ void tor_merge_kernel(const cplx* __restrict__ cpart, const double* __restrict__ cpart_abs, int np, int B,
                                  cplx* __restrict__ out, double* __restrict__ aux) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -453,13 +562,14 @@ __global__ void tor_merge_kernel(const cplx* __restrict__ cpart, const double* _
   }
   out[b] = s;
   if (KAPPA) aux[b] = a;
+  if (CERT) aux[b] = INFINITY;
 }
 
-__global__ void tor_n0_kernel(int B, cplx* out, double* aux) {
+__global__ void tor_n0_kernel(int B, cplx* out, double* aux, double auxval) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   out[b] = cplx{1.0, 0.0};
-  if (aux) aux[b] = 1.0;
+  if (aux) aux[b] = auxval;
 }
 
 template <class Ar>
@@ -506,8 +616,14 @@ static int launch_tor(int nm, const cplx* d_in, int B, typename Ar::P* part, dou
 // fp64 tier: real-domain kernel; workgroups that found a complex entry or a
 // non-positive pivot are recomputed by the complex-symmetric kernel, whose split
 // is chosen so that both kernels partition the instance identically (same T1).
-template <bool KAPPA>
+// rounding depth of the certified bound: per mode a Schur level (<= 16 bound
+// operations), per lane the 2^R leaf sums, the butterfly and the np-term reduce
+static double tor_cert_depth(int nm, int np) { return 64.0 * nm + 64.0 + 16.0 + 2.0 * np + 64.0; }
+
+template <class ArR, bool KAPPA>
 static int run_tor_fp64(int nm, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
+  static_assert(ArR::RMAX == RF64::RMAX, "the real split must be the fp64 tier's");
+  constexpr bool CERT = ArR::CERT;
   const TorSplit sr = tor_split(nm, RF64::RMAX), sc = tor_split(nm, RC64::RMAX);
   const int np = 1 << (sr.T1 > sc.T1 ? sr.T1 : sc.T1);
   (void)np;
@@ -517,18 +633,22 @@ static int run_tor_fp64(int nm, const cplx* d_in, int B, cplx* d_out, double* d_
   // and the complex kernel recomputes whole instances whose sum is NaN
   const int npr = 1 << sr.T1, npc = 1 << sc.T1;
   double* part_abs = reinterpret_cast<double*>(part + (size_t)B * npr);
-  int rc = launch_tor<RF64, KAPPA, false>(nm, d_in, B, part, part_abs, st);
+  int rc = launch_tor<ArR, KAPPA, false>(nm, d_in, B, part, part_abs, st);
   if (rc) return rc;
-  hipLaunchKernelGGL((tor_reduce_kernel<false, KAPPA>), dim3((B + 255) / 256), dim3(256), 0, st, part, part_abs, npr,
-                     B, d_out, d_aux);
+  if constexpr (CERT)
+    hipLaunchKernelGGL(tor_reduce_cert_kernel, dim3((B + 255) / 256), dim3(256), 0, st, part, part_abs, npr, B,
+                       tor_cert_depth(nm, npr), d_out, d_aux);
+  else
+    hipLaunchKernelGGL((tor_reduce_kernel<false, KAPPA>), dim3((B + 255) / 256), dim3(256), 0, st, part, part_abs,
+                       npr, B, d_out, d_aux);
   cplx* cpart = reinterpret_cast<cplx*>(part_abs + (size_t)B * npr);
   double* cpart_abs = reinterpret_cast<double*>(cpart + (size_t)B * npc);
   hipLaunchKernelGGL(tor_mark_kernel, dim3((unsigned)(((size_t)B * npc + 255) / 256)), dim3(256), 0, st, d_out, npc,
                      B, cpart);
   rc = launch_tor<RC64, KAPPA, true>(nm, d_in, B, cpart, cpart_abs, st);
   if (rc) return rc;
-  hipLaunchKernelGGL((tor_merge_kernel<KAPPA>), dim3((B + 255) / 256), dim3(256), 0, st, cpart, cpart_abs, npc, B,
-                     d_out, d_aux);
+  hipLaunchKernelGGL((tor_merge_kernel<KAPPA, CERT>), dim3((B + 255) / 256), dim3(256), 0, st, cpart, cpart_abs, npc,
+                     B, d_out, d_aux);
   return (int)hipGetLastError();
 }
 
@@ -549,7 +669,7 @@ size_t tor_workspace_bytes(int tier, int dim, int B) {
     const TorSplit sp = tor_split(dim / 2, RDD::RMAX);
     return (size_t)B * ((size_t)1 << sp.T1) * (sizeof(dd) + sizeof(double));
   }
-  if (tier != GBSK_FP64 && tier != GBSK_KAPPA) return 0;
+  if (tier != GBSK_FP64 && tier != GBSK_KAPPA && tier != GBSK_CERTIFIED) return 0;
   const TorSplit sr = tor_split(dim / 2, RF64::RMAX), sc = tor_split(dim / 2, RC64::RMAX);
   return (size_t)B * (((size_t)1 << sr.T1) + ((size_t)1 << sc.T1)) * (sizeof(cplx) + sizeof(double));
 }
@@ -558,7 +678,8 @@ int tor_max_dim(int tier) {
   switch (tier) {
     case GBSK_FP64:
     case GBSK_DD:
-    case GBSK_KAPPA: return 2 * TOR_MAX_MODES;
+    case GBSK_KAPPA:
+    case GBSK_CERTIFIED: return 2 * TOR_MAX_MODES;
   }
   return 0;
 }
@@ -566,17 +687,19 @@ int tor_max_dim(int tier) {
 int tor_launch(int tier, int dim, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
   if (B <= 0) return 0;
   if (dim < 0 || (dim & 1) || dim > 2 * TOR_MAX_MODES) return GBSK_ERR_DIM;
-  if (tier != GBSK_FP64 && tier != GBSK_KAPPA && tier != GBSK_DD) return GBSK_ERR_FUNC;
+  if (tier != GBSK_FP64 && tier != GBSK_KAPPA && tier != GBSK_DD && tier != GBSK_CERTIFIED) return GBSK_ERR_FUNC;
   const int nm = dim / 2;
-  if (nm == 0) {
+  if (nm == 0) {  // tor of the empty matrix: exactly 1 (kappa 1, bound 0)
     hipLaunchKernelGGL(tor_n0_kernel, dim3((B + 255) / 256), dim3(256), 0, st, B, d_out,
-                       tier == GBSK_KAPPA ? d_aux : nullptr);
+                       (tier == GBSK_KAPPA || tier == GBSK_CERTIFIED) ? d_aux : nullptr,
+                       tier == GBSK_CERTIFIED ? 0.0 : 1.0);
     return (int)hipGetLastError();
   }
   if (!ws) return GBSK_ERR_WORKSPACE;
   if (tier == GBSK_DD) return run_tor_dd(nm, d_in, B, d_out, ws, st);
-  return tier == GBSK_KAPPA ? run_tor_fp64<true>(nm, d_in, B, d_out, d_aux, ws, st)
-                            : run_tor_fp64<false>(nm, d_in, B, d_out, nullptr, ws, st);
+  if (tier == GBSK_CERTIFIED) return run_tor_fp64<RCERT, false>(nm, d_in, B, d_out, d_aux, ws, st);
+  return tier == GBSK_KAPPA ? run_tor_fp64<RF64, true>(nm, d_in, B, d_out, d_aux, ws, st)
+                            : run_tor_fp64<RF64, false>(nm, d_in, B, d_out, nullptr, ws, st);
 }
 
 }  // namespace gbsk

@@ -1,0 +1,114 @@
+"""Synthetic inputs of the benchmark workloads (numpy only).
+
+The constructions follow the reference's bench inputs in behaviour
+(``bench/_inputs.py:184-196,242-249``: Haar unitaries by QR of a complex
+Gaussian with fixed phases; lossy Gaussian states) and its threshold-detector
+matrix definition (``sampling/gbs.py:241-275``: O = I - Sigma_x^-1 with
+Sigma_x = (cov + I) / 2, hbar = 2); the generators themselves are written here.
+
+* :func:`perm_rows` -- permanent inputs: n random rows (with repetition) of the
+  first n columns of a Haar-random 64 x 64 unitary;
+* :func:`gbs_adjacency` / :func:`haf_subset` -- complex symmetric
+  A = U diag(tanh r) U^T (U Haar 32 x 32, r ~ U[0.5, 1.2]) and its principal
+  submatrices; loop hafnians replace the diagonal by gamma ~ CN(0, 0.3^2);
+* :func:`threshold_O` -- torontonian inputs of two-mode squeezed pairs through a
+  Haar interferometer with loss;
+* :func:`served_mix` -- a request stream of mixed function, size and tier with
+  Poisson arrivals.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def haar_unitary(m: int, rng: np.random.Generator) -> np.ndarray:
+    z = (rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))) / np.sqrt(2.0)
+    q, r = np.linalg.qr(z)
+    ph = np.diagonal(r).copy()
+    ph /= np.abs(ph)
+    return q * ph
+
+
+def threshold_O(nm: int, rng: np.random.Generator, eta: float = 0.7, rmin: float = 0.3,
+                rmax: float = 1.0) -> np.ndarray:
+    """Real 2nm x 2nm O (xxpp) of two-mode squeezed pairs (modes 2i, 2i+1,
+    r ~ U[rmin, rmax]) through a Haar interferometer with loss eta."""
+    S = np.eye(2 * nm)
+    for i in range(0, nm - 1, 2):
+        r = rng.uniform(rmin, rmax)
+        ch, sh = np.cosh(r), np.sinh(r)
+        a, b = i, i + 1
+        T = np.eye(2 * nm)
+        T[a, a] = T[b, b] = ch
+        T[a, b] = T[b, a] = sh                      # x quadratures
+        T[nm + a, nm + a] = T[nm + b, nm + b] = ch
+        T[nm + a, nm + b] = T[nm + b, nm + a] = -sh  # p quadratures
+        S = T @ S
+    U = haar_unitary(nm, rng)
+    SU = np.block([[U.real, -U.imag], [U.imag, U.real]])
+    S = SU @ S
+    cov = eta * (S @ S.T) + (1.0 - eta) * np.eye(2 * nm)
+    sig = (cov + np.eye(2 * nm)) / 2.0
+    sig = (sig + sig.T) / 2.0
+    O = np.eye(2 * nm) - np.linalg.inv(sig)
+    return np.ascontiguousarray((O + O.T) / 2.0)
+
+
+def perm_rows(U: np.ndarray, n: int, rng: np.random.Generator) -> np.ndarray:
+    rows = rng.integers(0, U.shape[0], n)
+    return np.ascontiguousarray(U[rows][:, :n])
+
+
+def gbs_adjacency(rng: np.random.Generator, m: int = 32) -> np.ndarray:
+    U = haar_unitary(m, rng)
+    r = rng.uniform(0.5, 1.2, m)
+    return (U * np.tanh(r)) @ U.T
+
+
+def haf_subset(A: np.ndarray, N: int, rng: np.random.Generator, loop: bool) -> np.ndarray:
+    S = np.sort(rng.choice(A.shape[0], N, replace=False))
+    M = np.array(A[np.ix_(S, S)])
+    if loop:
+        M[np.diag_indices(N)] = 0.3 * (rng.standard_normal(N) + 1j * rng.standard_normal(N)) / np.sqrt(2.0)
+    return M
+
+
+def geometric_size(rng: np.random.Generator, p: float, lo: int, hi: int) -> int:
+    """lo - 1 + Geom(p) on {lo, lo+1, ...}, resampled until <= hi."""
+    while True:
+        n = lo - 1 + int(rng.geometric(p))
+        if n <= hi:
+            return n
+
+
+def served_mix(n_req: int, seed: int, funcs=("perm", "haf", "lhaf"), nmax: int = 24, p: float = 0.15,
+               tiers=(("auto", 1.0),), rate: float = 5e4):
+    """Request stream: (func, tier, matrix) items and Poisson arrival offsets (s).
+
+    func uniform over ``funcs``; perm size n = 1 + Geom(p) on 2..nmax; haf / lhaf
+    size N = 2k with k ~ Geom(p) on 1..nmax/2 (config-3 style submatrices); tor
+    mode count nm = k on 1..nmax/2; tier drawn from ``tiers`` (name, weight).
+    """
+    rng = np.random.default_rng(seed)
+    U64 = haar_unitary(64, np.random.default_rng(1))
+    A32 = gbs_adjacency(np.random.default_rng(3))
+    tor_pool = {nm: [threshold_O(nm, np.random.default_rng(4000 + 16 * nm + j)) for j in range(4)]
+                for nm in range(1, nmax // 2 + 1)}
+    names = [t for t, _ in tiers]
+    w = np.array([x for _, x in tiers], np.float64)
+    w /= w.sum()
+    items = []
+    for _ in range(n_req):
+        f = funcs[int(rng.integers(len(funcs)))]
+        t = names[int(rng.choice(len(names), p=w))]
+        if f == "perm":
+            M = perm_rows(U64, geometric_size(rng, p, 2, nmax), rng)
+        elif f in ("haf", "lhaf"):
+            M = haf_subset(A32, 2 * geometric_size(rng, p, 1, nmax // 2), rng, f == "lhaf")
+        else:
+            nm = geometric_size(rng, p, 1, nmax // 2)
+            M = tor_pool[nm][int(rng.integers(4))]
+        items.append((f, t, M))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    return items, arrivals

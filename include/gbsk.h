@@ -117,6 +117,75 @@ const char* gbs_error_string(int rc);
 /* 1 when the library was compiled for a GPU (always: there is no host build) */
 int gbs_is_device_build(void);
 
+/* ---- 5. heterogeneous batches (no reference counterpart) ------------------
+ * Replaces the per-bucket loop of Workspace._ragged / _dispatch_bucket
+ * (gbskernels/__init__.py:813-852), which pays one synchronous round trip per
+ * (function, size) bucket. Item i is a dim[i] x dim[i] matrix (tor: dim = 2n)
+ * at in + off[i] (in gbs_complex units) evaluated by func[i] at tier[i]
+ * (GBSK_FP64..GBSK_DD_CERTIFIED); out[i] receives its value and aux[i] (may be
+ * NULL) its kappa / certificate. Items are grouped into (func, tier, dim)
+ * buckets that run on `nstreams` streams after one packed host-to-device copy;
+ * with use_graphs the device part is captured into a hipGraph, cached by the
+ * bucket signature (the multiset of (func, tier, dim, count)). Every bucket runs
+ * the kernels of a uniform call, so values are bit-identical to evaluating each
+ * item alone. budget_bytes (0 = no limit) caps the device bytes of one batch:
+ * a larger batch returns GBSK_ERR_WORKSPACE. A handle is reentrant (calls are
+ * serialised on it); distinct handles run concurrently. */
+typedef struct gbs_batch gbs_batch;
+typedef struct {
+  unsigned long long batches, items, buckets, graphs_captured, graph_replays;
+} gbs_batch_stats;
+int gbs_batch_create(gbs_batch** out, size_t budget_bytes, int nstreams, int use_graphs);
+int gbs_batch_run(gbs_batch* b, int n, const int* func, const int* tier, const int* dim, const gbs_complex* in,
+                  const long long* off, gbs_complex* out, double* aux);
+int gbs_batch_get_stats(gbs_batch* b, gbs_batch_stats* st);
+int gbs_batch_destroy(gbs_batch* b);
+
+/* ---- 6. continuous-batching server (no reference counterpart) -------------
+ * Requests are submitted one at a time, from any thread, while earlier ones are
+ * on the device. A dispatcher thread forms a batch from everything pending
+ * whenever one of `slots` batch slots is free and the batch's device bytes fit
+ * in what the in-flight batches leave of budget_bytes; batches in different
+ * slots overlap on the device. Each bucket's results are copied back and
+ * completed as soon as its own kernels finish. Tier GBSK_AUTO evaluates with
+ * kappa and re-evaluates in double-double when sum|term| / |value| >=
+ * kappa_max (the reference's auto tier, gbskernels/__init__.py:240-297). The
+ * request's matrix is copied at submission. Timestamps are gbs_now_ns(). */
+#define GBSK_AUTO 5
+typedef struct gbs_server gbs_server;
+typedef struct {
+  size_t budget_bytes;  /* device bytes admitted batches may occupy together; 0 = no limit */
+  int slots;            /* batches in flight at once, 1..8 */
+  int streams_per_slot; /* 1..16 */
+  int max_batch;        /* requests per batch; 0 = no limit */
+  double kappa_max;     /* GBSK_AUTO escalation threshold (the reference uses 1e8) */
+} gbs_server_config;
+typedef struct {
+  long long arrival_ns, admitted_ns, done_ns;
+  int tier_used; /* tier of the returned value (GBSK_AUTO -> GBSK_KAPPA or GBSK_DD) */
+  int status;    /* 0, or the error that failed the request */
+} gbs_request_info;
+typedef struct {
+  unsigned long long submitted, completed, batches, buckets, escalated, max_in_flight;
+} gbs_server_stats;
+long long gbs_now_ns(void);
+int gbs_server_create(gbs_server** out, const gbs_server_config* cfg);
+/* arrival_ns <= 0: now. *ticket identifies the request in gbs_server_wait. */
+int gbs_server_submit(gbs_server* s, int func, int tier, int dim, const gbs_complex* in, long long arrival_ns,
+                      long long* ticket);
+/* blocks until the request is done; releases the ticket. aux and info may be NULL. */
+int gbs_server_wait(gbs_server* s, long long ticket, gbs_complex* out, double* aux, gbs_request_info* info);
+/* open-loop replay: request i is submitted from the calling thread at its due
+ * time t0 + arrival_offset_ns[i] (t0 = the call's start; offsets ascending),
+ * which is also its arrival_ns, however late the submission; then every result
+ * is collected. Returns when all are done. aux and info may be NULL. */
+int gbs_server_replay(gbs_server* s, int n, const int* func, const int* tier, const int* dim, const gbs_complex* in,
+                      const long long* off, const long long* arrival_offset_ns, gbs_complex* out, double* aux,
+                      gbs_request_info* info);
+int gbs_server_get_stats(gbs_server* s, gbs_server_stats* st);
+/* waits for every submitted request, then stops the dispatcher and frees everything */
+int gbs_server_destroy(gbs_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,7 +12,7 @@ ROOT = Path(__file__).resolve().parent.parent
 
 def header_symbols():
     text = (ROOT / "include" / "gbsk.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char\*)\s+(gbs_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|long long|const char\*)\s+(gbs_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_the_binding_surface():

@@ -66,7 +66,7 @@ def test_auto_tier_reruns_only_risky(oracle, func):
     assert out[0] == plain[0] and out[2] == plain[2]
 
 
-@pytest.mark.parametrize("func", ["perm", "haf", "lhaf"])
+@pytest.mark.parametrize("func", ["perm", "haf", "lhaf", "tor"])
 def test_certified_api_and_ladder(oracle, func):
     mats = np.stack([MK[func](1e-1), MK[func](1e-9)])
     out, diags = getattr(gk, f"{func}_batched")(mats, precision="certified", backend="gpu", return_diagnostics=True)
