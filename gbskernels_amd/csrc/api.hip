@@ -26,7 +26,8 @@ static_assert(sizeof(gbs_complex) == sizeof(cplx), "gbs_complex layout");
 // kernel dimension the launchers take: matrix side for perm/haf/lhaf, modes for tor
 static size_t ws_bytes(int func, int tier, int dim, int B) {
   switch (func) {
-    case GBSK_PERM: return perm_workspace_bytes(dim, tier == GBSK_DD ? PERM_DD : PERM_FP64, B);
+    case GBSK_PERM:
+      return perm_workspace_bytes(dim, tier == GBSK_DD ? PERM_DD : (tier == GBSK_DD_CERTIFIED ? PERM_DDCERT : PERM_FP64), B);
     case GBSK_HAF:
     case GBSK_LHAF: return haf_workspace_bytes(func == GBSK_LHAF, tier, dim, B);
     case GBSK_TOR: return tor_workspace_bytes(tier, dim, B);
@@ -50,6 +51,9 @@ static int launch(int func, int tier, const cplx* d_in, int dim, int B, cplx* d_
         case GBSK_CERTIFIED:
           if (!d_aux) return GBSK_ERR_ARG;
           return perm_launch(dim, PERM_CERT, d_in, B, d_out, d_aux, ws, st);
+        case GBSK_DD_CERTIFIED:
+          if (!d_aux) return GBSK_ERR_ARG;
+          return perm_launch(dim, PERM_DDCERT, d_in, B, d_out, d_aux, ws, st);
         default: return GBSK_ERR_FUNC;
       }
     case GBSK_HAF:
@@ -257,13 +261,15 @@ int gbs_eval_device(int func, int tier, const gbs_complex* d_in, int dim, int B,
                     void* d_ws, size_t ws_size, void* stream) {
   if (dim < 0) return GBSK_ERR_DIM;
   if (func == GBSK_TOR && (dim & 1)) return GBSK_ERR_DIM;
-  if (dim > gbs_max_dim(func, tier)) return GBSK_ERR_DIM;
+  const int cap = gbs_max_dim(func, tier);
+  if (cap == 0) return GBSK_ERR_FUNC;  // (func, tier) not built
+  if (dim > cap) return GBSK_ERR_DIM;
   return launch(func, tier, (const cplx*)d_in, dim, B, (cplx*)d_out, d_aux, d_ws, ws_size, (hipStream_t)stream);
 }
 
 int gbs_max_dim(int func, int tier) {
   switch (func) {
-    case GBSK_PERM: return (tier == GBSK_DD_CERTIFIED) ? 0 : PERM_MAX_N;
+    case GBSK_PERM: return (tier >= GBSK_FP64 && tier <= GBSK_DD_CERTIFIED) ? PERM_MAX_N : 0;
     case GBSK_HAF:
     case GBSK_LHAF: return haf_max_dim(func == GBSK_LHAF, tier);
     case GBSK_TOR: return tor_max_dim(tier);

@@ -69,3 +69,97 @@ __device__ inline void cert_group_sum(cplx& v, double& e) {
 }
 
 }  // namespace gbsk
+
+// ---------------------------------------------------------------------------
+// Certified double-double. Each operation returns the value of the plain dd
+// routine in common.hpp (same operations, same order, so certified-dd values
+// are bit-identical to the dd tier's) and the error of that one operation on
+// exact operands, bounded a posteriori from its own intermediates:
+//   every rounded binary64 operation x (+,-,*,fma) errs by <= u|fl(x)| + UFL;
+//   TwoSum / TwoProduct are exact (TwoProduct up to UFL);
+//   a FastTwoSum(x, y) -> r is exact unless its precondition fails, and its
+//   error is then |TwoSum(x, y).lo - r.lo| exactly (both share the high part).
+// Division and square root are certified from dd residuals (|q - a/b| =
+// |a - q b| / |b|, |y - sqrt(a)| <= |y^2 - a| / y), divided by a guaranteed
+// lower bound of the dd magnitude: fl(fl(|hi| - |lo|) (1 - 2u)) <= |hi + lo|.
+
+namespace gbsk {
+
+__device__ inline double cdd_fts_err(double x, double y, dd r) {
+  const dd t = two_sum(x, y);  // t.hi == r.hi: both round x + y
+  return (1.0 + 2.0 * CERT_U) * fabs(t.lo - r.lo);
+}
+__device__ inline double cdd_mag(dd a) { return fabs(a.hi) + fabs(a.lo); }
+// guaranteed lower bound of |a.hi + a.lo| (0 if not provably positive)
+__device__ inline double cdd_low(dd a) {
+  const double d = (fabs(a.hi) - fabs(a.lo)) * (1.0 - 2.0 * CERT_U);
+  return d > 1e-290 ? d : 0.0;
+}
+
+// r = dd_add(a, b); *e = |r - (a + b)|
+__device__ inline dd cdd_add(dd a, dd b, double* e) {
+  const dd s = two_sum(a.hi, b.hi);
+  const double t = a.lo + b.lo;
+  const double w = s.lo + t;
+  const dd r = fast_two_sum(s.hi, w);
+  *e = fma(CERT_U, fabs(t) + fabs(w), cdd_fts_err(s.hi, w, r));
+  return r;
+}
+// r = dd_mul(a, b); *e = |r - a b|
+__device__ inline dd cdd_mul(dd a, dd b, double* e) {
+  const dd p = two_prod(a.hi, b.hi);
+  const double m = a.lo * b.hi;
+  const double q = fma(a.hi, b.lo, m);
+  const double w = p.lo + q;
+  const dd r = fast_two_sum(p.hi, w);
+  const double dropped = (1.0 + 2.0 * CERT_U) * fabs(a.lo * b.lo);  // a.lo b.lo is not computed
+  *e = fma(CERT_U, fabs(m) + fabs(q) + fabs(w), dropped + cdd_fts_err(p.hi, w, r) + 4.0 * CERT_UFL);
+  return r;
+}
+
+}  // namespace gbsk
+
+namespace gbsk {
+
+// r = dd_add_d(a, b); *e = |r - (a + b)|
+__device__ inline dd cdd_add_d(dd a, double b, double* e) {
+  const dd s = two_sum(a.hi, b);
+  const double w = s.lo + a.lo;
+  const dd r = fast_two_sum(s.hi, w);
+  *e = fma(CERT_U, fabs(w), cdd_fts_err(s.hi, w, r));
+  return r;
+}
+__device__ inline double cddc_norm1(ddc a) { return cdd_mag(a.re) + cdd_mag(a.im); }
+// complex dd (1-norm errors). Values are exactly ddc_add / ddc_sub / ddc_mul /
+// ddc_add_scaled of common.hpp; *e receives the operation's own error.
+__device__ inline ddc cddc_add(ddc a, ddc b, double* e) {
+  double er, ei;
+  const ddc r{cdd_add(a.re, b.re, &er), cdd_add(a.im, b.im, &ei)};
+  *e = er + ei;
+  return r;
+}
+__device__ inline ddc cddc_sub(ddc a, ddc b, double* e) { return cddc_add(a, ddc_neg(b), e); }
+__device__ inline ddc cddc_add_scaled(ddc a, double s, cplx z, double* e) {
+  double er, ei;
+  const ddc r{cdd_add_d(a.re, s * z.re, &er), cdd_add_d(a.im, s * z.im, &ei)};
+  *e = er + ei;
+  return r;
+}
+__device__ inline ddc cddc_mul(ddc a, ddc b, double* e) {
+  double e1, e2, e3, e4, e5, e6;
+  const dd rr = cdd_mul(a.re, b.re, &e1), ii = cdd_mul(a.im, b.im, &e2);
+  const dd ri = cdd_mul(a.re, b.im, &e3), ir = cdd_mul(a.im, b.re, &e4);
+  const ddc r{cdd_add(rr, dd_neg(ii), &e5), cdd_add(ri, ir, &e6)};
+  *e = ((e1 + e2) + (e3 + e4)) + (e5 + e6);
+  return r;
+}
+// bound of |a^ b^ - a b| from operand bounds ea, eb and the product's own error
+__device__ inline double cddc_mul_bound(ddc a, double ea, ddc b, double eb, double eop) {
+  return fma(cddc_norm1(a), eb, fma(cddc_norm1(b), ea, fma(ea, eb, eop)));
+}
+// |ddc_to_c(a) - a| exactly (1-norm): the two collapse roundings
+__device__ inline double cddc_collapse_err(ddc a) {
+  return fabs(two_sum(a.re.hi, a.re.lo).lo) + fabs(two_sum(a.im.hi, a.im.lo).lo);
+}
+
+}  // namespace gbsk

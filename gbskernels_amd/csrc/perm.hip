@@ -197,6 +197,110 @@ __device__ inline void glynn_range_dd(const cplx* __restrict__ AT, uint32_t star
   acc_out = acc;
 }
 
+// glynn_range_dd with a running bound (certified double-double). The values
+// are glynn_range_dd's bit for bit. E bounds the error of every row sum (each
+// row receives the same updates, so one bound serves all rows: each step adds
+// the largest of the N update errors); products and the signed accumulation
+// propagate it with the 1-norm product rule plus each operation's own error.
+template <int N>
+__device__ inline void glynn_range_ddcert(const cplx* __restrict__ AT, uint32_t start, int M, ddc& acc_out,
+                                          double& e_out) {
+  ddc rs[N];
+  const uint32_t g = start ^ (start >> 1);
+  double E = 0.0;
+#pragma unroll
+  for (int r = 0; r < N; ++r) rs[r] = ddc_from(AT[r]);
+#pragma unroll 1
+  for (int c = 1; c < N; ++c) {
+    const double s = ((g >> (c - 1)) & 1u) ? -1.0 : 1.0;
+    double m = 0.0;
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+      double e;
+      rs[r] = cddc_add_scaled(rs[r], s, AT[c * N + r], &e);
+      m = fmax(m, e);
+    }
+    E += m;
+  }
+  auto product = [&](double& ep) {
+    ddc t0 = rs[0];
+    double e0 = E;
+#pragma unroll
+    for (int r = 2; r < N; r += 2) {
+      double eo;
+      const ddc t = cddc_mul(t0, rs[r], &eo);
+      e0 = cddc_mul_bound(t0, e0, rs[r], E, eo);
+      t0 = t;
+    }
+    if constexpr (N == 1) {
+      ep = e0;
+      return t0;
+    }
+    ddc t1 = rs[1];
+    double e1 = E;
+#pragma unroll
+    for (int r = 3; r < N; r += 2) {
+      double eo;
+      const ddc t = cddc_mul(t1, rs[r], &eo);
+      e1 = cddc_mul_bound(t1, e1, rs[r], E, eo);
+      t1 = t;
+    }
+    double eo;
+    const ddc t = cddc_mul(t0, t1, &eo);
+    ep = cddc_mul_bound(t0, e0, t1, e1, eo);
+    return t;
+  };
+  double ea;
+  ddc acc = product(ea);
+  const uint32_t nsteps = 1u << M;
+  for (uint32_t j = 1; j < nsteps; ++j) {
+    const int k = __builtin_ctz(j);
+    const uint32_t i = start + j;
+    const double step = ((i >> (k + 1)) & 1u) ? 2.0 : -2.0;
+    const cplx* col = AT + (k + 1) * N;
+    double m = 0.0;
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+      double e;
+      rs[r] = cddc_add_scaled(rs[r], step, col[r], &e);
+      m = fmax(m, e);
+    }
+    E += m;
+    double ep, eo;
+    const ddc p = product(ep);
+    acc = (j & 1u) ? cddc_sub(acc, p, &eo) : cddc_add(acc, p, &eo);
+    ea = ea + ep + eo;
+  }
+  acc_out = acc;
+  e_out = ea;
+}
+
+// butterfly sum over G lanes of (ddc value, bound): group_sum<G>(ddc)'s order
+template <int G>
+__device__ inline void cddc_group_sum(ddc& v, double& e) {
+#pragma unroll
+  for (int m = 1; m < G; m <<= 1) {
+    ddc o;
+    o.re.hi = shfl_xor_d(v.re.hi, m);
+    o.re.lo = shfl_xor_d(v.re.lo, m);
+    o.im.hi = shfl_xor_d(v.im.hi, m);
+    o.im.lo = shfl_xor_d(v.im.lo, m);
+    const double eo = shfl_xor_d(e, m);
+    const bool low = (threadIdx.x & m) == 0;
+    double r;
+    v = low ? cddc_add(v, o, &r) : cddc_add(o, v, &r);
+    e = e + eo + r;
+  }
+}
+
+// rounding depth of the certified-dd bound: the row bound's 2^M-step
+// accumulation, the product chains, the butterfly and the chunk sum
+template <int N>
+__device__ constexpr double perm_ddcert_depth() {
+  using C = PermCfg<N, 1>;
+  return 8.0 * (double)(1u << C::M) + 16.0 * N + 8.0 * C::W + 128.0;
+}
+
 // One workgroup = one wave = MPW matrices x LPM lanes (or one chunk of one
 // matrix when the matrix is split over W workgroups).
 //   out      : final values (W == 1) or chunk partials [B][W] (W > 1)
@@ -204,7 +308,7 @@ __device__ inline void glynn_range_dd(const cplx* __restrict__ AT, uint32_t star
 template <int N, int MODE>
 __global__ __launch_bounds__(64) void perm_kernel(const cplx* __restrict__ mats, int B, cplx* __restrict__ out,
                                                   double* __restrict__ aux) {
-  using C = PermCfg<N, MODE == PERM_DD ? 1 : 0>;
+  using C = PermCfg<N, perm_is_dd(MODE) ? 1 : 0>;
   constexpr int LPM = C::LPM, MPW = 64 / LPM, W = C::W, M = C::M;
   __shared__ cplx AT[MPW][N * N];
   const int lane = threadIdx.x;
@@ -235,17 +339,30 @@ __global__ __launch_bounds__(64) void perm_kernel(const cplx* __restrict__ mats,
   __syncthreads();
   const int64_t b = mat0 + gidx;
   const uint32_t start = ((uint32_t)chunk * LPM + (uint32_t)sub) << M;
-  if constexpr (MODE == PERM_DD) {
+  if constexpr (perm_is_dd(MODE)) {
+    constexpr bool DDCERT = MODE == PERM_DDCERT;
     ddc acc;
-    glynn_range_dd<N>(AT[gidx], start, M, acc);
-    acc = group_sum<LPM>(acc);
+    double ea = 0.0;
+    if constexpr (DDCERT) {
+      glynn_range_ddcert<N>(AT[gidx], start, M, acc, ea);
+      cddc_group_sum<LPM>(acc, ea);
+    } else {
+      glynn_range_dd<N>(AT[gidx], start, M, acc);
+      acc = group_sum<LPM>(acc);
+    }
     if (sub == 0 && b < B) {
       if constexpr (W == 1) {
-        out[b] = ddc_to_c(ddc_mul_d(acc, 1.0 / (double)(1ull << (N - 1))));
+        const double scale = 1.0 / (double)(1ull << (N - 1));
+        const ddc sc = ddc_mul_d(acc, scale);  // exact (power of two)
+        out[b] = ddc_to_c(sc);
+        if constexpr (DDCERT)
+          aux[b] = cert_finish(fma(ea, scale, cddc_collapse_err(sc) + 4.0 * CERT_UFL), perm_ddcert_depth<N>());
       } else {
-        // chunk partials stay double-double: hi in out, lo in aux-as-cplx
+        // chunk partials stay double-double: hi in out, lo in aux-as-cplx,
+        // then (certified-dd) the bounds after the B * W lo entries
         out[b * W + chunk] = cplx{acc.re.hi, acc.im.hi};
         reinterpret_cast<cplx*>(aux)[b * W + chunk] = cplx{acc.re.lo, acc.im.lo};
+        if constexpr (DDCERT) reinterpret_cast<double*>(reinterpret_cast<cplx*>(aux) + (size_t)B * W)[b * W + chunk] = ea;
       }
     }
   } else {
@@ -278,19 +395,30 @@ __global__ __launch_bounds__(64) void perm_kernel(const cplx* __restrict__ mats,
 template <int N, int MODE>
 __global__ void perm_reduce_kernel(const cplx* __restrict__ part, const double* __restrict__ part_aux, int B,
                                    cplx* __restrict__ out, double* __restrict__ aux) {
-  using C = PermCfg<N, MODE == PERM_DD ? 1 : 0>;
+  using C = PermCfg<N, perm_is_dd(MODE) ? 1 : 0>;
   constexpr int W = C::W;
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   const double scale = 1.0 / (double)(1ull << (N - 1));
-  if constexpr (MODE == PERM_DD) {
+  if constexpr (perm_is_dd(MODE)) {
     const cplx* lo = reinterpret_cast<const cplx*>(part_aux);
+    const double* pe = reinterpret_cast<const double*>(lo + (size_t)B * W);
     ddc s = ddc_zero();
+    double e = 0.0;
     for (int w = 0; w < W; ++w) {
       cplx h = part[b * W + w], l = lo[b * W + w];
-      s = ddc_add(s, ddc{dd{h.re, l.re}, dd{h.im, l.im}});
+      if constexpr (MODE == PERM_DDCERT) {
+        double r;
+        s = cddc_add(s, ddc{dd{h.re, l.re}, dd{h.im, l.im}}, &r);
+        e = e + pe[b * W + w] + r;
+      } else {
+        s = ddc_add(s, ddc{dd{h.re, l.re}, dd{h.im, l.im}});
+      }
     }
-    out[b] = ddc_to_c(ddc_mul_d(s, scale));
+    const ddc sc = ddc_mul_d(s, scale);
+    out[b] = ddc_to_c(sc);
+    if constexpr (MODE == PERM_DDCERT)
+      aux[b] = cert_finish(fma(e, scale, cddc_collapse_err(sc) + 4.0 * CERT_UFL), perm_ddcert_depth<N>());
   } else {
     cplx s{0.0, 0.0};
     double a = 0.0;
@@ -307,13 +435,15 @@ __global__ void perm_reduce_kernel(const cplx* __restrict__ part, const double* 
 
 template <int N, int MODE>
 static int launch_perm_n(const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
-  using C = PermCfg<N, MODE == PERM_DD ? 1 : 0>;
+  using C = PermCfg<N, perm_is_dd(MODE) ? 1 : 0>;
   constexpr int MPW = 64 / C::LPM, W = C::W;
   const int64_t groups = ((int64_t)B + MPW - 1) / MPW;
   const int64_t grid = (W == 1) ? groups : ((groups + 7) / 8) * 8 * W;
   if (W == 1) {
     hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, d_in, B, d_out, d_aux);
   } else {
+    // partials: values [B][W] cplx, then [B][W] of 16 bytes (kappa / bound
+    // doubles, or the dd low parts), then (certified dd) [B][W] bounds
     cplx* part = reinterpret_cast<cplx*>(ws);
     double* part_aux = reinterpret_cast<double*>(part + (size_t)B * W);
     hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, d_in, B, part, part_aux);
@@ -342,10 +472,11 @@ static int dispatch_perm(int n, const cplx* d_in, int B, cplx* d_out, double* d_
 }
 
 template <int... Ns>
-static size_t ws_perm(int n, int tier, int B, std::integer_sequence<int, Ns...>) {
+static size_t ws_perm(int n, int mode, int B, std::integer_sequence<int, Ns...>) {
   size_t r = 0;
-  ((n == Ns ? (r = (tier == PERM_DD ? PermCfg<Ns, 1>::W : PermCfg<Ns, 0>::W) > 1
-                       ? (size_t)B * (tier == PERM_DD ? PermCfg<Ns, 1>::W : PermCfg<Ns, 0>::W) * (16 + 16)
+  const size_t per_chunk = mode == PERM_DDCERT ? 16 + 16 + 8 : 16 + 16;
+  ((n == Ns ? (r = (perm_is_dd(mode) ? PermCfg<Ns, 1>::W : PermCfg<Ns, 0>::W) > 1
+                       ? (size_t)B * (perm_is_dd(mode) ? PermCfg<Ns, 1>::W : PermCfg<Ns, 0>::W) * per_chunk
                        : 0,
                0)
             : 0),
@@ -369,7 +500,8 @@ int perm_launch(int n, int mode, const cplx* d_in, int B, cplx* d_out, double* d
   if (n < 0 || n > PERM_MAX_N) return GBSK_ERR_DIM;
   if (n == 0) {
     hipLaunchKernelGGL(perm_n0_kernel, dim3((B + 255) / 256), dim3(256), 0, st, B, d_out,
-                       (mode == PERM_KAPPA || mode == PERM_CERT) ? d_aux : nullptr, mode == PERM_KAPPA ? 1.0 : 0.0);
+                       (mode == PERM_KAPPA || mode == PERM_CERT || mode == PERM_DDCERT) ? d_aux : nullptr,
+                       mode == PERM_KAPPA ? 1.0 : 0.0);
     return (int)hipGetLastError();
   }
   switch (mode) {
@@ -377,6 +509,7 @@ int perm_launch(int n, int mode, const cplx* d_in, int B, cplx* d_out, double* d
     case PERM_KAPPA: return dispatch_perm<PERM_KAPPA>(n, d_in, B, d_out, d_aux, ws, st, PermSizes{});
     case PERM_DD: return dispatch_perm<PERM_DD>(n, d_in, B, d_out, d_aux, ws, st, PermSizes{});
     case PERM_CERT: return dispatch_perm<PERM_CERT>(n, d_in, B, d_out, d_aux, ws, st, PermSizes{});
+    case PERM_DDCERT: return dispatch_perm<PERM_DDCERT>(n, d_in, B, d_out, d_aux, ws, st, PermSizes{});
   }
   return GBSK_ERR_ARG;
 }

@@ -11,7 +11,9 @@
 namespace gbsk {
 
 constexpr int PERM_MAX_N = 28;  // 2^27 Gray terms; 32-bit term indices
-enum PermMode { PERM_FP64 = 0, PERM_DD = 1, PERM_KAPPA = 2, PERM_CERT = 3 };
+enum PermMode { PERM_FP64 = 0, PERM_DD = 1, PERM_KAPPA = 2, PERM_CERT = 3, PERM_DDCERT = 4 };
+// double-double modes share one work split (so their values are identical)
+constexpr bool perm_is_dd(int mode) { return mode == PERM_DD || mode == PERM_DDCERT; }
 
 // Work split of one n x n permanent (depends on n and the tier only, never on B):
 //   LPM lanes per matrix (64 / LPM matrices share one wave),

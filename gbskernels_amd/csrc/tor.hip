@@ -212,6 +212,85 @@ struct RCERT {
   __device__ static double bound(T a) { return a.e; }
 };
 
+// Certified double-double: RDD's values (same split and operations, so values
+// are bit-identical to the dd tier's) with a running absolute bound. Each
+// operation adds the bound of its own rounding (cert.hpp, a posteriori) to the
+// propagated operand errors:
+//   a +- b   ea + eb
+//   a * b    |a| eb + |b| ea + ea eb                       (|a| = |hi| + |lo|)
+//   a / b    needs lb(b) >= 4 eb; 2 (ea + (|q| + e_q) eb) / lb(b)
+//   sqrt(a)  needs lb(a) >= 4 ea and lb(y) >= 2 e_y; 2 ea / lb(y)
+// where lb() is a guaranteed lower bound of a dd magnitude. The division's and
+// square root's own errors come from dd residuals (a - q b, y^2 - a).
+struct cdd {
+  dd v;
+  double e;
+};
+struct RCDD {
+  using T = cdd;
+  static constexpr int RMAX = RDD::RMAX;  // the dd tier's split, for identical values
+  static constexpr bool CERT = true;
+  __device__ static T zero() { return cdd{dd{0.0, 0.0}, 0.0}; }
+  __device__ static T one() { return cdd{dd{1.0, 0.0}, 0.0}; }
+  __device__ static T one_minus(cplx o) { return cdd{two_sum(1.0, -o.re), 0.0}; }  // exact
+  __device__ static T neg(cplx o) { return cdd{dd{-o.re, 0.0}, 0.0}; }
+  __device__ static bool domain(cplx o) { return o.im == 0.0; }
+  __device__ static double mag(T a) { return dd_to_d(a.v); }
+  __device__ static T add(T a, T b) {
+    double r;
+    const dd v = cdd_add(a.v, b.v, &r);
+    return cdd{v, a.e + b.e + r};
+  }
+  __device__ static T sub(T a, T b) {
+    double r;
+    const dd v = cdd_add(a.v, dd_neg(b.v), &r);
+    return cdd{v, a.e + b.e + r};
+  }
+  __device__ static T mul(T a, T b) {
+    double r;
+    const dd v = cdd_mul(a.v, b.v, &r);
+    const double t = fma(cdd_mag(a.v), b.e, fma(cdd_mag(b.v), a.e, a.e * b.e));
+    return cdd{v, t + r};
+  }
+  __device__ static T fms(T a, T b, T c) { return sub(c, mul(a, b)); }  // = dd_sub(c, dd_mul(a, b))
+  __device__ static T div(T a, T b) {
+    const dd q = dd_div(a.v, b.v);
+    const double lb = cdd_low(b.v);
+    if (!(lb > 0.0) || !(lb >= 4.0 * b.e)) return cdd{q, INFINITY};
+    double e1, e2;
+    const dd p = cdd_mul(q, b.v, &e1);
+    const dd res = cdd_add(a.v, dd_neg(p), &e2);  // a - q b, up to e1 + e2
+    const double eq = (cdd_mag(res) + e1 + e2) / lb;
+    const double prop = 2.0 * fma(cdd_mag(q) + eq, b.e, a.e) / lb;
+    return cdd{q, eq + prop};
+  }
+  __device__ static T sqrt_(T a) {
+    const dd y = dd_sqrt(a.v);
+    if (a.v.hi == 0.0 && a.v.lo == 0.0 && a.e == 0.0) return cdd{y, 0.0};
+    const double la = cdd_low(a.v), ly = cdd_low(y);
+    if (!(la > 0.0) || !(la >= 4.0 * a.e) || !(ly > 0.0)) return cdd{y, INFINITY};
+    double e1, e2;
+    const dd yy = cdd_mul(y, y, &e1);
+    const dd res = cdd_add(yy, dd_neg(a.v), &e2);  // y^2 - a
+    const double ey = (cdd_mag(res) + e1 + e2) / ly;
+    if (!(ly >= 2.0 * ey)) return cdd{y, INFINITY};
+    return cdd{y, ey + 2.0 * a.e / ly};
+  }
+  __device__ static bool pos(T a) { return a.v.hi > 0.0; }
+  __device__ static T sum64(T x) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+      const T o{dd{shfl_xor_d(x.v.hi, m), shfl_xor_d(x.v.lo, m)}, shfl_xor_d(x.e, m)};
+      x = ((threadIdx.x & m) == 0) ? add(x, o) : add(o, x);  // RDD::sum64's order
+    }
+    return x;
+  }
+  using P = dd;
+  __device__ static P part(T a) { return a.v; }
+  __device__ static P part_nan() { return dd{NAN, NAN}; }
+  __device__ static double bound(T a) { return a.e; }
+};
+
 struct TorSplit {
   int R, T2, T1;
 };
@@ -535,6 +614,26 @@ __global__ void tor_reduce_cert_kernel(const cplx* __restrict__ part, const doub
   aux[b] = (f == f) ? f : INFINITY;
 }
 
+// certified-dd reduce: the dd tier's fixed-order dd sum of the partials (same
+// bits) and its collapse to binary64, whose rounding is measured exactly
+__global__ void tor_reduce_ddcert_kernel(const dd* __restrict__ part, const double* __restrict__ part_e, int np,
+                                         int B, double depth, cplx* __restrict__ out, double* __restrict__ aux) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  dd s{0.0, 0.0};
+  double e = 0.0;
+  for (int i = 0; i < np; ++i) {
+    double r;
+    s = cdd_add(s, part[b * np + i], &r);
+    e = e + part_e[b * np + i] + r;
+  }
+  const double v = dd_to_d(s);
+  e += fabs(two_sum(s.hi, s.lo).lo);  // |v - (s.hi + s.lo)|, exact
+  out[b] = cplx{v, 0.0};
+  const double f = cert_finish(e, depth);
+  aux[b] = (f == f) ? f : INFINITY;
+}
+
 // complex fallback bookkeeping: every partial of an instance whose real-kernel
 // value is NaN is marked NaN (to be recomputed), the others 0 (skipped)
 __global__ void tor_mark_kernel(const cplx* __restrict__ out, int np, int B, cplx* __restrict__ cpart) {
@@ -663,9 +762,27 @@ static int run_tor_dd(int nm, const cplx* d_in, int B, cplx* d_out, void* ws, hi
   return (int)hipGetLastError();
 }
 
+// certified dd: the dd tier's kernels with bounds (no complex fallback: the
+// dd tiers are real-domain; off-domain instances are NaN with bound +inf)
+static int run_tor_ddcert(int nm, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
+  const TorSplit sp = tor_split(nm, RCDD::RMAX);
+  const int np = 1 << sp.T1;
+  dd* part = reinterpret_cast<dd*>(ws);
+  double* part_e = reinterpret_cast<double*>(part + (size_t)B * np);
+  int rc = launch_tor<RCDD, false, false>(nm, d_in, B, part, part_e, st);
+  if (rc) return rc;
+  // bound rounding depth: per mode <= 4 divisions / square roots and 8
+  // products of ~12 bound operations each, the 2^R leaf sums, the butterfly
+  // and the np-term reduce
+  const double depth = 160.0 * nm + 64.0 + 96.0 + 4.0 * np + 64.0;
+  hipLaunchKernelGGL(tor_reduce_ddcert_kernel, dim3((B + 255) / 256), dim3(256), 0, st, part, part_e, np, B, depth,
+                     d_out, d_aux);
+  return (int)hipGetLastError();
+}
+
 size_t tor_workspace_bytes(int tier, int dim, int B) {
   if (dim < 2 || (dim & 1) || dim > 2 * TOR_MAX_MODES || B <= 0) return 0;
-  if (tier == GBSK_DD) {
+  if (tier == GBSK_DD || tier == GBSK_DD_CERTIFIED) {
     const TorSplit sp = tor_split(dim / 2, RDD::RMAX);
     return (size_t)B * ((size_t)1 << sp.T1) * (sizeof(dd) + sizeof(double));
   }
@@ -679,7 +796,8 @@ int tor_max_dim(int tier) {
     case GBSK_FP64:
     case GBSK_DD:
     case GBSK_KAPPA:
-    case GBSK_CERTIFIED: return 2 * TOR_MAX_MODES;
+    case GBSK_CERTIFIED:
+    case GBSK_DD_CERTIFIED: return 2 * TOR_MAX_MODES;
   }
   return 0;
 }
@@ -687,16 +805,17 @@ int tor_max_dim(int tier) {
 int tor_launch(int tier, int dim, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
   if (B <= 0) return 0;
   if (dim < 0 || (dim & 1) || dim > 2 * TOR_MAX_MODES) return GBSK_ERR_DIM;
-  if (tier != GBSK_FP64 && tier != GBSK_KAPPA && tier != GBSK_DD && tier != GBSK_CERTIFIED) return GBSK_ERR_FUNC;
+  if (tier < GBSK_FP64 || tier > GBSK_DD_CERTIFIED) return GBSK_ERR_FUNC;
   const int nm = dim / 2;
   if (nm == 0) {  // tor of the empty matrix: exactly 1 (kappa 1, bound 0)
+    const bool cert = tier == GBSK_CERTIFIED || tier == GBSK_DD_CERTIFIED;
     hipLaunchKernelGGL(tor_n0_kernel, dim3((B + 255) / 256), dim3(256), 0, st, B, d_out,
-                       (tier == GBSK_KAPPA || tier == GBSK_CERTIFIED) ? d_aux : nullptr,
-                       tier == GBSK_CERTIFIED ? 0.0 : 1.0);
+                       (tier == GBSK_KAPPA || cert) ? d_aux : nullptr, cert ? 0.0 : 1.0);
     return (int)hipGetLastError();
   }
   if (!ws) return GBSK_ERR_WORKSPACE;
   if (tier == GBSK_DD) return run_tor_dd(nm, d_in, B, d_out, ws, st);
+  if (tier == GBSK_DD_CERTIFIED) return run_tor_ddcert(nm, d_in, B, d_out, d_aux, ws, st);
   if (tier == GBSK_CERTIFIED) return run_tor_fp64<RCERT, false>(nm, d_in, B, d_out, d_aux, ws, st);
   return tier == GBSK_KAPPA ? run_tor_fp64<RF64, true>(nm, d_in, B, d_out, d_aux, ws, st)
                             : run_tor_fp64<RF64, false>(nm, d_in, B, d_out, nullptr, ws, st);

@@ -196,3 +196,42 @@ def test_perm_certified_refuses_overflow(ext):
     A = np.full((1, 12, 12), 1e30 + 0j)
     v, bound = ext.perm_certified(A)
     assert not np.isfinite(bound[0]) or not np.isfinite(v[0])
+
+
+# ------------------------------------------------------------ certified dd
+@pytest.mark.parametrize("n", [1, 2, 5, 8, 11, 14])
+def test_perm_dd_certified_encloses_quad_oracle(ext, oracle, n):
+    print(f"perm dd certified n={n}", flush=True)
+    A = rng_complex((16, n, n), 900 + n)
+    v, bound = ext.perm_dd_certified(A)
+    assert np.array_equal(v, ext.perm_dd(A))       # value path is the dd tier's
+    exact = oracle.perm_q(A)
+    assert np.all(np.isfinite(bound))
+    assert np.all(np.abs(v - exact) <= bound)       # 100% enclosure
+    # dominated by the final rounding to binary64 (1-norm): a few ulps
+    assert np.all(bound <= 4 * EPS * (np.abs(v.real) + np.abs(v.imag)) + 1e-300)
+
+
+@pytest.mark.parametrize("n", [16, 20])
+def test_perm_dd_certified_chunked(ext, oracle, n):
+    print(f"perm dd certified chunked n={n}", flush=True)
+    A = rng_complex((2, n, n), 950 + n, 0.5)
+    v, bound = ext.perm_dd_certified(A)
+    assert np.array_equal(v, ext.perm_dd(A))
+    assert np.all(np.abs(v - oracle.perm_dd(A)) <= bound)
+    assert np.all(bound <= 4 * EPS * (np.abs(v.real) + np.abs(v.imag)))
+
+
+def test_perm_dd_certified_resolves_cancellation(ext, oracle):
+    """The adversarial walk of the fp64 test: dd certifies a tight bound."""
+    g = np.random.default_rng(12)
+    R = g.standard_normal((7, 7)) + 1j * g.standard_normal((7, 7))
+    A = np.zeros((9, 9), complex)
+    A[:2, :2] = [[1, 1], [1, -1 + 1e-8]]
+    A[2:, 2:] = R
+    v, bound = ext.perm_dd_certified(A[None])
+    exact = oracle.perm_q(A)[0]
+    assert abs(v[0] - exact) <= bound[0]
+    assert bound[0] / abs(exact) < 1e-14
+    v0, b0 = ext.perm_dd_certified(np.zeros((3, 0, 0), complex))
+    assert np.all(v0 == 1) and np.all(b0 == 0)

@@ -103,3 +103,45 @@ def test_tor_certified_batch_independent(ext):
     for i in (0, 4, 8):
         v1, b1 = ext.tor_certified(O[i:i + 1])
         assert v1[0] == v[i] and b1[0] == b[i]
+
+
+# ---- certified double-double -------------------------------------------------
+
+@pytest.mark.parametrize("nm", [1, 2, 3, 5, 6])
+def test_tor_dd_certified_encloses_quad_oracle(ext, oracle, nm):
+    print(f"tor dd certified nm={nm}", flush=True)
+    O = physical_stack(nm, 4, 500 + nm)
+    v, bound = ext.tor_dd_certified(O)
+    assert np.array_equal(v, ext.tor_dd(O))            # value path is the dd tier's
+    exact = oracle.tor_q(O)                             # __float128 subset determinants
+    assert np.all(np.isfinite(bound))
+    assert np.all(np.abs(v - exact) <= bound)
+    # the bound is dominated by the final rounding to binary64: a few ulps
+    assert np.all(bound <= 4 * 2.0 ** -52 * np.abs(v) + 1e-300)
+
+
+@pytest.mark.parametrize("nm", [9, 12, 16])
+def test_tor_dd_certified_large(ext, oracle, nm):
+    print(f"tor dd certified large nm={nm}", flush=True)
+    O = physical_stack(nm, 2, 600 + nm)
+    v, bound = ext.tor_dd_certified(O)
+    assert np.array_equal(v, ext.tor_dd(O))
+    ref = oracle.tor_dd(O)                              # DD oracle, error << one binary64 ulp
+    assert np.all(np.isfinite(bound))
+    assert np.all(np.abs(v - ref) <= bound)
+    assert np.all(bound <= 4 * 2.0 ** -52 * np.abs(v))
+
+
+def test_tor_dd_certified_cancellation_and_refusal(ext):
+    for delta in (1e-3, 1e-9, 1e-13):
+        a = delta                                       # tor(a I_2) = a / (1 - a): cancels
+        O = (a * np.eye(2))[None].astype(np.complex128)
+        v, bound = ext.tor_dd_certified(O)
+        fa = Fraction(a)
+        exact = fa / (1 - fa)
+        assert abs(Fraction(v[0].real) - exact) <= Fraction(bound[0])
+        assert bound[0] <= 2.0 ** -52 * abs(v[0])      # dd resolves the cancellation
+    v, bound = ext.tor_dd_certified((1.5 * np.eye(4))[None].astype(np.complex128))
+    assert np.isinf(bound[0])                           # I - O not SPD: refused
+    v, bound = ext.tor_dd_certified(np.zeros((2, 0, 0), np.complex128))
+    assert np.all(v == 1) and np.all(bound == 0)
