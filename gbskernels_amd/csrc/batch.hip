@@ -383,8 +383,9 @@ static long long now_ns() {
 struct Req {
   int func = 0, tier = 0, dim = 0;
   int run_tier = 0;  // tier of the pass being evaluated
-  std::vector<cplx> a;
-  bool real_input = true;
+  std::vector<cplx> a;        // owned copy of the matrix (gbs_server_submit)
+  const cplx* src = nullptr;  // or the caller's matrix, alive until done (replay)
+  const cplx* mat() const { return src ? src : a.data(); }
   long long arrival = 0, admitted = 0, done = 0;
   gbs_complex out{0.0, 0.0};
   double aux = 0.0;
@@ -451,7 +452,11 @@ static void complete_bucket(gbs_server* s, Slot& sl, Bucket& bk, int err, std::v
       // kappa = sum|term| / max(|value|, 1e-300) >= kappa_max, as the auto tier
       const double kappa = ax / std::max(std::hypot(v.re, v.im), 1e-300);
       const bool risky = kappa >= s->cfg.kappa_max;
-      const bool dd_ok = r->dim <= gbs_max_dim(r->func, GBSK_DD) && (r->func != GBSK_TOR || r->real_input);
+      bool dd_ok = r->dim <= gbs_max_dim(r->func, GBSK_DD);
+      if (dd_ok && r->func == GBSK_TOR) {  // the dd torontonian is real-domain
+        const cplx* m = r->mat();
+        for (size_t e = 0; e < (size_t)r->dim * r->dim && dd_ok; ++e) dd_ok = m[e].im == 0.0;
+      }
       if (risky && dd_ok) {
         r->run_tier = GBSK_DD;
         requeue.push_back(r);
@@ -544,7 +549,7 @@ static void dispatcher(gbs_server* s) {
         fn[i] = take[i]->func;
         tr[i] = take[i]->run_tier;
         dm[i] = take[i]->dim;
-        ptrs[i] = take[i]->a.data();
+        ptrs[i] = take[i]->mat();
         if (!take[i]->admitted) take[i]->admitted = t;
       }
       sl.reqs = take;
@@ -702,9 +707,13 @@ int gbs_server_create(gbs_server** out, const gbs_server_config* cfg) {
   return 0;
 }
 
-int gbs_server_submit(gbs_server* s, int func, int tier, int dim, const gbs_complex* in, long long arrival_ns,
-                      long long* ticket) {
-  if (!s || !ticket || (!in && dim > 0)) return GBSK_ERR_ARG;
+}  // extern "C"
+
+namespace gbsk {
+
+// a validated request; copy: own the matrix, else borrow the caller's
+static int make_req(int func, int tier, int dim, const gbs_complex* in, long long arrival_ns, bool copy, Req** out) {
+  if (!in && dim > 0) return GBSK_ERR_ARG;
   if (func < GBSK_PERM || func > GBSK_TOR || tier < GBSK_FP64 || tier > GBSK_AUTO) return GBSK_ERR_FUNC;
   const int run = tier == GBSK_AUTO ? GBSK_KAPPA : tier;
   const int cap = gbs_max_dim(func, run);
@@ -718,27 +727,50 @@ int gbs_server_submit(gbs_server* s, int func, int tier, int dim, const gbs_comp
   r->dim = dim;
   r->arrival = arrival_ns > 0 ? arrival_ns : now_ns();
   const size_t m2 = (size_t)dim * dim;
-  try {
-    r->a.resize(m2);
-  } catch (...) {
-    delete r;
-    return GBSK_ERR_ALLOC;
+  if (copy) {
+    try {
+      r->a.resize(m2);
+    } catch (...) {
+      delete r;
+      return GBSK_ERR_ALLOC;
+    }
+    if (m2) std::memcpy(r->a.data(), in, m2 * sizeof(cplx));
+  } else {
+    r->src = (const cplx*)in;
   }
-  if (m2) std::memcpy(r->a.data(), in, m2 * sizeof(cplx));
-  for (size_t e = 0; e < m2 && r->real_input; ++e) r->real_input = r->a[e].im == 0.0;
+  *out = r;
+  return 0;
+}
+
+// queue requests under one lock; tickets[i] receives request i's ticket
+static int enqueue_reqs(gbs_server* s, Req* const* rs, int n, long long* tickets) {
   {
     std::lock_guard<std::mutex> g(s->mu);
-    if (s->stop) {
-      delete r;
-      return GBSK_ERR_ARG;
+    if (s->stop) return GBSK_ERR_ARG;
+    for (int i = 0; i < n; ++i) {
+      tickets[i] = s->next_ticket++;
+      s->table.emplace(tickets[i], rs[i]);
+      s->pending.push_back(rs[i]);
     }
-    *ticket = s->next_ticket++;
-    s->table.emplace(*ticket, r);
-    s->pending.push_back(r);
-    ++s->stats.submitted;
+    s->stats.submitted += (unsigned long long)n;
   }
   s->cv_work.notify_one();
   return 0;
+}
+
+}  // namespace gbsk
+
+extern "C" {
+
+int gbs_server_submit(gbs_server* s, int func, int tier, int dim, const gbs_complex* in, long long arrival_ns,
+                      long long* ticket) {
+  if (!s || !ticket) return GBSK_ERR_ARG;
+  Req* r = nullptr;
+  int rc = make_req(func, tier, dim, in, arrival_ns, true, &r);
+  if (rc) return rc;
+  rc = enqueue_reqs(s, &r, 1, ticket);
+  if (rc) delete r;
+  return rc;
 }
 
 int gbs_server_wait(gbs_server* s, long long ticket, gbs_complex* out, double* aux, gbs_request_info* info) {
@@ -771,17 +803,40 @@ int gbs_server_replay(gbs_server* s, int n, const int* func, const int* tier, co
   if (n <= 0) return 0;
   if (!func || !tier || !dim || !in || !off || !arrival_offset_ns || !out) return GBSK_ERR_ARG;
   std::vector<long long> tickets((size_t)n, 0);
+  std::vector<Req*> rs;
+  rs.reserve(1024);
   int first = 0;
   const long long t0 = now_ns();
   int i = 0;
-  for (; i < n; ++i) {
-    const long long due = t0 + arrival_offset_ns[i];
-    while (now_ns() < due) std::this_thread::yield();
-    const int rc = gbs_server_submit(s, func[i], tier[i], dim[i], in + off[i], due, &tickets[i]);
-    if (rc) {
-      first = rc;
-      break;
+  // every request already due is queued under one lock; the matrices are
+  // borrowed from `in`, which outlives the call
+  while (i < n && !first) {
+    long long t = now_ns();
+    if (t < t0 + arrival_offset_ns[i]) {
+      std::this_thread::yield();
+      continue;
     }
+    int j = i;
+    rs.clear();
+    while (j < n && t0 + arrival_offset_ns[j] <= t && j - i < 4096) {
+      Req* r = nullptr;
+      const int rc = make_req(func[j], tier[j], dim[j], in + off[j], t0 + arrival_offset_ns[j], false, &r);
+      if (rc) {
+        first = rc;
+        break;
+      }
+      rs.push_back(r);
+      ++j;
+    }
+    if (!rs.empty()) {
+      const int rc = enqueue_reqs(s, rs.data(), (int)rs.size(), tickets.data() + i);
+      if (rc) {
+        for (Req* r : rs) delete r;
+        first = rc;
+        break;
+      }
+    }
+    i += (int)rs.size();
   }
   for (int j = 0; j < i; ++j) {
     const int rc = gbs_server_wait(s, tickets[j], out + j, aux ? aux + j : nullptr, info ? info + j : nullptr);

@@ -163,3 +163,55 @@ __device__ inline double cddc_collapse_err(ddc a) {
 }
 
 }  // namespace gbsk
+
+namespace gbsk {
+
+// r = dd_mul_d(a, b); *e = |r - a b|
+__device__ inline dd cdd_mul_d(dd a, double b, double* e) {
+  const dd p = two_prod(a.hi, b);
+  const double w = fma(a.lo, b, p.lo);
+  const dd r = fast_two_sum(p.hi, w);
+  *e = fma(CERT_U, fabs(w), cdd_fts_err(p.hi, w, r) + 2.0 * CERT_UFL);
+  return r;
+}
+// r = dd_div_d(a, b), b a nonzero double; *e = |r - a / b| from the residual
+// a - r b (computed in dd with its own bound)
+__device__ inline dd cdd_div_d(dd a, double b, double* e) {
+  const dd q = dd_div_d(a, b);
+  double e1, e2;
+  const dd t = cdd_mul_d(q, b, &e1);
+  const dd res = cdd_add(a, dd_neg(t), &e2);
+  *e = (cdd_mag(res) + e1 + e2) / fabs(b);
+  return q;
+}
+// the complex dd helpers of haf_impl.hpp's ArDD, with their own errors
+__device__ inline ddc cddc_fma_c(ddc a, cplx z, ddc acc, double* e) {  // acc + a z
+  double e1, e2, e3, e4, e5, e6, e7, e8;
+  const dd x1 = cdd_mul_d(a.re, z.re, &e1), x2 = cdd_mul_d(a.im, z.im, &e2);
+  const dd x3 = cdd_mul_d(a.re, z.im, &e3), x4 = cdd_mul_d(a.im, z.re, &e4);
+  const dd re = cdd_add(x1, dd_neg(x2), &e5), im = cdd_add(x3, x4, &e6);
+  const ddc r{cdd_add(acc.re, re, &e7), cdd_add(acc.im, im, &e8)};
+  *e = ((e1 + e2) + (e3 + e4)) + ((e5 + e6) + (e7 + e8));
+  return r;
+}
+__device__ inline ddc cddc_fma(ddc a, ddc b, ddc acc, double* e) {  // acc + a b
+  double e1, e2;
+  const ddc p = cddc_mul(a, b, &e1);
+  const ddc r = cddc_add(acc, p, &e2);
+  *e = e1 + e2;
+  return r;
+}
+__device__ inline ddc cddc_scale(ddc a, double s, double* e) {
+  double e1, e2;
+  const ddc r{cdd_mul_d(a.re, s, &e1), cdd_mul_d(a.im, s, &e2)};
+  *e = e1 + e2;
+  return r;
+}
+__device__ inline ddc cddc_div_d(ddc a, double s, double* e) {
+  double e1, e2;
+  const ddc r{cdd_div_d(a.re, s, &e1), cdd_div_d(a.im, s, &e2)};
+  *e = e1 + e2;
+  return r;
+}
+
+}  // namespace gbsk

@@ -22,9 +22,8 @@ __global__ void haf_trivial_kernel(int N, bool loop, int B, cplx* out, double* a
 
 size_t haf_workspace_bytes(bool, int tier, int N, int B) {
   if (N < 2 || (N & 1) || N > HAF_MAX_N || B <= 0) return 0;
-  const size_t n = N / 2;
-  const size_t elem = (tier == GBSK_DD) ? sizeof(ddc) : sizeof(cplx);
-  return (size_t)B * n * (elem + sizeof(double));
+  const size_t elem = (tier == GBSK_DD || tier == GBSK_DD_CERTIFIED) ? sizeof(ddc) : sizeof(cplx);
+  return (size_t)B * haf_nparts(N / 2) * (elem + sizeof(double));
 }
 
 int haf_max_dim(bool, int tier) {
@@ -32,7 +31,8 @@ int haf_max_dim(bool, int tier) {
     case GBSK_FP64:
     case GBSK_DD:
     case GBSK_KAPPA:
-    case GBSK_CERTIFIED: return HAF_MAX_N;
+    case GBSK_CERTIFIED:
+    case GBSK_DD_CERTIFIED: return HAF_MAX_N;
   }
   return 0;
 }
@@ -41,10 +41,11 @@ int haf_launch(bool loop, int tier, int N, const cplx* d_in, int B, cplx* d_out,
                hipStream_t st) {
   if (B <= 0) return 0;
   if (N < 0 || N > HAF_MAX_N) return GBSK_ERR_DIM;
-  if (tier != GBSK_FP64 && tier != GBSK_DD && tier != GBSK_KAPPA && tier != GBSK_CERTIFIED) return GBSK_ERR_FUNC;
+  if (tier < GBSK_FP64 || tier > GBSK_DD_CERTIFIED) return GBSK_ERR_FUNC;
   if (N == 0 || (N & 1)) {
+    const bool aux = tier == GBSK_KAPPA || tier == GBSK_CERTIFIED || tier == GBSK_DD_CERTIFIED;
     hipLaunchKernelGGL(haf_trivial_kernel, dim3((B + 255) / 256), dim3(256), 0, st, N, loop, B, d_out,
-                       (tier == GBSK_KAPPA || tier == GBSK_CERTIFIED) ? d_aux : nullptr, tier == GBSK_KAPPA);
+                       aux ? d_aux : nullptr, tier == GBSK_KAPPA);
     return (int)hipGetLastError();
   }
   if (!ws) return GBSK_ERR_WORKSPACE;
@@ -52,6 +53,8 @@ int haf_launch(bool loop, int tier, int N, const cplx* d_in, int B, cplx* d_out,
                                    : run_haf<ArDD, false, HM_PLAIN>(N, d_in, B, d_out, nullptr, ws, st);
   if (tier == GBSK_KAPPA) return loop ? run_haf<ArF64, true, HM_KAPPA>(N, d_in, B, d_out, d_aux, ws, st)
                                       : run_haf<ArF64, false, HM_KAPPA>(N, d_in, B, d_out, d_aux, ws, st);
+  if (tier == GBSK_DD_CERTIFIED) return loop ? run_haf<ArDD, true, HM_DDCERT>(N, d_in, B, d_out, d_aux, ws, st)
+                                             : run_haf<ArDD, false, HM_DDCERT>(N, d_in, B, d_out, d_aux, ws, st);
   if (tier == GBSK_CERTIFIED) return loop ? run_haf<ArF64, true, HM_CERT>(N, d_in, B, d_out, d_aux, ws, st)
                                           : run_haf<ArF64, false, HM_CERT>(N, d_in, B, d_out, d_aux, ws, st);
   return loop ? run_haf<ArF64, true, HM_PLAIN>(N, d_in, B, d_out, nullptr, ws, st)

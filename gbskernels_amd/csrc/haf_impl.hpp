@@ -125,7 +125,7 @@ __device__ inline typename Ar::T group_reduce(typename Ar::T v, typename Ar::T* 
 // error of the computed complex value relative to the exact value of the same
 // formula on the exact inputs.
 
-enum HafMode { HM_PLAIN = 0, HM_KAPPA = 1, HM_CERT = 2 };
+enum HafMode { HM_PLAIN = 0, HM_KAPPA = 1, HM_CERT = 2, HM_DDCERT = 3 };
 
 // gamma_k = k u / (1 - k u) <= k u (1 + 1e-6) for k u < 1e-7 (k <= 2 HAF_MAX_N here)
 __device__ inline double cert_gamma(int k) { return (double)k * CERT_U * 1.000001; }
@@ -161,6 +161,56 @@ __device__ inline void cert_group_reduce(cplx& v, double& e, cplx* red, double* 
   e = E;
 }
 
+// certified dd (HM_DDCERT, Ar = ArDD): the same reduction with each dd
+// addition's own error (cert.hpp) added to the operands' bounds
+template <int S>
+__device__ inline void ddcert_group_reduce(ddc& v, double& e, ddc* red, double* rede) {
+  const int lane = threadIdx.x;
+  red[lane] = v;
+  rede[lane] = e;
+  __syncthreads();
+  const int base = (lane / S < 64 / S) ? (lane / S) * S : 0;
+  ddc s = red[base];
+  double E = rede[base];
+#pragma unroll
+  for (int i = 1; i < S; ++i) {
+    double r;
+    s = cddc_add(s, red[base + i], &r);
+    E = E + rede[base + i] + r;
+  }
+  __syncthreads();
+  v = s;
+  e = E;
+}
+
+// Work split of a phase: its C(n, m) subsets are cut into chunks of
+// HAF_CHUNK_ITERS * G consecutive subsets (G = 64 / S per wave iteration), one
+// workgroup per (instance, chunk), so that large N runs many waves per instance.
+// The split depends on (N, S) only; chunk partials are summed in chunk order.
+constexpr int HAF_CHUNK_ITERS = 4;
+__host__ __device__ inline int haf_chunk_size(int S) { return HAF_CHUNK_ITERS * (64 / S); }
+__host__ __device__ inline int haf_binom(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  int r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+__host__ __device__ inline int haf_nchunks(int n, int S) {
+  const int c = haf_binom(n, S / 2), ch = haf_chunk_size(S);
+  return (c + ch - 1) / ch;
+}
+// partial slots per instance (all phases) and the first slot of phase S
+__host__ __device__ inline int haf_nparts(int n) {
+  int t = 0;
+  for (int m = 1; m <= n; ++m) t += haf_nchunks(n, 2 * m);
+  return t;
+}
+__host__ __device__ inline int haf_part_offset(int n, int S) {
+  int t = 0;
+  for (int m = 1; m < S / 2; ++m) t += haf_nchunks(n, 2 * m);
+  return t;
+}
+
 // rounding depth of a phase's bound arithmetic (generous; see cert_finish)
 __device__ inline double haf_cert_depth(int S, int n, int count) {
   return 64.0 * (n + 1) * (S + 8) + 8.0 * count + 16.0 * n * n + 256.0;
@@ -170,7 +220,7 @@ __device__ inline double haf_cert_depth(int S, int n, int count) {
 // One workgroup (one wave) = one instance; evaluates every subset of size
 // m = S/2 and writes the phase partial (and sum |term| in kappa mode, the
 // bound in certified mode).
-//   part[b * n + (m - 1)]      phase partial
+//   part[b * np + off_S + chunk]      chunk partial (np = haf_nparts(n))
 template <class Ar, bool LOOP, int S, int MODE>
 __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ mats, int N, int B,
                                                        typename Ar::T* __restrict__ part,
@@ -178,6 +228,8 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
   using T = typename Ar::T;
   constexpr bool KAPPA = (MODE == HM_KAPPA);
   constexpr bool CERT = (MODE == HM_CERT);
+  constexpr bool DDC = (MODE == HM_DDCERT);  // certified double-double (Ar = ArDD)
+  constexpr bool ANYC = CERT || DDC;
   constexpr int M = S / 2;
   constexpr int G = 64 / S;
   extern __shared__ cplx As[];  // N * N, dynamic
@@ -185,13 +237,16 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
   __shared__ T wbuf[64];
   __shared__ T kgs[G][HAF_MAX_PAIRS + 1];  // k g_k of each group's current subset
   __shared__ T es[G][HAF_MAX_PAIRS + 1];   // Newton coefficients e_0..e_n
-  __shared__ double rede[CERT ? 64 : 1];
-  __shared__ double wbufe[CERT ? 64 : 1];
-  __shared__ double kge[CERT ? G : 1][HAF_MAX_PAIRS + 1];
-  __shared__ double ese[CERT ? G : 1][HAF_MAX_PAIRS + 1];
+  __shared__ double rede[ANYC ? 64 : 1];
+  __shared__ double wbufe[ANYC ? 64 : 1];
+  __shared__ double kge[ANYC ? G : 1][HAF_MAX_PAIRS + 1];
+  __shared__ double ese[ANYC ? G : 1][HAF_MAX_PAIRS + 1];
   const int lane = threadIdx.x;
   const int n = N / 2;
-  const int64_t b = blockIdx.x;
+  const int nch = haf_nchunks(N / 2, S);
+  const int64_t b = blockIdx.x / nch;
+  const int chunk = (int)(blockIdx.x % nch);
+  const int64_t pslot = b * haf_nparts(N / 2) + haf_part_offset(N / 2, S) + chunk;
   // stage A (row-major N x N); the ordinary hafnian zeroes the diagonal
   for (int e = lane; e < N * N; e += 64) {
     cplx v = mats[b * N * N + e];
@@ -210,22 +265,24 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
   T group_acc = Ar::zero();
   double group_abs = 0.0;  // kappa: sum |term|; certified: bound of group_acc
 
-  for (int q0 = 0; q0 < count; q0 += G) {
+  const int qbeg = chunk * haf_chunk_size(S);
+  const int qend = min(count, qbeg + haf_chunk_size(S));
+  for (int q0 = qbeg; q0 < qend; q0 += G) {
     const int q = q0 + (lane_ok ? g : 0);  // groups past G mirror group 0 (discarded)
-    const bool active = lane_ok && q < count;
+    const bool active = lane_ok && q < qend;
     int p[M];
     unrank<M>(active ? q : 0, n, p);
     // lane r owns row r of C = B_S X: C[r][2u] = A[R][2p_u + 1], C[r][2u+1] = A[R][2p_u]
     const int R = 2 * p[r >> 1] + (r & 1);
     const int gs = lane_ok ? g : 0;
     T P[S];
-    double eP[CERT ? S : 1];
+    double eP[ANYC ? S : 1];
 #pragma unroll
     for (int u = 0; u < M; ++u) {
       P[2 * u] = Ar::from(As[R * N + 2 * p[u] + 1]);
       P[2 * u + 1] = Ar::from(As[R * N + 2 * p[u]]);
     }
-    if (CERT) {
+    if (ANYC) {
 #pragma unroll
       for (int c = 0; c < S; ++c) eP[c] = 0.0;  // C is exact
     }
@@ -237,11 +294,27 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
 #pragma unroll
       for (int c = 0; c < S; ++c) {
         const T y = Ar::shfl_xor(P[c ^ 1], 1);
-        const T t2n = Ar::fma(P[c], y, t2);
-        if constexpr (CERT) e2 = cert_fma(P[c], 0.0, y, 0.0, t2, e2, t2n);
-        t2 = t2n;
+        if constexpr (DDC) {
+          double eo;
+          t2 = cddc_fma(P[c], y, t2, &eo);  // exact operands
+          e2 += eo;
+        } else {
+          const T t2n = Ar::fma(P[c], y, t2);
+          if constexpr (CERT) e2 = cert_fma(P[c], 0.0, y, 0.0, t2, e2, t2n);
+          t2 = t2n;
+        }
       }
-      if constexpr (CERT) {
+      if constexpr (DDC) {
+        double e1 = 0.0;
+        ddcert_group_reduce<S>(d1, e1, red, rede);
+        ddcert_group_reduce<S>(t2, e2, red, rede);
+        if (r == 0 && lane_ok) {  // halving is exact (up to underflow)
+          kgs[g][1] = Ar::scale(d1, 0.5);
+          kgs[g][2] = Ar::scale(t2, 0.5);
+          kge[g][1] = fma(0.5, e1, 2.0 * CERT_UFL);
+          kge[g][2] = fma(0.5, e2, 2.0 * CERT_UFL);
+        }
+      } else if constexpr (CERT) {
         double e1 = 0.0;
         cert_group_reduce<S>(d1, e1, red, rede);
         cert_group_reduce<S>(t2, e2, red, rede);
@@ -263,12 +336,12 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
     // C^j for j = 2..h: one matrix product each, then p_{2j-1}, p_{2j}
     for (int j = 2; j <= h; ++j) {
       T acc[S];
-      double eA[CERT ? S : 1];
+      double eA[ANYC ? S : 1];
 #pragma unroll
       for (int c = 0; c < S; ++c) acc[c] = Ar::zero();
-      if (CERT) {
+      if (ANYC) {
 #pragma unroll
-        for (int c = 0; c < S; ++c) eA[c] = (double)S * CERT_UFL;
+        for (int c = 0; c < S; ++c) eA[c] = CERT ? (double)S * CERT_UFL : 0.0;
       }
 #pragma unroll
       for (int t = 0; t < S; ++t) {
@@ -278,6 +351,14 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
 #pragma unroll
         for (int u = 0; u < M; ++u) {
           const cplx a1 = At[2 * p[u] + 1], a0 = At[2 * p[u]];
+          if constexpr (DDC) {
+            double o1, o0;
+            acc[2 * u] = cddc_fma_c(P[t], a1, acc[2 * u], &o1);
+            acc[2 * u + 1] = cddc_fma_c(P[t], a0, acc[2 * u + 1], &o0);
+            eA[2 * u] = fma(eP[t], cert_norm1(a1), eA[2 * u] + o1);
+            eA[2 * u + 1] = fma(eP[t], cert_norm1(a0), eA[2 * u + 1] + o0);
+            continue;
+          }
           acc[2 * u] = Ar::fma_c(P[t], a1, acc[2 * u]);
           acc[2 * u + 1] = Ar::fma_c(P[t], a0, acc[2 * u + 1]);
           if (CERT) {
@@ -292,6 +373,17 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
       for (int c = 0; c < S; ++c) {
         const T yo = Ar::shfl_xor(P[c ^ 1], 1);
         const T ye = Ar::shfl_xor(acc[c ^ 1], 1);
+        if constexpr (DDC) {
+          const double eyo = shfl_xor_d(eP[c ^ 1], 1), eye = shfl_xor_d(eA[c ^ 1], 1);
+          double oo, oe;
+          const T ton = cddc_fma(acc[c], yo, todd, &oo);
+          const T ten = cddc_fma(acc[c], ye, teven, &oe);
+          eo = eo + cddc_mul_bound(acc[c], eA[c], yo, eyo, oo);
+          ee = ee + cddc_mul_bound(acc[c], eA[c], ye, eye, oe);
+          todd = ton;
+          teven = ten;
+          continue;
+        }
         const T ton = Ar::fma(acc[c], yo, todd);
         const T ten = Ar::fma(acc[c], ye, teven);
         if constexpr (CERT) {
@@ -302,7 +394,20 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
         todd = ton;
         teven = ten;
       }
-      if constexpr (CERT) {
+      if constexpr (DDC) {
+        ddcert_group_reduce<S>(todd, eo, red, rede);
+        ddcert_group_reduce<S>(teven, ee, red, rede);
+        if (r == 0 && lane_ok) {
+          kgs[g][2 * j - 1] = Ar::scale(todd, 0.5);
+          kge[g][2 * j - 1] = fma(0.5, eo, 2.0 * CERT_UFL);
+          if (2 * j <= HAF_MAX_PAIRS) {
+            kgs[g][2 * j] = Ar::scale(teven, 0.5);
+            kge[g][2 * j] = fma(0.5, ee, 2.0 * CERT_UFL);
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < S; ++c) eP[c] = eA[c];
+      } else if constexpr (CERT) {
         cert_group_reduce<S>(todd, eo, red, rede);
         cert_group_reduce<S>(teven, ee, red, rede);
         if (r == 0 && lane_ok) {
@@ -332,8 +437,20 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
       T w = Ar::from(As[R * N + R]);
       double ew = 0.0;
       for (int k = 1; k <= n; ++k) {
-        T vpart = Ar::fma_c(w, dpartner, Ar::zero());
-        if constexpr (CERT) {
+        if constexpr (DDC) {
+          double ev;
+          T vpart = cddc_fma_c(w, dpartner, Ar::zero(), &ev);
+          ev = fma(ew, cert_norm1(dpartner), ev);
+          ddcert_group_reduce<S>(vpart, ev, red, rede);
+          if (r == 0 && lane_ok) {
+            double os, oa;
+            const T kv = cddc_scale(vpart, 0.5 * (double)k, &os);
+            const T sum = cddc_add(kgs[g][k], kv, &oa);
+            kge[g][k] = kge[g][k] + fma(0.5 * (double)k, ev, os) + oa;
+            kgs[g][k] = sum;
+          }
+        } else if constexpr (CERT) {
+          T vpart = Ar::fma_c(w, dpartner, Ar::zero());
           double ev = cert_fma(Ar::to_c(w), ew, dpartner, 0.0, cplx{0.0, 0.0}, 0.0, vpart);
           cert_group_reduce<S>(vpart, ev, red, rede);
           if (r == 0 && lane_ok) {
@@ -345,12 +462,12 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
             kgs[g][k] = sum;
           }
         } else {
-          const T vk = group_reduce<Ar, S>(vpart, red);
+          const T vk = group_reduce<Ar, S>(Ar::fma_c(w, dpartner, Ar::zero()), red);
           if (r == 0 && lane_ok) kgs[g][k] = Ar::add(kgs[g][k], Ar::scale(vk, 0.5 * (double)k));
         }
         if (k < n) {
           wbuf[lane] = w;
-          if (CERT) wbufe[lane] = ew;
+          if (ANYC) wbufe[lane] = ew;
           __syncthreads();
           T nw = Ar::zero();
           double enw = CERT ? (double)S * CERT_UFL : 0.0;
@@ -358,6 +475,14 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
           for (int u = 0; u < M; ++u) {
             const T w1 = wbuf[gs * S + 2 * u], w0 = wbuf[gs * S + 2 * u + 1];
             const cplx a1 = As[R * N + 2 * p[u] + 1], a0 = As[R * N + 2 * p[u]];
+            if constexpr (DDC) {
+              double o1, o0;
+              nw = cddc_fma_c(w1, a1, nw, &o1);
+              nw = cddc_fma_c(w0, a0, nw, &o0);
+              enw = fma(wbufe[gs * S + 2 * u], cert_norm1(a1), enw + o1);
+              enw = fma(wbufe[gs * S + 2 * u + 1], cert_norm1(a0), enw + o0);
+              continue;
+            }
             nw = Ar::fma_c(w1, a1, nw);
             nw = Ar::fma_c(w0, a0, nw);
             if (CERT) {
@@ -378,10 +503,22 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
     if (r == 0 && lane_ok) {
       T* e = es[g];
       e[0] = Ar::one();
-      if (CERT) ese[g][0] = 0.0;
+      if (ANYC) ese[g][0] = 0.0;
       for (int jj = 1; jj <= n; ++jj) {
         T a = Ar::zero();
         double ea = 0.0;
+        if constexpr (DDC) {
+          for (int k = 1; k <= jj; ++k) {
+            double o;
+            const T an = cddc_fma(kgs[g][k], e[jj - k], a, &o);
+            ea = ea + cddc_mul_bound(kgs[g][k], kge[g][k], e[jj - k], ese[g][jj - k], o);
+            a = an;
+          }
+          double od;
+          e[jj] = cddc_div_d(a, (double)jj, &od);
+          ese[g][jj] = ea / (double)jj + od;
+          continue;
+        }
         for (int k = 1; k <= jj; ++k) {
           const T an = Ar::fma(kgs[g][k], e[jj - k], a);
           if (CERT) ea = cert_fma(Ar::to_c(kgs[g][k]), kge[g][k], Ar::to_c(e[jj - k]), ese[g][jj - k], Ar::to_c(a), ea,
@@ -392,11 +529,18 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
         if (CERT) ese[g][jj] = cert_round(Ar::to_c(e[jj]), ea / (double)jj);
       }
       coeff = e[n];
-      if (CERT) ecoeff = ese[g][n];
+      if (ANYC) ecoeff = ese[g][n];
     }
     __syncthreads();
     if ((n - M) & 1) coeff = Ar::neg(coeff);
-    if (active && r == 0) {
+    if constexpr (DDC) {
+      if (active && r == 0) {
+        double o;
+        group_acc = cddc_add(group_acc, coeff, &o);
+        group_abs = group_abs + ecoeff + o;
+      }
+    }
+    if (!DDC && active && r == 0) {
       group_acc = Ar::add(group_acc, coeff);
       if (KAPPA) {
         const cplx cc = Ar::to_c(coeff);
@@ -407,17 +551,25 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
   }
   // combine the G group partials in group order
   if (r == 0 && lane_ok) red[g] = group_acc;
-  if (CERT && r == 0 && lane_ok) rede[g] = group_abs;
+  if (ANYC && r == 0 && lane_ok) rede[g] = group_abs;
   __syncthreads();
   if (lane == 0) {
     T s = red[0];
-    double E = CERT ? rede[0] : 0.0;
+    double E = ANYC ? rede[0] : 0.0;
     for (int i = 1; i < G; ++i) {
-      s = Ar::add(s, red[i]);
-      if (CERT) E = cert_round(Ar::to_c(s), E + rede[i]);
+      if constexpr (DDC) {
+        double o;
+        s = cddc_add(s, red[i], &o);
+        E = E + rede[i] + o;
+      } else {
+        s = Ar::add(s, red[i]);
+        if (CERT) E = cert_round(Ar::to_c(s), E + rede[i]);
+      }
     }
-    part[b * n + (M - 1)] = s;
-    if (CERT) part_abs[b * n + (M - 1)] = cert_finish(E, haf_cert_depth(S, n, count));
+    part[pslot] = s;
+    // certified dd: each bound operation above is a few roundings deeper than
+    // its fp64 counterpart, hence 4x the fp64 depth
+    if (ANYC) part_abs[pslot] = cert_finish(E, (DDC ? 4.0 : 1.0) * haf_cert_depth(S, n, count));
   }
   if (KAPPA) {
     __syncthreads();
@@ -427,7 +579,7 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
     if (lane == 0) {
       double s = rabs[0];
       for (int i = 1; i < G; ++i) s += rabs[i];
-      part_abs[b * n + (M - 1)] = s;
+      part_abs[pslot] = s;
     }
   }
 }
@@ -438,21 +590,33 @@ __global__ void haf_reduce_kernel(const typename Ar::T* __restrict__ part, const
                                   int n, int B, cplx* __restrict__ out, double* __restrict__ aux) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
-  typename Ar::T s = part[b * n];
-  double a = MODE != HM_PLAIN ? part_abs[b * n] : 0.0;
-  for (int m = 1; m < n; ++m) {
-    s = Ar::add(s, part[b * n + m]);
-    if (MODE == HM_KAPPA) a += part_abs[b * n + m];
-    if (MODE == HM_CERT) a = cert_round(Ar::to_c(s), a + part_abs[b * n + m]);
+  const int np = haf_nparts(n);
+  typename Ar::T s = part[b * np];
+  double a = MODE != HM_PLAIN ? part_abs[b * np] : 0.0;
+  for (int m = 1; m < np; ++m) {
+    if constexpr (MODE == HM_DDCERT) {
+      double o;
+      s = cddc_add(s, part[b * np + m], &o);
+      a = a + part_abs[b * np + m] + o;
+    } else {
+      s = Ar::add(s, part[b * np + m]);
+      if (MODE == HM_KAPPA) a += part_abs[b * np + m];
+      if (MODE == HM_CERT) a = cert_round(Ar::to_c(s), a + part_abs[b * np + m]);
+    }
   }
   out[b] = Ar::to_c(s);
   if (MODE == HM_KAPPA) aux[b] = a;
-  if (MODE == HM_CERT) aux[b] = cert_finish(a, 4.0 * n + 8.0);
+  if (MODE == HM_CERT) aux[b] = cert_finish(a, 4.0 * np + 8.0);
+  if constexpr (MODE == HM_DDCERT) {
+    const double f = cert_finish(a + cddc_collapse_err(s), 8.0 * np + 16.0);
+    aux[b] = (f == f) ? f : INFINITY;
+  }
 }
 
 template <class Ar, bool LOOP, int MODE, int S>
 static void launch_phase(const cplx* d_in, int N, int B, typename Ar::T* part, double* part_abs, hipStream_t st) {
-  hipLaunchKernelGGL((haf_phase_kernel<Ar, LOOP, S, MODE>), dim3((unsigned)B), dim3(64),
+  const unsigned grid = (unsigned)((int64_t)B * haf_nchunks(N / 2, S));
+  hipLaunchKernelGGL((haf_phase_kernel<Ar, LOOP, S, MODE>), dim3(grid), dim3(64),
                      (size_t)N * N * sizeof(cplx), st, d_in, N, B, part, part_abs);
 }
 
@@ -469,7 +633,7 @@ int run_haf(int N, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws
   using T = typename Ar::T;
   const int n = N / 2;
   T* part = reinterpret_cast<T*>(ws);
-  double* part_abs = reinterpret_cast<double*>(part + (size_t)B * n);
+  double* part_abs = reinterpret_cast<double*>(part + (size_t)B * haf_nparts(n));
   launch_phases<Ar, LOOP, MODE>(d_in, N, B, part, part_abs, st, std::make_integer_sequence<int, HAF_MAX_PAIRS>{});
   hipLaunchKernelGGL((haf_reduce_kernel<Ar, MODE>), dim3((B + 255) / 256), dim3(256), 0, st, part, part_abs, n, B,
                      d_out, d_aux);
@@ -486,5 +650,7 @@ extern template int run_haf<ArF64, false, HM_CERT>(int, const cplx*, int, cplx*,
 extern template int run_haf<ArF64, true, HM_CERT>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
 extern template int run_haf<ArDD, false, HM_PLAIN>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
 extern template int run_haf<ArDD, true, HM_PLAIN>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+extern template int run_haf<ArDD, false, HM_DDCERT>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+extern template int run_haf<ArDD, true, HM_DDCERT>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
 
 }  // namespace gbsk

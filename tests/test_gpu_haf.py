@@ -166,3 +166,44 @@ def test_certified_config1_relative_bound(ext, oracle):
     exact = oracle.haf_q(A)
     assert np.all(np.abs(v - exact) <= bound)
     assert np.median(bound / np.abs(exact)) < 1e-11
+
+
+# ------------------------------------------------------------ certified dd
+@pytest.mark.parametrize("N", [2, 4, 8, 12])
+@pytest.mark.parametrize("func", ["haf", "lhaf"])
+def test_dd_certified_encloses_quad_oracle(ext, oracle, func, N):
+    print(f"{func} dd certified N={N}", flush=True)
+    A = rng_symmetric(6, N, 70 * N + (func == "lhaf"))
+    v, bound = getattr(ext, f"{func}_dd_certified")(A)
+    assert np.array_equal(v, getattr(ext, f"{func}_dd")(A))   # the dd tier's value
+    exact = getattr(oracle, f"{func}_q")(A)
+    assert np.all(np.isfinite(bound))
+    assert np.all(np.abs(v - exact) <= bound)
+    # the final rounding to binary64 dominates unless sum|term| / |value| is huge
+    scale = getattr(oracle, f"{func}_abs")(A)
+    assert np.all(bound <= 4 * EPS * (np.abs(v.real) + np.abs(v.imag)) + 1e-25 * scale)
+
+
+@pytest.mark.parametrize("func", ["haf", "lhaf"])
+def test_dd_certified_large(ext, oracle, func):
+    print(f"{func} dd certified N=16", flush=True)
+    A = rng_symmetric(2, 16, 80 + (func == "lhaf"), 0.5)
+    v, bound = getattr(ext, f"{func}_dd_certified")(A)
+    assert np.array_equal(v, getattr(ext, f"{func}_dd")(A))
+    assert np.all(np.abs(v - getattr(oracle, f"{func}_dd")(A)) <= bound)
+    assert np.all(bound <= 8 * EPS * (np.abs(v.real) + np.abs(v.imag)))
+
+
+def test_dd_certified_cancellation(ext, oracle):
+    """[[1, 3], [3, d - 3]] (+) R: lhaf = d lhaf(R) cancels; dd certifies it."""
+    g = np.random.default_rng(3)
+    R = g.standard_normal((4, 4)) + 1j * g.standard_normal((4, 4))
+    A = np.zeros((6, 6), complex)
+    A[:2, :2] = [[1.0, 3.0], [3.0, 1e-9 - 3.0]]
+    A[2:, 2:] = R + R.T
+    v, bound = ext.lhaf_dd_certified(A[None])
+    exact = oracle.lhaf_q(A[None])[0]
+    assert abs(v[0] - exact) <= bound[0]
+    assert bound[0] / abs(exact) < 1e-14
+    v0, b0 = ext.haf_dd_certified(np.zeros((2, 0, 0), complex))
+    assert np.all(v0 == 1) and np.all(b0 == 0)

@@ -106,14 +106,16 @@ def test_batch_graph_replays_match_eager():
 
 def test_batch_budget_and_errors():
     from gbskernels_amd import _lib
-    from gbskernels_amd.serving import Batch
+    from gbskernels_amd.serving import Batch, pack
     items = [("perm", "fp64", rng_complex((12, 12), 5))] * 8
     with Batch(budget_bytes=1024) as b:
         with pytest.raises(_lib.GbskError):
             b.run(items)
     with Batch() as b:
+        # a tier code the library does not know is refused as unimplemented
+        f, t, d, flat, off = pack([("perm", "fp64", rng_complex((3, 3), 1))])
         with pytest.raises(NotImplementedError):
-            b.run([("tor", "dd_certified", threshold_O(2, np.random.default_rng(0)))])
+            b.run_packed(f, np.full(1, 9, np.int32), d, flat, off)
         with pytest.raises(ValueError):
             b.run([("perm", "fp64", rng_complex((30, 30), 1))])
         # the handle stays usable after a rejected batch
@@ -136,10 +138,16 @@ def test_server_submit_wait_bitwise():
         assert same(v, a, w, c), (f, t, A.shape)
 
 
-def test_server_auto_escalates_to_dd():
+def test_server_auto_escalates_to_dd(oracle):
     from gbskernels_amd.serving import Server
     d = 1e-11
-    cancel = {"perm": np.array([[1, 2], [-1, 2 + d]]), "lhaf": np.array([[1, 3], [3, d - 3]])}
+    # lhaf: [[1, 3], [3, d - 3]] (+) R, lhaf = d lhaf(R); the direct sum spreads
+    # the cancellation over many subset terms, which is what kappa sees (a lone
+    # 2 x 2 block cancels inside a single power-trace term)
+    L = np.zeros((6, 6), complex)
+    L[:2, :2] = [[1.0, 3.0], [3.0, d - 3.0]]
+    L[2:, 2:] = rng_symmetric(1, 4, 5)[0]
+    cancel = {"perm": np.array([[1, 2], [-1, 2 + d]]), "lhaf": L}
     good = rng_complex((10, 10), 3, 0.5)
     with Server(slots=2) as s:
         t_bad = s.submit("perm", cancel["perm"], "auto")
@@ -154,8 +162,10 @@ def test_server_auto_escalates_to_dd():
     assert ab == alone("perm", "kappa", cancel["perm"])[1]
     assert il.tier_used == "dd" and vl == alone("lhaf", "dd", cancel["lhaf"])[0]
     assert ig.tier_used == "kappa" and (vg, ag) == alone("perm", "kappa", good)
-    # both values are exact in binary64 (Sterbenz): fl(2 + d) - 2 and fl(d - 3) + 3
-    assert vb == (np.float64(2 + d) - 2.0) and vl == (np.float64(d - 3) + 3.0)
+    # perm: fl(2 + d) - 2 is exact in binary64 (Sterbenz); lhaf against the oracle
+    assert vb == (np.float64(2 + d) - 2.0)
+    exact = oracle.lhaf_q(L[None])[0]
+    assert abs(vl - exact) <= 1e-12 * abs(exact)
 
 
 def test_server_replay_poisson_and_threads():

@@ -64,7 +64,8 @@ def test_tor_certified_closed_form_and_empty(ext):
         fa = Fraction(a)                       # tor(a I_2n) = (a / (1 - a))^n exactly
         exact = (fa / (1 - fa)) ** nm
         assert abs(Fraction(v[0].real) - exact) <= Fraction(bound[0]) and v[0].imag == 0
-        assert bound[0] < 1e-13
+        # sum|term| = (1 + 1 / (1 - a))^n: the bound is a few ulps of it
+        assert bound[0] < 16 * nm * 2.0 ** -52 * (1 + 1 / (1 - a)) ** nm
     v, bound = ext.tor_certified(np.zeros((3, 0, 0), np.complex128))
     assert np.all(v == 1) and np.all(bound == 0)
 
@@ -107,7 +108,7 @@ def test_tor_certified_batch_independent(ext):
 
 # ---- certified double-double -------------------------------------------------
 
-@pytest.mark.parametrize("nm", [1, 2, 3, 5, 6])
+@pytest.mark.parametrize("nm", [2, 3, 5, 6])
 def test_tor_dd_certified_encloses_quad_oracle(ext, oracle, nm):
     print(f"tor dd certified nm={nm}", flush=True)
     O = physical_stack(nm, 4, 500 + nm)
@@ -145,3 +146,22 @@ def test_tor_dd_certified_cancellation_and_refusal(ext):
     assert np.isinf(bound[0])                           # I - O not SPD: refused
     v, bound = ext.tor_dd_certified(np.zeros((2, 0, 0), np.complex128))
     assert np.all(v == 1) and np.all(bound == 0)
+
+
+def test_tor_dd_certified_tiny_value_mpmath():
+    """One mode, O = [[0, d], [d, 0]]: tor = 1 / sqrt(1 - d^2) - 1 ~ d^2 / 2,
+    below the resolution of __float128 next to 1; checked with mpmath."""
+    import mpmath
+    for d in (1.2e-17, 3e-9, 0.25):
+        O = np.array([[[0.0, d], [d, 0.0]]], np.complex128)
+        v, bound = ext_mod().tor_dd_certified(O)
+        with mpmath.workdps(80):
+            exact = 1 / mpmath.sqrt(1 - mpmath.mpf(d) ** 2) - 1
+            assert abs(mpmath.mpf(v[0].real) - exact) <= mpmath.mpf(bound[0])
+        # informative: below the dd resolution of the O(1) subset terms
+        assert bound[0] <= 2.0 ** -51 * abs(v[0]) + 1e-45
+
+
+def ext_mod():
+    import gbskernels_ext
+    return gbskernels_ext
