@@ -38,6 +38,9 @@ sys.path.insert(0, str(ROOT))
 METRIC = "evals/s perm/haf n=16,24 fp64+dd; served p50/p99 latency; rel err vs mpmath"
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
+# HBM bytes of one perm_kernel<24, fp64> launch (1024 x 24x24), measured with
+# hardware counters (FETCH_SIZE, WRITE_SIZE) calibrated against a plain copy
+TRAFFIC_N24 = 10583040
 
 
 def workload():
@@ -76,6 +79,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--haf", type=int, default=1, help="also time haf/lhaf buckets at N=16,24 (0 skips)")
+    ap.add_argument("--tor", type=int, default=1,
+                    help="also time the torontonian (fp64, certified, certified-dd) at nm = 8, 12, 16, 20 (0 skips)")
     ap.add_argument("--served-requests", type=int, default=40000,
                     help="requests of the served mixed-size workload (0 skips)")
     ap.add_argument("--served-batch", type=int, default=2048, help="requests per batch / window")
@@ -176,7 +181,10 @@ def main() -> int:
     # compute-bound ("mfma" class of the schema), but on the FP64 vector ALU: the
     # Glynn walk has no matrix-product shape; peak = MI355X FP64 vector 78.6 TFLOP/s
     roofline = {"bound": "mfma", "achieved": dom_flops / dom_s / 1e12, "peak": PEAK_FP64 / 1e12,
-                "unit": "TFLOP/s", "frac": dom_flops / dom_s / PEAK_FP64, "traffic": None,
+                "unit": "TFLOP/s", "frac": dom_flops / dom_s / PEAK_FP64, "traffic": TRAFFIC_N24,
+                "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0), "
+                                "scaled by a 256 MiB copy; profiles/perm_glynn_n24_hbm_traffic.md; "
+                                "algorithmic input 9437184 B",
                 "kernel": f"perm_kernel<{dom_n},fp64>", "flops_per_launch": dom_flops,
                 "avg_launch_ms": kern_ms[dom_n], "pipe": "FP64 VALU (no MFMA)"}
 
@@ -193,6 +201,9 @@ def main() -> int:
     if args.haf:
         print("bench: haf/lhaf n=16,24", file=sys.stderr, flush=True)
         per_kernel.update(haf_kernels(L, torch, dev, stream))
+    if args.tor:
+        print("bench: tor nm=8,12,16,20 and haf config 1", file=sys.stderr, flush=True)
+        per_kernel.update(tor_kernels(L, torch, dev, stream))
     served = None
     if args.served_requests > 0:
         served = served_bench(args)
@@ -257,6 +268,67 @@ def haf_kernels(L, torch, dev, stream):
                 f = haf_flops(N, func == "lhaf") * B
                 out[f"{func}_{tname}_n{N}"] = {"batch": B, "avg_ms": t * 1e3, "evals_per_s": B / t,
                                                "tflops_canonical": f / t / 1e12 * (10 if tier else 1)}
+    return out
+
+
+def tor_flops(nm: int) -> float:
+    """canonical recursive-Cholesky torontonian flops (2-row appends + leaves)"""
+    from math import comb
+    return float(sum(comb(nm, k) * (8 * (2 * k) ** 2 + 4) for k in range(1, nm + 1)) + 3 * 2 ** nm)
+
+
+def tor_kernels(L, torch, dev, stream):
+    """Torontonian of threshold-detector O matrices (two-mode squeezed pairs,
+    r ~ U[0.3, 1.0], Haar interferometer, loss 0.7), 4096 per mode count
+    (512 distinct, tiled), fp64 / certified / certified-dd; plus the haf of
+    4096 real symmetric 8x8 matrices G + G^T, default_rng(0). Kernel time by
+    HIP events on the launch stream."""
+    from gbskernels_amd.workloads import threshold_O
+    out = {}
+
+    def timed(func, tier, m, reps):
+        B, dim = m.shape[0], m.shape[1]
+        d_in = torch.from_numpy(np.ascontiguousarray(m).view(np.float64)).to(dev)
+        d_out = torch.empty(2 * B, dtype=torch.float64, device=dev)
+        d_aux = torch.empty(B, dtype=torch.float64, device=dev)
+        wsb = int(L.gbs_workspace_bytes(func, tier, dim, B))
+        ws = torch.empty(max(wsb, 256), dtype=torch.uint8, device=dev)
+
+        def go():
+            rc = L.gbs_eval_device(func, tier, d_in.data_ptr(), dim, B, d_out.data_ptr(), d_aux.data_ptr(),
+                                   ws.data_ptr(), ws.numel(), stream.cuda_stream)
+            if rc:
+                raise RuntimeError(f"gbs_eval_device rc={rc}")
+        go()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            go()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        aux = d_aux.cpu().numpy()
+        return float(np.mean(ms)) * 1e-3, aux
+
+    for nm in (8, 12, 16, 20):
+        rng = np.random.default_rng(4000 + nm)
+        uniq = np.stack([threshold_O(nm, rng) for _ in range(512)]).astype(np.complex128)
+        m = np.ascontiguousarray(np.tile(uniq, (8, 1, 1)))
+        for tier, tname in ((0, "fp64"), (3, "certified"), (4, "dd_certified")):
+            B = m.shape[0] if (tier != 4 or nm <= 16) else 512
+            t, aux = timed(3, tier, m[:B], 1 if nm >= 16 else 3)
+            rec = {"batch": B, "avg_ms": t * 1e3, "evals_per_s": B / t,
+                   "tflops_canonical": tor_flops(nm) * B / t / 1e12 * (10 if tier == 4 else 1)}
+            if tier in (3, 4):
+                rec["certified_finite"] = float(np.mean(np.isfinite(aux)))
+            out[f"tor_{tname}_nm{nm}"] = rec
+    G = np.random.default_rng(0).standard_normal((4096, 8, 8))
+    A = (G + G.transpose(0, 2, 1)).astype(np.complex128)
+    t, _ = timed(1, 0, A, 5)
+    out["haf_fp64_config1_n8"] = {"batch": 4096, "avg_ms": t * 1e3, "evals_per_s": 4096 / t}
     return out
 
 

@@ -68,7 +68,8 @@ double canonical_flops(int func, int tier, int dim) {
 struct Bucket {
   int func, tier, dim;
   std::vector<int> items;
-  size_t in_off = 0, out_off = 0, ws_off = 0, ws_bytes = 0;
+  size_t in_off = 0, ws_off = 0, ws_bytes = 0;
+  size_t res_off = 0;  // bytes: values (16 B each) then auxiliaries (8 B each)
   double cost = 0.0;
   int stream = 0;     // index into the owner's stream pool
   bool done = false;  // per-bucket completion (server slots)
@@ -90,12 +91,12 @@ struct gbs_batch {
   std::vector<hipStream_t> streams;
   std::vector<hipEvent_t> joins;
   hipEvent_t ev_in = nullptr;
-  void *d_in = nullptr, *d_out = nullptr, *d_aux = nullptr, *d_ws = nullptr;
-  size_t cap_in = 0, cap_out = 0, cap_aux = 0, cap_ws = 0;
+  void *d_in = nullptr, *d_res = nullptr, *d_ws = nullptr;
+  size_t cap_in = 0, cap_res = 0, cap_ws = 0;
   cplx* h_in = nullptr;
-  cplx* h_out = nullptr;
-  double* h_aux = nullptr;
-  size_t hcap_in = 0, hcap_out = 0, hcap_aux = 0;
+  char* h_res = nullptr;  // per bucket: values then auxiliaries (one copy back each)
+  size_t hcap_in = 0, hcap_res = 0;
+  size_t res_total = 0;
   size_t budget = 0;
   bool use_graphs = false;
   std::vector<GraphEntry> graphs;  // small LRU keyed by bucket signature
@@ -151,7 +152,7 @@ static void drop_graphs(gbs_batch* b) {
 // first (!lpt: mean completion time of a served batch) and each is given to the
 // least-loaded of nstreams streams.
 static int plan(int n, const int* func, const int* tier, const int* dim, std::vector<Bucket>& out, size_t* in_elems,
-                size_t* ws_total, bool lpt, int nstreams) {
+                size_t* ws_total, size_t* res_total, bool lpt, int nstreams) {
   std::map<long long, int> index;
   out.clear();
   for (int i = 0; i < n; ++i) {
@@ -180,21 +181,22 @@ static int plan(int n, const int* func, const int* tier, const int* dim, std::ve
   else
     std::stable_sort(out.begin(), out.end(), [](const Bucket& a, const Bucket& c) { return a.cost < c.cost; });
   std::vector<double> load((size_t)(nstreams < 1 ? 1 : nstreams), 0.0);
-  size_t ie = 0, oe = 0, ws = 0;
+  size_t ie = 0, re = 0, ws = 0;
   for (auto& bk : out) {
     const size_t k = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
     bk.stream = (int)k;
     load[k] += bk.cost;
     bk.in_off = ie;
-    bk.out_off = oe;
+    bk.res_off = re;
     ie += bk.items.size() * (size_t)bk.dim * bk.dim;
-    oe += bk.items.size();
+    re += (bk.items.size() * (sizeof(cplx) + sizeof(double)) + 15) & ~(size_t)15;
     bk.ws_bytes = gbs_workspace_bytes(bk.func, bk.tier, bk.dim, (int)bk.items.size());
     bk.ws_off = ws;
     ws += (bk.ws_bytes + 255) & ~(size_t)255;
   }
   *in_elems = ie;
   *ws_total = ws;
+  *res_total = re;
   return 0;
 }
 
@@ -202,11 +204,11 @@ static int plan(int n, const int* func, const int* tier, const int* dim, std::ve
 static bool tier_has_aux(int tier) { return tier == GBSK_KAPPA || tier == GBSK_CERTIFIED || tier == GBSK_DD_CERTIFIED; }
 
 // device bytes a batch occupies (inputs, outputs, auxiliaries, workspace)
-static size_t batch_bytes(size_t in_elems, size_t n, size_t ws_total) {
-  return in_elems * sizeof(cplx) + n * (sizeof(cplx) + sizeof(double)) + ws_total;
+static size_t batch_bytes(size_t in_elems, size_t res_total, size_t ws_total) {
+  return in_elems * sizeof(cplx) + res_total + ws_total;
 }
 
-static int enqueue_device(gbs_batch* b, const std::vector<Bucket>& bks, size_t in_elems, size_t n) {
+static int enqueue_device(gbs_batch* b, const std::vector<Bucket>& bks, size_t in_elems) {
   hipStream_t s0 = b->streams[0];
   int rc = 0;
   if (in_elems) rc = (int)hipMemcpyAsync(b->d_in, b->h_in, in_elems * sizeof(cplx), hipMemcpyHostToDevice, s0);
@@ -220,18 +222,17 @@ static int enqueue_device(gbs_batch* b, const std::vector<Bucket>& bks, size_t i
       rc = (int)hipStreamWaitEvent(s, b->ev_in, 0);
       waited[bk.stream] = 1;
     }
+    const size_t cnt = bk.items.size();
+    char* res = (char*)b->d_res + bk.res_off;
     if (!rc)
-      rc = gbs_eval_device(bk.func, bk.tier, (const gbs_complex*)((cplx*)b->d_in + bk.in_off), bk.dim,
-                           (int)bk.items.size(), (gbs_complex*)((cplx*)b->d_out + bk.out_off),
-                           (double*)b->d_aux + bk.out_off, (char*)b->d_ws + bk.ws_off, bk.ws_bytes, (void*)s);
+      rc = gbs_eval_device(bk.func, bk.tier, (const gbs_complex*)((cplx*)b->d_in + bk.in_off), bk.dim, (int)cnt,
+                           (gbs_complex*)res, (double*)(res + cnt * sizeof(cplx)), (char*)b->d_ws + bk.ws_off,
+                           bk.ws_bytes, (void*)s);
     if (!rc && b->per_bucket) {
-      // this bucket's results come back on its own stream, then its event fires
-      const size_t cnt = bk.items.size();
-      rc = (int)hipMemcpyAsync(b->h_out + bk.out_off, (cplx*)b->d_out + bk.out_off, cnt * sizeof(cplx),
+      // this bucket's values and auxiliaries come back in one copy on its own
+      // stream, then its event fires
+      rc = (int)hipMemcpyAsync(b->h_res + bk.res_off, res, cnt * (sizeof(cplx) + sizeof(double)),
                                hipMemcpyDeviceToHost, s);
-      if (!rc)
-        rc = (int)hipMemcpyAsync(b->h_aux + bk.out_off, (double*)b->d_aux + bk.out_off, cnt * sizeof(double),
-                                 hipMemcpyDeviceToHost, s);
       if (!rc) rc = (int)hipEventRecord(b->bucket_ev[k], s);
     }
   }
@@ -243,8 +244,8 @@ static int enqueue_device(gbs_batch* b, const std::vector<Bucket>& bks, size_t i
     rc = (int)hipEventRecord(b->joins[k], b->streams[k]);
     if (!rc) rc = (int)hipStreamWaitEvent(s0, b->joins[k], 0);
   }
-  if (!rc) rc = (int)hipMemcpyAsync(b->h_out, b->d_out, n * sizeof(cplx), hipMemcpyDeviceToHost, s0);
-  if (!rc) rc = (int)hipMemcpyAsync(b->h_aux, b->d_aux, n * sizeof(double), hipMemcpyDeviceToHost, s0);
+  if (!rc && b->res_total)
+    rc = (int)hipMemcpyAsync(b->h_res, b->d_res, b->res_total, hipMemcpyDeviceToHost, s0);
   return rc;
 }
 
@@ -255,21 +256,20 @@ static int batch_submit(gbs_batch* b, int n, const int* func, const int* tier, c
   b->cur.clear();
   b->cur_n = 0;
   if (n <= 0) return 0;
-  size_t in_elems = 0, ws_total = 0;
-  int rc = plan(n, func, tier, dim, b->cur, &in_elems, &ws_total, !b->per_bucket, b->nstreams);
+  size_t in_elems = 0, ws_total = 0, res_total = 0;
+  int rc = plan(n, func, tier, dim, b->cur, &in_elems, &ws_total, &res_total, !b->per_bucket, b->nstreams);
   if (rc) return rc;
-  const size_t need = batch_bytes(in_elems, (size_t)n, ws_total);
+  b->res_total = res_total;
+  const size_t need = batch_bytes(in_elems, res_total, ws_total);
   if (b->budget && need > b->budget) return GBSK_ERR_WORKSPACE;
   // grow-only buffers; any reallocation changes addresses a captured graph
   // refers to, so cached graphs are dropped
   const unsigned long long g0 = b->grows;
   rc = grow_dev(&b->d_in, &b->cap_in, in_elems * sizeof(cplx) + 16, &b->grows);
-  if (!rc) rc = grow_dev(&b->d_out, &b->cap_out, (size_t)n * sizeof(cplx), &b->grows);
-  if (!rc) rc = grow_dev(&b->d_aux, &b->cap_aux, (size_t)n * sizeof(double), &b->grows);
+  if (!rc) rc = grow_dev(&b->d_res, &b->cap_res, res_total + 16, &b->grows);
   if (!rc) rc = grow_dev(&b->d_ws, &b->cap_ws, ws_total + 256, &b->grows);
   if (!rc) rc = grow_pinned(&b->h_in, &b->hcap_in, in_elems + 1, &b->grows);
-  if (!rc) rc = grow_pinned(&b->h_out, &b->hcap_out, (size_t)n, &b->grows);
-  if (!rc) rc = grow_pinned(&b->h_aux, &b->hcap_aux, (size_t)n, &b->grows);
+  if (!rc) rc = grow_pinned(&b->h_res, &b->hcap_res, res_total + 16, &b->grows);
   if (rc) return rc;
   if (b->grows != g0) drop_graphs(b);
   const std::vector<Bucket>& bks = b->cur;
@@ -292,6 +292,7 @@ static int batch_submit(gbs_batch* b, int n, const int* func, const int* tier, c
     std::vector<long long> sig;
     sig.reserve(bks.size() * 4 + 1);
     sig.push_back((long long)in_elems);
+    sig.push_back((long long)res_total);
     for (const Bucket& bk : bks) {
       sig.push_back(bk.func);
       sig.push_back(bk.tier);
@@ -305,7 +306,7 @@ static int batch_submit(gbs_batch* b, int n, const int* func, const int* tier, c
       hipGraph_t graph = nullptr;
       rc = (int)hipStreamBeginCapture(b->streams[0], hipStreamCaptureModeThreadLocal);
       if (!rc) {
-        int rc2 = enqueue_device(b, bks, in_elems, (size_t)n);
+        int rc2 = enqueue_device(b, bks, in_elems);
         rc = (int)hipStreamEndCapture(b->streams[0], &graph);
         if (rc2) rc = rc2;
       }
@@ -330,7 +331,7 @@ static int batch_submit(gbs_batch* b, int n, const int* func, const int* tier, c
     rc = (int)hipGraphLaunch(hit->exec, b->streams[0]);
     done = true;
   }
-  if (!done && !rc) rc = enqueue_device(b, bks, in_elems, (size_t)n);
+  if (!done && !rc) rc = enqueue_device(b, bks, in_elems);
   if (!rc && !b->per_bucket) rc = (int)hipEventRecord(b->ev_done, b->streams[0]);
   if (!rc) b->cur_n = n;
   return rc;
@@ -344,8 +345,9 @@ static int batch_complete(gbs_batch* b, gbs_complex* out, double* aux) {
   for (const Bucket& bk : b->cur) {
     const bool has_aux = tier_has_aux(bk.tier);
     for (size_t j = 0; j < bk.items.size(); ++j) {
-      out[bk.items[j]] = ((gbs_complex*)b->h_out)[bk.out_off + j];
-      if (aux) aux[bk.items[j]] = has_aux ? b->h_aux[bk.out_off + j] : 0.0;
+      const char* res = b->h_res + bk.res_off;
+      out[bk.items[j]] = ((const gbs_complex*)res)[j];
+      if (aux) aux[bk.items[j]] = has_aux ? ((const double*)(res + bk.items.size() * sizeof(cplx)))[j] : 0.0;
     }
   }
   ++b->stats.batches;
@@ -393,18 +395,24 @@ struct Req {
   bool finished = false;
 };
 
+// A batch slot: FREE -> ASSIGNED (dispatcher) -> SUBMITTED (the slot's worker
+// thread packed and queued it) -> FREE (dispatcher, all buckets complete).
+// Each slot has its own worker so that packing and launching, the host cost of
+// a batch, run for several batches at once.
+enum SlotState { SLOT_FREE = 0, SLOT_ASSIGNED = 1, SLOT_SUBMITTED = 2 };
 struct Slot {
   gbs_batch* b = nullptr;
   std::vector<Req*> reqs;  // local item index -> request
   size_t reserved = 0;     // admitted device bytes
   size_t remaining = 0;    // buckets not yet complete
-  bool busy = false;
+  int state = SLOT_FREE;
+  std::thread worker;
 };
 
 // device bytes one more item of (func, tier, dim) adds to a bucket of c items
 static size_t item_bytes(int func, int tier, int dim, int c) {
   const size_t w1 = gbs_workspace_bytes(func, tier, dim, c + 1), w0 = c ? gbs_workspace_bytes(func, tier, dim, c) : 0;
-  return (size_t)dim * dim * sizeof(cplx) + sizeof(cplx) + sizeof(double) + (w1 > w0 ? w1 - w0 : 0) + (c ? 0 : 256);
+  return (size_t)dim * dim * sizeof(cplx) + sizeof(cplx) + sizeof(double) + (w1 > w0 ? w1 - w0 : 0) + (c ? 0 : 256 + 16);
 }
 
 }  // namespace gbsk
@@ -413,13 +421,14 @@ struct gbs_server {
   gbs_server_config cfg{};
   int device = 0;
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_work, cv_done, cv_slot;
   std::deque<Req*> pending;
   std::map<long long, Req*> table;
   long long next_ticket = 1;
   std::vector<Slot> slots;
   size_t in_use = 0;
   bool stop = false;
+  bool stop_workers = false;
   std::thread th;
   gbs_server_stats stats{};
   unsigned long long in_flight = 0;
@@ -443,8 +452,9 @@ static void complete_bucket(gbs_server* s, Slot& sl, Bucket& bk, int err, std::v
       finish_req(r, t);
       continue;
     }
-    const gbs_complex v = ((gbs_complex*)sl.b->h_out)[bk.out_off + j];
-    const double ax = sl.b->h_aux[bk.out_off + j];
+    const char* res = sl.b->h_res + bk.res_off;
+    const gbs_complex v = ((const gbs_complex*)res)[j];
+    const double ax = ((const double*)(res + bk.items.size() * sizeof(cplx)))[j];
     if (r->tier == GBSK_AUTO && r->run_tier == GBSK_KAPPA) {
       r->out = v;
       r->aux = ax;
@@ -478,15 +488,65 @@ static void complete_bucket(gbs_server* s, Slot& sl, Bucket& bk, int err, std::v
   bk.done = true;
 }
 
+// release a slot whose batch is complete or failed (s->mu held)
+static void free_slot(gbs_server* s, Slot& sl) {
+  sl.state = SLOT_FREE;
+  s->in_use -= sl.reserved;
+  sl.reserved = 0;
+  sl.reqs.clear();
+}
+
+// The slot's worker: packs and queues each batch the dispatcher assigns.
+static void slot_worker(gbs_server* s, Slot* sl) {
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    s->cv_slot.wait(lk, [s, sl] { return sl->state == SLOT_ASSIGNED || s->stop_workers; });
+    if (sl->state != SLOT_ASSIGNED) return;
+    const int n = (int)sl->reqs.size();
+    std::vector<int> fn(n), tr(n), dm(n);
+    std::vector<const cplx*> ptrs(n);
+    for (int i = 0; i < n; ++i) {
+      fn[i] = sl->reqs[i]->func;
+      tr[i] = sl->reqs[i]->run_tier;
+      dm[i] = sl->reqs[i]->dim;
+      ptrs[i] = sl->reqs[i]->mat();
+    }
+    lk.unlock();  // pack and launch without holding the server lock
+    const int rc = batch_submit(sl->b, n, fn.data(), tr.data(), dm.data(), ptrs.data());
+    if (rc)
+      for (hipStream_t st : sl->b->streams) (void)hipStreamSynchronize(st);
+    lk.lock();
+    if (rc) {
+      // nothing of a failed submission is awaited: fail its requests
+      const long long t = now_ns();
+      for (Req* r : sl->reqs) {
+        r->status = rc;
+        finish_req(r, t);
+      }
+      for (auto& bk : sl->b->cur) bk.done = true;
+      sl->remaining = 0;
+      free_slot(s, *sl);
+      s->cv_done.notify_all();
+    } else {
+      sl->remaining = sl->b->cur.size();
+      s->stats.buckets += (unsigned long long)sl->remaining;
+      sl->state = SLOT_SUBMITTED;
+    }
+    s->cv_work.notify_all();
+  }
+}
+
 static void dispatcher(gbs_server* s) {
   (void)hipSetDevice(s->device);
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
     bool progress = false;
     bool any_busy = false;
-    // 1. completions
+    // 1. completions of submitted batches, bucket by bucket
     for (Slot& sl : s->slots) {
-      if (!sl.busy) continue;
+      if (sl.state == SLOT_ASSIGNED) any_busy = true;
+      if (sl.state != SLOT_SUBMITTED) continue;
       std::vector<Req*> requeue;
       bool fin = false;
       for (size_t k = 0; k < sl.b->cur.size(); ++k) {
@@ -503,18 +563,14 @@ static void dispatcher(gbs_server* s) {
         progress = true;
         s->cv_done.notify_all();
       }
-      if (sl.remaining == 0) {
-        sl.busy = false;
-        s->in_use -= sl.reserved;
-        sl.reserved = 0;
-        sl.reqs.clear();
-      } else {
+      if (sl.remaining == 0)
+        free_slot(s, sl);
+      else
         any_busy = true;
-      }
     }
     // 2. admission: everything pending that fits, into each free slot
     for (Slot& sl : s->slots) {
-      if (sl.busy || s->pending.empty()) continue;
+      if (sl.state != SLOT_FREE || s->pending.empty()) continue;
       std::vector<Req*> take;
       std::map<long long, int> counts;
       size_t bytes = 0;
@@ -541,50 +597,21 @@ static void dispatcher(gbs_server* s) {
         s->pending.pop_front();
       }
       if (take.empty()) continue;
-      const int n = (int)take.size();
-      std::vector<int> fn(n), tr(n), dm(n);
-      std::vector<const cplx*> ptrs(n);
       const long long t = now_ns();
-      for (int i = 0; i < n; ++i) {
-        fn[i] = take[i]->func;
-        tr[i] = take[i]->run_tier;
-        dm[i] = take[i]->dim;
-        ptrs[i] = take[i]->mat();
-        if (!take[i]->admitted) take[i]->admitted = t;
-      }
-      sl.reqs = take;
+      for (Req* r : take)
+        if (!r->admitted) r->admitted = t;
+      sl.reqs = std::move(take);
       sl.reserved = bytes;
       s->in_use += bytes;
-      s->in_flight += (unsigned long long)n;
-      lk.unlock();  // pack and launch without blocking submitters
-      const int rc = batch_submit(sl.b, n, fn.data(), tr.data(), dm.data(), ptrs.data());
-      lk.lock();
+      sl.state = SLOT_ASSIGNED;
+      ++s->stats.batches;
       progress = true;
       any_busy = true;
-      sl.busy = true;
-      ++s->stats.batches;
-      if (rc) {
-        // nothing of a failed submission is awaited: fail its requests
-        for (hipStream_t st : sl.b->streams) (void)hipStreamSynchronize(st);
-        for (Req* r : take) {
-          r->status = rc;
-          finish_req(r, now_ns());
-        }
-        for (auto& bk : sl.b->cur) bk.done = true;
-        sl.remaining = 0;
-        s->cv_done.notify_all();
-        sl.busy = false;
-        s->in_use -= sl.reserved;
-        sl.reserved = 0;
-        sl.reqs.clear();
-      } else {
-        sl.remaining = sl.b->cur.size();
-        s->stats.buckets += (unsigned long long)sl.remaining;
-      }
       unsigned long long inf = 0;
       for (const Slot& o : s->slots)
-        if (o.busy) inf += (unsigned long long)o.reqs.size();
+        if (o.state != SLOT_FREE) inf += (unsigned long long)o.reqs.size();
       if (inf > s->stats.max_in_flight) s->stats.max_in_flight = inf;
+      s->cv_slot.notify_all();
     }
     if (s->stop && s->pending.empty() && !any_busy) break;
     if (!progress) {
@@ -650,12 +677,10 @@ int gbs_batch_destroy(gbs_batch* b) {
       if (s) (void)hipStreamSynchronize(s);
     drop_graphs(b);
     if (b->d_in) (void)hipFree(b->d_in);
-    if (b->d_out) (void)hipFree(b->d_out);
-    if (b->d_aux) (void)hipFree(b->d_aux);
+    if (b->d_res) (void)hipFree(b->d_res);
     if (b->d_ws) (void)hipFree(b->d_ws);
     if (b->h_in) (void)hipHostFree(b->h_in);
-    if (b->h_out) (void)hipHostFree(b->h_out);
-    if (b->h_aux) (void)hipHostFree(b->h_aux);
+    if (b->h_res) (void)hipHostFree(b->h_res);
     for (auto e : b->joins)
       if (e) (void)hipEventDestroy(e);
     if (b->ev_in) (void)hipEventDestroy(b->ev_in);
@@ -692,12 +717,23 @@ int gbs_server_create(gbs_server** out, const gbs_server_config* cfg) {
   }
   if (!rc) {
     try {
+      for (auto& sl : s->slots) sl.worker = std::thread(slot_worker, s, &sl);
       s->th = std::thread(dispatcher, s);
     } catch (...) {
       rc = GBSK_ERR_ALLOC;
     }
   }
   if (rc) {
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      s->stop = true;
+      s->stop_workers = true;
+    }
+    s->cv_slot.notify_all();
+    s->cv_work.notify_all();
+    if (s->th.joinable()) s->th.join();
+    for (auto& sl : s->slots)
+      if (sl.worker.joinable()) sl.worker.join();
     for (auto& sl : s->slots)
       if (sl.b) gbs_batch_destroy(sl.b);
     delete s;
@@ -859,7 +895,14 @@ int gbs_server_destroy(gbs_server* s) {
     s->stop = true;
   }
   s->cv_work.notify_all();
-  if (s->th.joinable()) s->th.join();
+  if (s->th.joinable()) s->th.join();  // drains: every slot is free afterwards
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop_workers = true;
+  }
+  s->cv_slot.notify_all();
+  for (auto& sl : s->slots)
+    if (sl.worker.joinable()) sl.worker.join();
   for (auto& sl : s->slots)
     if (sl.b) gbs_batch_destroy(sl.b);
   for (auto& kv : s->table) delete kv.second;

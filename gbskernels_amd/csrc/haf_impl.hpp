@@ -187,7 +187,7 @@ __device__ inline void ddcert_group_reduce(ddc& v, double& e, ddc* red, double* 
 // HAF_CHUNK_ITERS * G consecutive subsets (G = 64 / S per wave iteration), one
 // workgroup per (instance, chunk), so that large N runs many waves per instance.
 // The split depends on (N, S) only; chunk partials are summed in chunk order.
-constexpr int HAF_CHUNK_ITERS = 4;
+constexpr int HAF_CHUNK_ITERS = 1;
 __host__ __device__ inline int haf_chunk_size(int S) { return HAF_CHUNK_ITERS * (64 / S); }
 __host__ __device__ inline int haf_binom(int n, int k) {
   if (k < 0 || k > n) return 0;

@@ -307,7 +307,7 @@ __device__ constexpr double perm_ddcert_depth() {
 //   aux      : kappa sum|term| (final or partial), may be null in plain mode
 template <int N, int MODE>
 __global__ __launch_bounds__(64) void perm_kernel(const cplx* __restrict__ mats, int B, cplx* __restrict__ out,
-                                                  double* __restrict__ aux) {
+                                                  double* __restrict__ aux, int g8) {
   using C = PermCfg<N, perm_is_dd(MODE) ? 1 : 0>;
   constexpr int LPM = C::LPM, MPW = 64 / LPM, W = C::W, M = C::M;
   __shared__ cplx AT[MPW][N * N];
@@ -323,8 +323,9 @@ __global__ __launch_bounds__(64) void perm_kernel(const cplx* __restrict__ mats,
     grp = wg;
     chunk = 0;
   } else {
-    const int64_t k = wg >> 3;
-    grp = (k / W) * 8 + (wg & 7);
+    // g8 = min(8, groups): with fewer than 8 groups no padding workgroups
+    const int64_t k = wg / g8;
+    grp = (k / W) * g8 + wg % g8;
     chunk = (int)(k % W);
   }
   const int64_t mat0 = grp * MPW;
@@ -438,15 +439,16 @@ static int launch_perm_n(const cplx* d_in, int B, cplx* d_out, double* d_aux, vo
   using C = PermCfg<N, perm_is_dd(MODE) ? 1 : 0>;
   constexpr int MPW = 64 / C::LPM, W = C::W;
   const int64_t groups = ((int64_t)B + MPW - 1) / MPW;
-  const int64_t grid = (W == 1) ? groups : ((groups + 7) / 8) * 8 * W;
+  const int g8 = groups < 8 ? (int)groups : 8;
+  const int64_t grid = (W == 1) ? groups : ((groups + g8 - 1) / g8) * g8 * W;
   if (W == 1) {
-    hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, d_in, B, d_out, d_aux);
+    hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, d_in, B, d_out, d_aux, g8);
   } else {
     // partials: values [B][W] cplx, then [B][W] of 16 bytes (kappa / bound
     // doubles, or the dd low parts), then (certified dd) [B][W] bounds
     cplx* part = reinterpret_cast<cplx*>(ws);
     double* part_aux = reinterpret_cast<double*>(part + (size_t)B * W);
-    hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, d_in, B, part, part_aux);
+    hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, d_in, B, part, part_aux, g8);
     hipLaunchKernelGGL((perm_reduce_kernel<N, MODE>), dim3((unsigned)((B + 255) / 256)), dim3(256), 0, st, part,
                        part_aux, B, d_out, d_aux);
   }

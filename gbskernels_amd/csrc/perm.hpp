@@ -27,7 +27,11 @@ struct PermCfg {
   static constexpr int LOGL_ = LOGT - LOGMIN < 0 ? 0 : (LOGT - LOGMIN > 6 ? 6 : LOGT - LOGMIN);
   static constexpr int LOGL = LOGL_;
   static constexpr int LPM = 1 << LOGL;
-  static constexpr int MAXM = DD ? 8 : 11;  // at most 2^MAXM terms per lane, else split over workgroups
+  // at most 2^MAXM terms per lane, else split over workgroups: short enough
+  // that one large matrix spreads over hundreds of waves (latency of a small
+  // batch), long enough that the per-lane start (n^2 row-sum terms) stays a
+  // few percent of the walk
+  static constexpr int MAXM = DD ? 6 : 9;
   static constexpr int LOGW = LOGT - LOGL - MAXM > 0 ? LOGT - LOGL - MAXM : 0;
   static constexpr int W = 1 << LOGW;
   static constexpr int M = LOGT - LOGL - LOGW;
