@@ -198,15 +198,23 @@ def main() -> int:
             per_kernel[f"perm_dd_n{n}"] = {"batch": m.shape[0], "avg_ms": dd_kern_ms[n],
                                            "evals_per_s": m.shape[0] / (dd_kern_ms[n] * 1e-3)}
 
+    # the side measurements report a Python-level failure in the line instead of
+    # losing the headline figure
+    def side(name, fn):
+        print(f"bench: {name}", file=sys.stderr, flush=True)
+        try:
+            return fn()
+        except Exception as exc:  # noqa: BLE001
+            print(f"bench: {name} failed: {exc!r}", file=sys.stderr, flush=True)
+            return {"error": repr(exc)}
+
     if args.haf:
-        print("bench: haf/lhaf n=16,24", file=sys.stderr, flush=True)
-        per_kernel.update(haf_kernels(L, torch, dev, stream))
+        per_kernel.update(side("haf/lhaf n=16,24", lambda: haf_kernels(L, torch, dev, stream)))
     if args.tor:
-        print("bench: tor nm=8,12,16,20 and haf config 1", file=sys.stderr, flush=True)
-        per_kernel.update(tor_kernels(L, torch, dev, stream))
+        per_kernel.update(side("tor nm=8,12,16,20 and haf config 1", lambda: tor_kernels(L, torch, dev, stream)))
     served = None
     if args.served_requests > 0:
-        served = served_bench(args)
+        served = side("served mix", lambda: served_bench(args))
 
     cpu = None
     if not args.no_cpu:

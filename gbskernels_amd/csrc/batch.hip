@@ -73,6 +73,10 @@ struct Bucket {
   double cost = 0.0;
   int stream = 0;     // index into the owner's stream pool
   bool done = false;  // per-bucket completion (server slots)
+  // hafnian buckets of one (function, tier, size class) share their launches:
+  // the leader (first in launch order) launches all members, itself included
+  int leader = -1;           // index of the group's leader, -1 if not grouped
+  std::vector<int> members;  // (leader only) the group, leader first
 };
 
 struct GraphEntry {
@@ -180,12 +184,41 @@ static int plan(int n, const int* func, const int* tier, const int* dim, std::ve
     std::stable_sort(out.begin(), out.end(), [](const Bucket& a, const Bucket& c) { return a.cost > c.cost; });
   else
     std::stable_sort(out.begin(), out.end(), [](const Bucket& a, const Bucket& c) { return a.cost < c.cost; });
+  // group hafnian buckets: same function and tier, sizes below / from 14
+  // (small and large ones complete separately), at most HAF_MAX_JOBS each
+  {
+    std::map<long long, int> open;
+    for (int k = 0; k < (int)out.size(); ++k) {
+      Bucket& bk = out[k];
+      if (bk.func != GBSK_HAF && bk.func != GBSK_LHAF) continue;
+      const long long key = ((long long)bk.func << 8) | ((long long)bk.tier << 1) | (bk.dim >= 14 ? 1 : 0);
+      auto it = open.find(key);
+      if (it != open.end() && (int)out[it->second].members.size() < HAF_MAX_JOBS) {
+        bk.leader = it->second;
+        out[it->second].members.push_back(k);
+      } else {
+        bk.leader = k;
+        bk.members.assign(1, k);
+        open[key] = k;
+      }
+    }
+  }
   std::vector<double> load((size_t)(nstreams < 1 ? 1 : nstreams), 0.0);
   size_t ie = 0, re = 0, ws = 0;
-  for (auto& bk : out) {
-    const size_t k = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
-    bk.stream = (int)k;
-    load[k] += bk.cost;
+  for (int kb = 0; kb < (int)out.size(); ++kb) {
+    Bucket& bk = out[kb];
+    if (bk.leader >= 0 && bk.leader != kb) {
+      bk.stream = out[bk.leader].stream;  // members run in their leader's launches
+    } else {
+      double cost = 0.0;
+      if (bk.leader == kb)
+        for (int m : bk.members) cost += out[m].cost;
+      else
+        cost = bk.cost;
+      const size_t k = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+      bk.stream = (int)k;
+      load[k] += cost;
+    }
     bk.in_off = ie;
     bk.res_off = re;
     ie += bk.items.size() * (size_t)bk.dim * bk.dim;
@@ -222,18 +255,35 @@ static int enqueue_device(gbs_batch* b, const std::vector<Bucket>& bks, size_t i
       rc = (int)hipStreamWaitEvent(s, b->ev_in, 0);
       waited[bk.stream] = 1;
     }
-    const size_t cnt = bk.items.size();
-    char* res = (char*)b->d_res + bk.res_off;
-    if (!rc)
+    if (bk.leader >= 0 && bk.leader != (int)k) continue;  // launched with its group
+    auto res_of = [&](const Bucket& x) { return (char*)b->d_res + x.res_off; };
+    if (!rc && bk.leader == (int)k) {
+      HafJob jobs[HAF_MAX_JOBS];
+      int nj = 0;
+      for (int m : bk.members) {
+        const Bucket& x = bks[m];
+        const size_t c = x.items.size();
+        jobs[nj++] = HafJob{(const cplx*)b->d_in + x.in_off, (cplx*)res_of(x), (double*)(res_of(x) + c * sizeof(cplx)),
+                            (char*)b->d_ws + x.ws_off, x.dim, (int)c};
+      }
+      rc = haf_launch_multi(bk.func == GBSK_LHAF, bk.tier, jobs, nj, s);
+    } else if (!rc) {
+      const size_t cnt = bk.items.size();
       rc = gbs_eval_device(bk.func, bk.tier, (const gbs_complex*)((cplx*)b->d_in + bk.in_off), bk.dim, (int)cnt,
-                           (gbs_complex*)res, (double*)(res + cnt * sizeof(cplx)), (char*)b->d_ws + bk.ws_off,
-                           bk.ws_bytes, (void*)s);
+                           (gbs_complex*)res_of(bk), (double*)(res_of(bk) + cnt * sizeof(cplx)),
+                           (char*)b->d_ws + bk.ws_off, bk.ws_bytes, (void*)s);
+    }
     if (!rc && b->per_bucket) {
-      // this bucket's values and auxiliaries come back in one copy on its own
-      // stream, then its event fires
-      rc = (int)hipMemcpyAsync(b->h_res + bk.res_off, res, cnt * (sizeof(cplx) + sizeof(double)),
-                               hipMemcpyDeviceToHost, s);
-      if (!rc) rc = (int)hipEventRecord(b->bucket_ev[k], s);
+      // each bucket's values and auxiliaries come back in one copy on the
+      // launching stream, then its event fires (a group's buckets together)
+      const std::vector<int> one(1, (int)k);
+      for (int m : (bk.leader == (int)k ? bk.members : one)) {
+        const Bucket& x = bks[m];
+        if (!rc)
+          rc = (int)hipMemcpyAsync(b->h_res + x.res_off, res_of(x), x.items.size() * (sizeof(cplx) + sizeof(double)),
+                                   hipMemcpyDeviceToHost, s);
+        if (!rc) rc = (int)hipEventRecord(b->bucket_ev[m], s);
+      }
     }
   }
   if (b->per_bucket) return rc;

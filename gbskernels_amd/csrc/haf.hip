@@ -37,29 +37,57 @@ int haf_max_dim(bool, int tier) {
   return 0;
 }
 
+static int run_multi(bool loop, int tier, const HafJob* jobs, int njobs, hipStream_t st) {
+  switch (tier) {
+    case GBSK_DD:
+      return loop ? run_haf_multi<ArDD, true, HM_PLAIN>(jobs, njobs, st)
+                  : run_haf_multi<ArDD, false, HM_PLAIN>(jobs, njobs, st);
+    case GBSK_KAPPA:
+      return loop ? run_haf_multi<ArF64, true, HM_KAPPA>(jobs, njobs, st)
+                  : run_haf_multi<ArF64, false, HM_KAPPA>(jobs, njobs, st);
+    case GBSK_CERTIFIED:
+      return loop ? run_haf_multi<ArF64, true, HM_CERT>(jobs, njobs, st)
+                  : run_haf_multi<ArF64, false, HM_CERT>(jobs, njobs, st);
+    case GBSK_DD_CERTIFIED:
+      return loop ? run_haf_multi<ArDD, true, HM_DDCERT>(jobs, njobs, st)
+                  : run_haf_multi<ArDD, false, HM_DDCERT>(jobs, njobs, st);
+  }
+  return loop ? run_haf_multi<ArF64, true, HM_PLAIN>(jobs, njobs, st)
+              : run_haf_multi<ArF64, false, HM_PLAIN>(jobs, njobs, st);
+}
+
+int haf_launch_multi(bool loop, int tier, const HafJob* jobs, int njobs, hipStream_t st) {
+  if (njobs < 0 || njobs > HAF_MAX_JOBS || (njobs && !jobs)) return GBSK_ERR_ARG;
+  if (tier < GBSK_FP64 || tier > GBSK_DD_CERTIFIED) return GBSK_ERR_FUNC;
+  const bool aux = tier == GBSK_KAPPA || tier == GBSK_CERTIFIED || tier == GBSK_DD_CERTIFIED;
+  HafJob work[HAF_MAX_JOBS];
+  int nw = 0;
+  for (int j = 0; j < njobs; ++j) {
+    const HafJob& J = jobs[j];
+    if (J.B <= 0) continue;
+    if (J.N < 0 || J.N > HAF_MAX_N) return GBSK_ERR_DIM;
+    if (!J.in || !J.out || (aux && !J.aux)) return GBSK_ERR_ARG;
+    if (J.N == 0 || (J.N & 1)) {  // empty matrix or odd size: closed forms
+      hipLaunchKernelGGL(haf_trivial_kernel, dim3((J.B + 255) / 256), dim3(256), 0, st, J.N, loop, J.B, J.out,
+                         aux ? J.aux : nullptr, tier == GBSK_KAPPA);
+      continue;
+    }
+    if (!J.ws) return GBSK_ERR_WORKSPACE;
+    work[nw++] = J;
+  }
+  if (nw) {
+    const int rc = run_multi(loop, tier, work, nw, st);
+    if (rc) return rc;
+  }
+  return (int)hipGetLastError();
+}
+
 int haf_launch(bool loop, int tier, int N, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws,
                hipStream_t st) {
   if (B <= 0) return 0;
   if (N < 0 || N > HAF_MAX_N) return GBSK_ERR_DIM;
-  if (tier < GBSK_FP64 || tier > GBSK_DD_CERTIFIED) return GBSK_ERR_FUNC;
-  if (N == 0 || (N & 1)) {
-    const bool aux = tier == GBSK_KAPPA || tier == GBSK_CERTIFIED || tier == GBSK_DD_CERTIFIED;
-    hipLaunchKernelGGL(haf_trivial_kernel, dim3((B + 255) / 256), dim3(256), 0, st, N, loop, B, d_out,
-                       aux ? d_aux : nullptr, tier == GBSK_KAPPA);
-    return (int)hipGetLastError();
-  }
-  if (!ws) return GBSK_ERR_WORKSPACE;
-  if (tier == GBSK_DD) return loop ? run_haf<ArDD, true, HM_PLAIN>(N, d_in, B, d_out, nullptr, ws, st)
-                                   : run_haf<ArDD, false, HM_PLAIN>(N, d_in, B, d_out, nullptr, ws, st);
-  if (tier == GBSK_KAPPA) return loop ? run_haf<ArF64, true, HM_KAPPA>(N, d_in, B, d_out, d_aux, ws, st)
-                                      : run_haf<ArF64, false, HM_KAPPA>(N, d_in, B, d_out, d_aux, ws, st);
-  if (tier == GBSK_DD_CERTIFIED) return loop ? run_haf<ArDD, true, HM_DDCERT>(N, d_in, B, d_out, d_aux, ws, st)
-                                             : run_haf<ArDD, false, HM_DDCERT>(N, d_in, B, d_out, d_aux, ws, st);
-  if (tier == GBSK_CERTIFIED) return loop ? run_haf<ArF64, true, HM_CERT>(N, d_in, B, d_out, d_aux, ws, st)
-                                          : run_haf<ArF64, false, HM_CERT>(N, d_in, B, d_out, d_aux, ws, st);
-  return loop ? run_haf<ArF64, true, HM_PLAIN>(N, d_in, B, d_out, nullptr, ws, st)
-              : run_haf<ArF64, false, HM_PLAIN>(N, d_in, B, d_out, nullptr, ws, st);
+  const HafJob j{d_in, d_out, d_aux, ws, N, B};
+  return haf_launch_multi(loop, tier, &j, 1, st);
 }
-
 
 }  // namespace gbsk

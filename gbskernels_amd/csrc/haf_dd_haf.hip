@@ -1,6 +1,6 @@
-// haf_dd_haf.hip -- instantiation unit: run_haf<ArDD, LOOP=false, HM_PLAIN>
+// haf_dd_haf.hip -- instantiation unit: run_haf_multi<ArDD, LOOP=false, HM_PLAIN>
 #include "haf_impl.hpp"
 
 namespace gbsk {
-template int run_haf<ArDD, false, HM_PLAIN>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+template int run_haf_multi<ArDD, false, HM_PLAIN>(const HafJob*, int, hipStream_t);
 }  // namespace gbsk

@@ -1,6 +1,6 @@
-// haf_ddcert_lhaf.hip -- instantiation unit: run_haf<ArDD, LOOP=true, HM_DDCERT>
+// haf_ddcert_lhaf.hip -- instantiation unit: run_haf_multi<ArDD, LOOP=true, HM_DDCERT>
 #include "haf_impl.hpp"
 
 namespace gbsk {
-template int run_haf<ArDD, true, HM_DDCERT>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+template int run_haf_multi<ArDD, true, HM_DDCERT>(const HafJob*, int, hipStream_t);
 }  // namespace gbsk

@@ -1,6 +1,6 @@
-// haf_f64_lhaf.hip -- instantiation unit: run_haf<ArF64, LOOP=true, HM_PLAIN>
+// haf_f64_lhaf.hip -- instantiation unit: run_haf_multi<ArF64, LOOP=true, HM_PLAIN>
 #include "haf_impl.hpp"
 
 namespace gbsk {
-template int run_haf<ArF64, true, HM_PLAIN>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+template int run_haf_multi<ArF64, true, HM_PLAIN>(const HafJob*, int, hipStream_t);
 }  // namespace gbsk

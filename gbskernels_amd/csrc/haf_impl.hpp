@@ -216,15 +216,26 @@ __device__ inline double haf_cert_depth(int S, int n, int count) {
   return 64.0 * (n + 1) * (S + 8) + 8.0 * count + 16.0 * n * n + 256.0;
 }
 
+// Launch descriptor of one phase over several buckets (kernel argument, by
+// value): bucket j owns workgroups [wg_begin[j], wg_begin[j+1]).
+template <class T>
+struct HafPhaseJobs {
+  const cplx* mats[HAF_MAX_JOBS];
+  T* part[HAF_MAX_JOBS];
+  double* part_abs[HAF_MAX_JOBS];
+  int N[HAF_MAX_JOBS];
+  int B[HAF_MAX_JOBS];
+  int wg_begin[HAF_MAX_JOBS + 1];
+  int n;
+};
+
 // ---------------------------------------------------------------- phase kernel
 // One workgroup (one wave) = one instance; evaluates every subset of size
 // m = S/2 and writes the phase partial (and sum |term| in kappa mode, the
 // bound in certified mode).
 //   part[b * np + off_S + chunk]      chunk partial (np = haf_nparts(n))
 template <class Ar, bool LOOP, int S, int MODE>
-__global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ mats, int N, int B,
-                                                       typename Ar::T* __restrict__ part,
-                                                       double* __restrict__ part_abs) {
+__global__ __launch_bounds__(64) void haf_phase_kernel(const HafPhaseJobs<typename Ar::T> jobs) {
   using T = typename Ar::T;
   constexpr bool KAPPA = (MODE == HM_KAPPA);
   constexpr bool CERT = (MODE == HM_CERT);
@@ -242,11 +253,19 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
   __shared__ double kge[ANYC ? G : 1][HAF_MAX_PAIRS + 1];
   __shared__ double ese[ANYC ? G : 1][HAF_MAX_PAIRS + 1];
   const int lane = threadIdx.x;
-  const int n = N / 2;
+  // this workgroup's bucket (wave-uniform scan of at most HAF_MAX_JOBS entries)
+  int jb = 0;
+  while (jb + 1 < jobs.n && (int)blockIdx.x >= jobs.wg_begin[jb + 1]) ++jb;
+  const cplx* __restrict__ mats = jobs.mats[jb];
+  const int N = jobs.N[jb];
+  typename Ar::T* __restrict__ part = jobs.part[jb];
+  double* __restrict__ part_abs = jobs.part_abs[jb];
+  const int local = (int)blockIdx.x - jobs.wg_begin[jb];
   const int nch = haf_nchunks(N / 2, S);
-  const int64_t b = blockIdx.x / nch;
-  const int chunk = (int)(blockIdx.x % nch);
-  const int64_t pslot = b * haf_nparts(N / 2) + haf_part_offset(N / 2, S) + chunk;
+  const int64_t b = local / nch;
+  const int chunk = local % nch;
+  const int n = N / 2;
+  const int64_t pslot = b * haf_nparts(n) + haf_part_offset(n, S) + chunk;
   // stage A (row-major N x N); the ordinary hafnian zeroes the diagonal
   for (int e = lane; e < N * N; e += 64) {
     cplx v = mats[b * N * N + e];
@@ -584,12 +603,32 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const cplx* __restrict__ 
   }
 }
 
+// Reduce descriptor over several buckets: bucket j owns instances
+// [b_begin[j], b_begin[j+1]) of the launch's thread range.
+template <class T>
+struct HafReduceJobs {
+  const T* part[HAF_MAX_JOBS];
+  const double* part_abs[HAF_MAX_JOBS];
+  cplx* out[HAF_MAX_JOBS];
+  double* aux[HAF_MAX_JOBS];
+  int n[HAF_MAX_JOBS];
+  int b_begin[HAF_MAX_JOBS + 1];
+  int njobs;
+};
+
 // out[b] = sum over phases m = 1..n in order (fixed association)
 template <class Ar, int MODE>
-__global__ void haf_reduce_kernel(const typename Ar::T* __restrict__ part, const double* __restrict__ part_abs,
-                                  int n, int B, cplx* __restrict__ out, double* __restrict__ aux) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
+__global__ void haf_reduce_kernel(const HafReduceJobs<typename Ar::T> jobs) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= jobs.b_begin[jobs.njobs]) return;
+  int jb = 0;
+  while (jb + 1 < jobs.njobs && t >= jobs.b_begin[jb + 1]) ++jb;
+  const typename Ar::T* __restrict__ part = jobs.part[jb];
+  const double* __restrict__ part_abs = jobs.part_abs[jb];
+  cplx* __restrict__ out = jobs.out[jb];
+  double* __restrict__ aux = jobs.aux[jb];
+  const int n = jobs.n[jb];
+  const int64_t b = t - jobs.b_begin[jb];
   const int np = haf_nparts(n);
   typename Ar::T s = part[b * np];
   double a = MODE != HM_PLAIN ? part_abs[b * np] : 0.0;
@@ -614,43 +653,73 @@ __global__ void haf_reduce_kernel(const typename Ar::T* __restrict__ part, const
 }
 
 template <class Ar, bool LOOP, int MODE, int S>
-static void launch_phase(const cplx* d_in, int N, int B, typename Ar::T* part, double* part_abs, hipStream_t st) {
-  const unsigned grid = (unsigned)((int64_t)B * haf_nchunks(N / 2, S));
-  hipLaunchKernelGGL((haf_phase_kernel<Ar, LOOP, S, MODE>), dim3(grid), dim3(64),
-                     (size_t)N * N * sizeof(cplx), st, d_in, N, B, part, part_abs);
+static void launch_phase(const HafJob* jobs, int njobs, typename Ar::T* const* part, double* const* part_abs,
+                         hipStream_t st) {
+  HafPhaseJobs<typename Ar::T> pj{};
+  int wg = 0, maxN = 0;
+  for (int j = 0; j < njobs; ++j) {
+    if (jobs[j].N / 2 < S / 2) continue;  // this size has no subsets of size S
+    const int k = pj.n++;
+    pj.mats[k] = jobs[j].in;
+    pj.part[k] = part[j];
+    pj.part_abs[k] = part_abs[j];
+    pj.N[k] = jobs[j].N;
+    pj.B[k] = jobs[j].B;
+    pj.wg_begin[k] = wg;
+    wg += jobs[j].B * haf_nchunks(jobs[j].N / 2, S);
+    maxN = jobs[j].N > maxN ? jobs[j].N : maxN;
+  }
+  if (pj.n == 0 || wg == 0) return;
+  pj.wg_begin[pj.n] = wg;
+  hipLaunchKernelGGL((haf_phase_kernel<Ar, LOOP, S, MODE>), dim3((unsigned)wg), dim3(64),
+                     (size_t)maxN * maxN * sizeof(cplx), st, pj);
 }
 
 template <class Ar, bool LOOP, int MODE, int... Ss>
-static void launch_phases(const cplx* d_in, int N, int B, typename Ar::T* part, double* part_abs, hipStream_t st,
-                          std::integer_sequence<int, Ss...>) {
-  const int n = N / 2;
-  ((2 * (Ss + 1) <= 2 * n ? (launch_phase<Ar, LOOP, MODE, 2 * (Ss + 1)>(d_in, N, B, part, part_abs, st), 0) : 0),
-   ...);
+static void launch_phases(const HafJob* jobs, int njobs, typename Ar::T* const* part, double* const* part_abs,
+                          hipStream_t st, std::integer_sequence<int, Ss...>) {
+  int maxn = 0;
+  for (int j = 0; j < njobs; ++j) maxn = jobs[j].N / 2 > maxn ? jobs[j].N / 2 : maxn;
+  ((Ss + 1 <= maxn ? (launch_phase<Ar, LOOP, MODE, 2 * (Ss + 1)>(jobs, njobs, part, part_abs, st), 0) : 0), ...);
 }
 
+// jobs: even N >= 2 and B >= 1 each, at most HAF_MAX_JOBS
 template <class Ar, bool LOOP, int MODE>
-int run_haf(int N, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
+int run_haf_multi(const HafJob* jobs, int njobs, hipStream_t st) {
   using T = typename Ar::T;
-  const int n = N / 2;
-  T* part = reinterpret_cast<T*>(ws);
-  double* part_abs = reinterpret_cast<double*>(part + (size_t)B * haf_nparts(n));
-  launch_phases<Ar, LOOP, MODE>(d_in, N, B, part, part_abs, st, std::make_integer_sequence<int, HAF_MAX_PAIRS>{});
-  hipLaunchKernelGGL((haf_reduce_kernel<Ar, MODE>), dim3((B + 255) / 256), dim3(256), 0, st, part, part_abs, n, B,
-                     d_out, d_aux);
+  T* part[HAF_MAX_JOBS];
+  double* part_abs[HAF_MAX_JOBS];
+  HafReduceJobs<T> rj{};
+  int bsum = 0;
+  for (int j = 0; j < njobs; ++j) {
+    part[j] = reinterpret_cast<T*>(jobs[j].ws);
+    part_abs[j] = reinterpret_cast<double*>(part[j] + (size_t)jobs[j].B * haf_nparts(jobs[j].N / 2));
+    rj.part[j] = part[j];
+    rj.part_abs[j] = part_abs[j];
+    rj.out[j] = jobs[j].out;
+    rj.aux[j] = jobs[j].aux;
+    rj.n[j] = jobs[j].N / 2;
+    rj.b_begin[j] = bsum;
+    bsum += jobs[j].B;
+  }
+  rj.njobs = njobs;
+  rj.b_begin[njobs] = bsum;
+  launch_phases<Ar, LOOP, MODE>(jobs, njobs, part, part_abs, st, std::make_integer_sequence<int, HAF_MAX_PAIRS>{});
+  hipLaunchKernelGGL((haf_reduce_kernel<Ar, MODE>), dim3((bsum + 255) / 256), dim3(256), 0, st, rj);
   return (int)hipGetLastError();
 }
 
 // explicit instantiations live in haf_<tier>_<kind>.hip, one translation unit
 // each so that the (heavily unrolled) kernels compile in parallel
-extern template int run_haf<ArF64, false, HM_PLAIN>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
-extern template int run_haf<ArF64, true, HM_PLAIN>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
-extern template int run_haf<ArF64, false, HM_KAPPA>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
-extern template int run_haf<ArF64, true, HM_KAPPA>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
-extern template int run_haf<ArF64, false, HM_CERT>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
-extern template int run_haf<ArF64, true, HM_CERT>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
-extern template int run_haf<ArDD, false, HM_PLAIN>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
-extern template int run_haf<ArDD, true, HM_PLAIN>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
-extern template int run_haf<ArDD, false, HM_DDCERT>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
-extern template int run_haf<ArDD, true, HM_DDCERT>(int, const cplx*, int, cplx*, double*, void*, hipStream_t);
+extern template int run_haf_multi<ArF64, false, HM_PLAIN>(const HafJob*, int, hipStream_t);
+extern template int run_haf_multi<ArF64, true, HM_PLAIN>(const HafJob*, int, hipStream_t);
+extern template int run_haf_multi<ArF64, false, HM_KAPPA>(const HafJob*, int, hipStream_t);
+extern template int run_haf_multi<ArF64, true, HM_KAPPA>(const HafJob*, int, hipStream_t);
+extern template int run_haf_multi<ArF64, false, HM_CERT>(const HafJob*, int, hipStream_t);
+extern template int run_haf_multi<ArF64, true, HM_CERT>(const HafJob*, int, hipStream_t);
+extern template int run_haf_multi<ArDD, false, HM_PLAIN>(const HafJob*, int, hipStream_t);
+extern template int run_haf_multi<ArDD, true, HM_PLAIN>(const HafJob*, int, hipStream_t);
+extern template int run_haf_multi<ArDD, false, HM_DDCERT>(const HafJob*, int, hipStream_t);
+extern template int run_haf_multi<ArDD, true, HM_DDCERT>(const HafJob*, int, hipStream_t);
 
 }  // namespace gbsk

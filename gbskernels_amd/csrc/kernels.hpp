@@ -15,6 +15,19 @@ size_t haf_workspace_bytes(bool loop, int tier, int N, int B);
 int haf_launch(bool loop, int tier, int N, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws,
                hipStream_t st);
 int haf_max_dim(bool loop, int tier);
+// Several uniform hafnian buckets (same function and tier, different sizes) in
+// one launch per subset size: each bucket keeps its own kernels' arithmetic and
+// work split (values identical to haf_launch), the launches just carry more
+// workgroups. ws: haf_workspace_bytes(loop, tier, N, B) bytes per job.
+struct HafJob {
+  const cplx* in;
+  cplx* out;
+  double* aux;
+  void* ws;
+  int N, B;
+};
+constexpr int HAF_MAX_JOBS = 16;
+int haf_launch_multi(bool loop, int tier, const HafJob* jobs, int njobs, hipStream_t st);
 
 // torontonian; dim = matrix side 2n
 size_t tor_workspace_bytes(int tier, int dim, int B);
