@@ -245,11 +245,11 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const HafPhaseJobs<typena
   constexpr int G = 64 / S;
   extern __shared__ cplx As[];  // N * N, dynamic
   __shared__ T red[64];
-  __shared__ T wbuf[64];
   __shared__ T kgs[G][HAF_MAX_PAIRS + 1];  // k g_k of each group's current subset
   __shared__ T es[G][HAF_MAX_PAIRS + 1];   // Newton coefficients e_0..e_n
   __shared__ double rede[ANYC ? 64 : 1];
-  __shared__ double wbufe[ANYC ? 64 : 1];
+  __shared__ T kvs[LOOP ? G : 1][HAF_MAX_PAIRS + 2];  // k v_k (loop hafnian)
+  __shared__ double kve[LOOP && ANYC ? G : 1][HAF_MAX_PAIRS + 2];
   __shared__ double kge[ANYC ? G : 1][HAF_MAX_PAIRS + 1];
   __shared__ double ese[ANYC ? G : 1][HAF_MAX_PAIRS + 1];
   const int lane = threadIdx.x;
@@ -304,6 +304,71 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const HafPhaseJobs<typena
     if (ANYC) {
 #pragma unroll
       for (int c = 0; c < S; ++c) eP[c] = 0.0;  // C is exact
+    }
+    // Loop hafnian: k v_k = k/2 d^T X C^(k-1) d with d = diag B_S. With
+    // u_j = C^j d (lane r: row r of C^j times d, no lane exchange) and
+    // (C^a)^T = X C^a X, d^T X C^(a+b) d = sum_r u_a[r^1] u_b[r], so every v_k
+    // up to k - 1 = 2h comes from u_0 .. u_h with a = floor((k-1)/2),
+    // b = ceil((k-1)/2): v_{2j}, v_{2j+1} at power j. Stored apart (kvs) and
+    // added to the trace part p_k / 2 before the Newton recurrence.
+    auto dot_d = [&](const T (&row)[S], const double (&erow)[ANYC ? S : 1], T& u, double& eu) {
+      u = Ar::zero();
+      eu = CERT ? (double)S * CERT_UFL : 0.0;
+#pragma unroll
+      for (int c = 0; c < S; ++c) {
+        const int Rc = 2 * p[c >> 1] + (c & 1);
+        const cplx dc = As[Rc * N + Rc];
+        if constexpr (DDC) {
+          double o;
+          u = cddc_fma_c(row[c], dc, u, &o);
+          eu = fma(erow[c], cert_norm1(dc), eu + o);
+        } else {
+          if constexpr (CERT) eu = fma(fma(gam, cert_norm1(Ar::to_c(row[c])), erow[c]), cert_norm1(dc), eu);
+          u = Ar::fma_c(row[c], dc, u);
+        }
+      }
+    };
+    auto add_v = [&](int k, const T& ua, double eua, const T& ub, double eub) {
+      if (k > n) return;  // uniform over the wave (reductions hold barriers)
+      const T x = Ar::shfl_xor(ua, 1);
+      const double ex = ANYC ? shfl_xor_d(eua, 1) : 0.0;
+      const double hk = 0.5 * (double)k;
+      if constexpr (DDC) {
+        double o;
+        T v = cddc_fma(x, ub, Ar::zero(), &o);
+        double ev = cddc_mul_bound(x, ex, ub, eub, o);
+        ddcert_group_reduce<S>(v, ev, red, rede);
+        if (r == 0 && lane_ok) {
+          double os;
+          kvs[g][k] = cddc_scale(v, hk, &os);
+          kve[g][k] = fma(hk, ev, os);
+        }
+      } else if constexpr (CERT) {
+        T v = Ar::fma(x, ub, Ar::zero());
+        double ev = cert_fma(Ar::to_c(x), ex, Ar::to_c(ub), eub, cplx{0.0, 0.0}, 0.0, Ar::to_c(v));
+        cert_group_reduce<S>(v, ev, red, rede);
+        if (r == 0 && lane_ok) {
+          const T kv = Ar::scale(v, hk);
+          kvs[g][k] = kv;
+          kve[g][k] = cert_round(Ar::to_c(kv), hk * ev);
+        }
+      } else {
+        const T v = group_reduce<Ar, S>(Ar::fma(x, ub, Ar::zero()), red);
+        if (r == 0 && lane_ok) kvs[g][k] = Ar::scale(v, hk);
+      }
+    };
+    T uprev = Ar::zero();  // u_{j-1}
+    double euprev = 0.0;
+    if constexpr (LOOP) {
+      const T u0 = Ar::from(As[R * N + R]);
+      T u1;
+      double eu1;
+      dot_d(P, eP, u1, eu1);
+      add_v(1, u0, 0.0, u0, 0.0);
+      add_v(2, u0, 0.0, u1, eu1);
+      add_v(3, u1, eu1, u1, eu1);
+      uprev = u1;
+      euprev = eu1;
     }
     // p_1 = tr C (C[r][r] = A[R][R^1]) and p_2 = tr(C C) through the partner row
     {
@@ -386,6 +451,15 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const HafPhaseJobs<typena
           }
         }
       }
+      if constexpr (LOOP) {
+        T uj;
+        double euj;
+        dot_d(acc, eA, uj, euj);
+        add_v(2 * j, uprev, euprev, uj, euj);
+        add_v(2 * j + 1, uj, euj, uj, euj);
+        uprev = uj;
+        euprev = euj;
+      }
       T todd = Ar::zero(), teven = Ar::zero();
       double eo = 0.0, ee = 0.0;
 #pragma unroll
@@ -451,67 +525,20 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const HafPhaseJobs<typena
       for (int c = 0; c < S; ++c) P[c] = acc[c];
     }
     if constexpr (LOOP) {
-      // w_1 = d, w_{k+1} = C w_k; v_k = 1/2 sum_a d_(a^1) w_k[a]; k g_k += k v_k
-      const cplx dpartner = As[(R ^ 1) * N + (R ^ 1)];
-      T w = Ar::from(As[R * N + R]);
-      double ew = 0.0;
-      for (int k = 1; k <= n; ++k) {
-        if constexpr (DDC) {
-          double ev;
-          T vpart = cddc_fma_c(w, dpartner, Ar::zero(), &ev);
-          ev = fma(ew, cert_norm1(dpartner), ev);
-          ddcert_group_reduce<S>(vpart, ev, red, rede);
-          if (r == 0 && lane_ok) {
-            double os, oa;
-            const T kv = cddc_scale(vpart, 0.5 * (double)k, &os);
-            const T sum = cddc_add(kgs[g][k], kv, &oa);
-            kge[g][k] = kge[g][k] + fma(0.5 * (double)k, ev, os) + oa;
+      // kg_k = p_k / 2 + k v_k
+      if (r == 0 && lane_ok) {
+        for (int k = 1; k <= n; ++k) {
+          if constexpr (DDC) {
+            double o;
+            kgs[g][k] = cddc_add(kgs[g][k], kvs[g][k], &o);
+            kge[g][k] = kge[g][k] + kve[g][k] + o;
+          } else if constexpr (CERT) {
+            const T sum = Ar::add(kgs[g][k], kvs[g][k]);
+            kge[g][k] = cert_round(Ar::to_c(sum), kge[g][k] + kve[g][k]);
             kgs[g][k] = sum;
+          } else {
+            kgs[g][k] = Ar::add(kgs[g][k], kvs[g][k]);
           }
-        } else if constexpr (CERT) {
-          T vpart = Ar::fma_c(w, dpartner, Ar::zero());
-          double ev = cert_fma(Ar::to_c(w), ew, dpartner, 0.0, cplx{0.0, 0.0}, 0.0, vpart);
-          cert_group_reduce<S>(vpart, ev, red, rede);
-          if (r == 0 && lane_ok) {
-            // kg_k + fl(fl(v * 0.5 k)): two roundings
-            const T kv = Ar::scale(vpart, 0.5 * (double)k);
-            const T sum = Ar::add(kgs[g][k], kv);
-            const double ekv = cert_round(kv, fma(0.5 * (double)k, ev, 0.0));
-            kge[g][k] = cert_round(sum, kge[g][k] + ekv);
-            kgs[g][k] = sum;
-          }
-        } else {
-          const T vk = group_reduce<Ar, S>(Ar::fma_c(w, dpartner, Ar::zero()), red);
-          if (r == 0 && lane_ok) kgs[g][k] = Ar::add(kgs[g][k], Ar::scale(vk, 0.5 * (double)k));
-        }
-        if (k < n) {
-          wbuf[lane] = w;
-          if (ANYC) wbufe[lane] = ew;
-          __syncthreads();
-          T nw = Ar::zero();
-          double enw = CERT ? (double)S * CERT_UFL : 0.0;
-#pragma unroll
-          for (int u = 0; u < M; ++u) {
-            const T w1 = wbuf[gs * S + 2 * u], w0 = wbuf[gs * S + 2 * u + 1];
-            const cplx a1 = As[R * N + 2 * p[u] + 1], a0 = As[R * N + 2 * p[u]];
-            if constexpr (DDC) {
-              double o1, o0;
-              nw = cddc_fma_c(w1, a1, nw, &o1);
-              nw = cddc_fma_c(w0, a0, nw, &o0);
-              enw = fma(wbufe[gs * S + 2 * u], cert_norm1(a1), enw + o1);
-              enw = fma(wbufe[gs * S + 2 * u + 1], cert_norm1(a0), enw + o0);
-              continue;
-            }
-            nw = Ar::fma_c(w1, a1, nw);
-            nw = Ar::fma_c(w0, a0, nw);
-            if (CERT) {
-              enw = fma(fma(gam, cert_norm1(Ar::to_c(w1)), wbufe[gs * S + 2 * u]), cert_norm1(a1), enw);
-              enw = fma(fma(gam, cert_norm1(Ar::to_c(w0)), wbufe[gs * S + 2 * u + 1]), cert_norm1(a0), enw);
-            }
-          }
-          __syncthreads();
-          w = nw;
-          ew = enw;
         }
       }
     }
