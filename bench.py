@@ -40,7 +40,7 @@ PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
 # HBM bytes of one perm_kernel<24, fp64> launch (1024 x 24x24), measured with
 # hardware counters (FETCH_SIZE, WRITE_SIZE) calibrated against a plain copy
-TRAFFIC_N24 = 10583040
+TRAFFIC_N24 = 13742848
 
 
 def workload():
