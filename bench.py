@@ -280,9 +280,11 @@ def haf_kernels(L, torch, dev, stream):
 
 
 def tor_flops(nm: int) -> float:
-    """canonical recursive-Cholesky torontonian flops (2-row appends + leaves)"""
+    """canonical recursive-Cholesky torontonian flops: appending the two rows
+    (x, p) of the k-th included mode costs sum_{c <= r} (2c + 2) over rows
+    r = 2k - 2, 2k - 1, i.e. 8 k^2, plus 4 per subset and 3 per leaf"""
     from math import comb
-    return float(sum(comb(nm, k) * (8 * (2 * k) ** 2 + 4) for k in range(1, nm + 1)) + 3 * 2 ** nm)
+    return float(sum(comb(nm, k) * (8 * k * k + 4) for k in range(1, nm + 1)) + 3 * 2 ** nm)
 
 
 def tor_kernels(L, torch, dev, stream):

@@ -56,7 +56,7 @@ double canonical_flops(int func, int tier, int dim) {
     double c = 1.0;
     for (int k = 1; k <= nm; ++k) {
       c = c * (nm - k + 1) / k;
-      f += c * (8.0 * (2.0 * k) * (2.0 * k) + 4.0);
+      f += c * (8.0 * k * k + 4.0);  // two appended Cholesky rows + the leaf
     }
     f += 3.0 * std::ldexp(1.0, nm);
   }

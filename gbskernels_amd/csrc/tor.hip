@@ -35,6 +35,8 @@ struct RF64 {
   using T = double;
   static constexpr int RMAX = 5;  // per-lane trailing modes
   static constexpr bool CERT = false;
+  // real SPD: 1/sqrt(det) = product of the reciprocal Cholesky diagonals
+  static constexpr bool ISD = true;
   __device__ static T zero() { return 0.0; }
   __device__ static T one() { return 1.0; }
   __device__ static T from(double x) { return x; }
@@ -46,6 +48,7 @@ struct RF64 {
   __device__ static T sub(T a, T b) { return a - b; }
   __device__ static T mul(T a, T b) { return a * b; }
   __device__ static T div(T a, T b) { return a / b; }
+  __device__ static T recip(T a) { return 1.0 / a; }
   __device__ static T sqrt_(T a) { return sqrt(a); }
   __device__ static T fms(T a, T b, T c) { return fma(-a, b, c); }  // c - a b
   __device__ static bool pos(T a) { return a > 0.0; }
@@ -59,6 +62,7 @@ struct RDD {
   using T = dd;
   static constexpr int RMAX = 3;
   static constexpr bool CERT = false;
+  static constexpr bool ISD = true;
   __device__ static T zero() { return dd{0.0, 0.0}; }
   __device__ static T one() { return dd{1.0, 0.0}; }
   __device__ static T from(double x) { return dd{x, 0.0}; }
@@ -70,6 +74,7 @@ struct RDD {
   __device__ static T sub(T a, T b) { return dd_sub(a, b); }
   __device__ static T mul(T a, T b) { return dd_mul(a, b); }
   __device__ static T div(T a, T b) { return dd_div(a, b); }
+  __device__ static T recip(T a) { return dd_div(dd{1.0, 0.0}, a); }
   __device__ static T sqrt_(T a) { return dd_sqrt(a); }
   __device__ static T fms(T a, T b, T c) { return dd_sub(c, dd_mul(a, b)); }
   __device__ static bool pos(T a) { return a.hi > 0.0; }
@@ -109,6 +114,9 @@ struct RC64 {
   using T = cplx;
   static constexpr int RMAX = 3;
   static constexpr bool CERT = false;
+  // complex: the principal sqrt of the subset determinant (not the product of
+  // the pivots' principal roots, which can differ in sign)
+  static constexpr bool ISD = false;
   __device__ static T zero() { return cplx{0.0, 0.0}; }
   __device__ static T one() { return cplx{1.0, 0.0}; }
   __device__ static T from(double x) { return cplx{x, x}; }
@@ -120,6 +128,7 @@ struct RC64 {
   __device__ static T sub(T a, T b) { return c_sub(a, b); }
   __device__ static T mul(T a, T b) { return c_mul(a, b); }
   __device__ static T div(T a, T b) { return c_div(a, b); }
+  __device__ static T recip(T a) { return c_div(cplx{1.0, 0.0}, a); }
   __device__ static T sqrt_(T a) { return c_sqrt(a); }
   __device__ static T fms(T a, T b, T c) { return c_sub(c, c_mul(a, b)); }
   __device__ static bool pos(T a) { return a.re != 0.0 || a.im != 0.0; }
@@ -152,6 +161,7 @@ struct RCERT {
   using T = cv;
   static constexpr int RMAX = RF64::RMAX;  // the fp64 tier's split, for identical values
   static constexpr bool CERT = true;
+  static constexpr bool ISD = true;
   __device__ static T zero() { return cv{0.0, 0.0}; }
   __device__ static T one() { return cv{1.0, 0.0}; }
   __device__ static T one_minus(cplx o) {
@@ -190,6 +200,7 @@ struct RCERT {
     const double t = fma((1.0 + 2.0 * CERT_U) * fabs(v), b.e, a.e);
     return cv{v, fma(CERT_U, fabs(v), fma(2.0, t / bb, CERT_UFL))};
   }
+  __device__ static T recip(T a) { return div(one(), a); }
   __device__ static T sqrt_(T a) {
     const double v = sqrt(a.v);
     if (a.e == 0.0) return cv{v, fma(CERT_U, v, CERT_UFL)};
@@ -230,6 +241,7 @@ struct RCDD {
   using T = cdd;
   static constexpr int RMAX = RDD::RMAX;  // the dd tier's split, for identical values
   static constexpr bool CERT = true;
+  static constexpr bool ISD = true;
   __device__ static T zero() { return cdd{dd{0.0, 0.0}, 0.0}; }
   __device__ static T one() { return cdd{dd{1.0, 0.0}, 0.0}; }
   __device__ static T one_minus(cplx o) { return cdd{two_sum(1.0, -o.re), 0.0}; }  // exact
@@ -264,6 +276,7 @@ struct RCDD {
     const double prop = 2.0 * fma(cdd_mag(q) + eq, b.e, a.e) / lb;
     return cdd{q, eq + prop};
   }
+  __device__ static T recip(T a) { return div(one(), a); }
   __device__ static T sqrt_(T a) {
     const dd y = dd_sqrt(a.v);
     if (a.v.hi == 0.0 && a.v.lo == 0.0 && a.e == 0.0) return cdd{y, 0.0};
@@ -311,6 +324,11 @@ __device__ constexpr int tri(int r, int c) { return r * (r + 1) / 2 + c; }
 // ----------------------------------------------------------- per-lane bottom
 // Prefix-Cholesky walk over the R trailing modes of one task. M is the task's
 // 2R x 2R Schur complement (packed lower, mode-major), L the factor prefix.
+// Real policies (ISD) keep the reciprocal of each diagonal entry in L[r][r]
+// (columns are scaled by it, no division per entry) and carry det as the
+// product of those reciprocals times 1/sqrt(det) of the task's upstream
+// pivots, so a leaf's term is that product (no square root per leaf). The
+// complex policy keeps the diagonal values and the determinant.
 template <class Ar, int R, int D, unsigned MASK, int C>
 struct Bottom {
   using T = typename Ar::T;
@@ -338,23 +356,23 @@ struct Bottom {
 #pragma unroll
         for (int u = 0; u < c; ++u) s = Ar::fms(L[r][u], L[c][u], s);
         if (c < r) {
-          L[r][c] = Ar::div(s, L[c][c]);
+          L[r][c] = Ar::ISD ? Ar::mul(s, L[c][c]) : Ar::div(s, L[c][c]);
         } else {
           bad |= !Ar::pos(s);
-          L[r][r] = Ar::sqrt_(s);
+          L[r][r] = Ar::ISD ? Ar::recip(Ar::sqrt_(s)) : Ar::sqrt_(s);
         }
       }
     }
     const T q = Ar::mul(L[2 * C][2 * C], L[2 * C + 1][2 * C + 1]);
-    Bottom<Ar, R, D + 1, MASK | (1u << D), C + 1>::run(M, L, Ar::mul(det, Ar::mul(q, q)), count + 1, nm, bad, acc,
-                                                       aacc);
+    const T child = Ar::ISD ? Ar::mul(det, q) : Ar::mul(det, Ar::mul(q, q));
+    Bottom<Ar, R, D + 1, MASK | (1u << D), C + 1>::run(M, L, child, count + 1, nm, bad, acc, aacc);
   }
 };
 
-// leaf: (-1)^(nm - count) / sqrt(det)
+// leaf: (-1)^(nm - count) / sqrt(det); `det` is already 1/sqrt(det) for ISD
 template <class Ar>
 __device__ __forceinline__ void tor_leaf(typename Ar::T det, int count, int nm, typename Ar::T& acc, double& aacc) {
-  const typename Ar::T t = Ar::div(Ar::one(), Ar::sqrt_(det));
+  const typename Ar::T t = Ar::ISD ? det : Ar::recip(Ar::sqrt_(det));
   acc = ((nm - count) & 1) ? Ar::sub(acc, t) : Ar::add(acc, t);
   aacc += Ar::mag(t);
 }
@@ -372,10 +390,12 @@ template <class Ar, int R>
 __device__ inline void bottom_walk(const typename Ar::T* M, typename Ar::T det, int count, int nm, bool& bad,
                                    typename Ar::T& acc, double& aacc) {
   typename Ar::T L[2 * R > 0 ? 2 * R : 1][2 * R > 0 ? 2 * R : 1];
+  // ISD: the walk carries 1/sqrt of the determinant (upstream pivots here)
+  const typename Ar::T d0 = Ar::ISD ? Ar::recip(Ar::sqrt_(det)) : det;
   if constexpr (R == 0) {
-    tor_leaf<Ar>(det, count, nm, acc, aacc);
+    tor_leaf<Ar>(d0, count, nm, acc, aacc);
   } else {
-    Bottom<Ar, R, 0, 0u, 0>::run(M, L, det, count, nm, bad, acc, aacc);
+    Bottom<Ar, R, 0, 0u, 0>::run(M, L, d0, count, nm, bad, acc, aacc);
   }
 }
 
@@ -385,10 +405,11 @@ __device__ inline void bottom_walk(const typename Ar::T* M, typename Ar::T det, 
 template <class Ar>
 __device__ inline typename Ar::T eliminate(typename Ar::T* X, int D, int p) {
   const typename Ar::T piv = X[p * D + p];
+  const typename Ar::T rp = Ar::recip(piv);
   __syncthreads();
   for (int e = threadIdx.x; e < D * D; e += 64) {
     const int i = e / D, j = e % D;
-    if (i != p && j != p) X[e] = Ar::fms(Ar::div(X[i * D + p], piv), X[p * D + j], X[e]);
+    if (i != p && j != p) X[e] = Ar::fms(Ar::mul(X[i * D + p], rp), X[p * D + j], X[e]);
   }
   __syncthreads();
   return piv;
@@ -410,15 +431,15 @@ __device__ inline void drop_mode(const T* X, int Dx, int md, T* Y) {
 // workgroup = (instance b, leading pattern p1); out partials [B][2^T1] (T)
 // RERUN: only workgroups whose partial is NaN (a real-path breakdown or a
 // complex O) run, overwriting it (the complex fallback of the fp64 tier).
-template <class Ar, int R, bool KAPPA, bool RERUN>
+template <class Ar, int R, int T2, bool KAPPA, bool RERUN>
 __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ mats, int nm, int B,
                                                       typename Ar::P* __restrict__ part,
                                                       double* __restrict__ part_abs) {
   using T = typename Ar::T;
   extern __shared__ double lds_raw[];
   T* lds = reinterpret_cast<T*>(lds_raw);
-  const TorSplit sp = tor_split(nm, Ar::RMAX);
-  const int T1 = sp.T1, T2 = sp.T2;
+  const TorSplit sp = tor_split(nm, Ar::RMAX);  // sp.R == R, sp.T2 == T2 (launch)
+  const int T1 = sp.T1;
   const int dim = 2 * nm;
   const int lane = threadIdx.x;
   const int64_t wg = blockIdx.x;
@@ -431,18 +452,14 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
   // LDS layout: level stack (T2+1 levels), then one region used first by the
   // work matrix (dim^2, dead once level 0 is extracted) and then by the task slots
   T* lev = lds;
-  int levoff[TOR_T2 + 2];
-  {
-    int o = 0;
-    for (int d = 0; d <= T2; ++d) {
-      levoff[d] = o;
-      const int Dd = 2 * (nm - T1 - d);
-      o += Dd * Dd;
-    }
-    levoff[T2 + 1] = o;
+  int levtot = 0;
+#pragma unroll
+  for (int d = 0; d <= T2; ++d) {
+    const int Dd = 2 * (nm - T1 - d);
+    levtot += Dd * Dd;
   }
   constexpr int SLOT = (2 * R) * (2 * R + 1) / 2;
-  T* W = lev + levoff[T2 + 1];
+  T* W = lev + levtot;
   T* slots = W;
   T* slot_det = slots + 64 * SLOT;
   int* slot_cnt = reinterpret_cast<int*>(slot_det + 64);
@@ -479,25 +496,29 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
     const int D0 = 2 * (nm - T1);
     for (int e = lane; e < D0 * D0; e += 64) {
       const int i = e / D0, j = e % D0;
-      lev[levoff[0] + e] = W[(2 * T1 + i) * dim + 2 * T1 + j];
+      lev[e] = W[(2 * T1 + i) * dim + 2 * T1 + j];
     }
     __syncthreads();
   }
 
-  // cooperative walk over the T2 middle modes
-  T dets[TOR_T2 + 1];
-  int cnts[TOR_T2 + 1];
-  int state[TOR_T2 + 1];
-  dets[0] = det;
-  cnts[0] = count;
+  // cooperative walk over the T2 middle modes' include / exclude tree,
+  // exclude first, one Schur complement per level (level d's Dd x Dd matrix,
+  // the next level right after it). The walk's state is wave-uniform and lives
+  // in LDS (every lane writes the same value), not in per-lane scratch. Leaf k
+  // in walk order becomes task k.
+  __shared__ T wdet[TOR_T2 + 1];
+  __shared__ int wcnt[TOR_T2 + 1], wstate[TOR_T2 + 1], woff[TOR_T2 + 1];
+  wdet[0] = det;
+  wcnt[0] = count;
+  wstate[0] = 0;
+  woff[0] = 0;
   int ntask = 0;
   int d = 0;
-  state[0] = 0;
   for (;;) {
+    const int Dd = 2 * (nm - T1 - d);
+    T* X = lev + woff[d];
     if (d == T2) {
       // leaf: the task is this level's matrix (the trailing R modes)
-      const int Dd = 2 * R;
-      const T* X = lev + levoff[d];
       for (int e = lane; e < SLOT; e += 64) {
         int r = 0;
         while (tri(r + 1, 0) <= e) ++r;
@@ -505,51 +526,56 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
         slots[ntask * SLOT + e] = X[r * Dd + c];
       }
       if (lane == 0) {
-        slot_det[ntask] = dets[d];
-        slot_cnt[ntask] = cnts[d];
+        slot_det[ntask] = wdet[d];
+        slot_cnt[ntask] = wcnt[d];
       }
       ++ntask;
       if (d == 0) break;
       --d;
       continue;
     }
-    const int Dd = 2 * (nm - T1 - d);
-    T* X = lev + levoff[d];
-    T* Y = lev + levoff[d + 1];
-    if (state[d] == 0) {  // exclude middle mode d: drop its rows
-      state[d] = 1;
+    T* Y = X + Dd * Dd;
+    const int st = wstate[d];
+    if (st == 0) {  // exclude middle mode d: drop its rows
+      wstate[d] = 1;
       drop_mode(X, Dd, 0, Y);
-      dets[d + 1] = dets[d];
-      cnts[d + 1] = cnts[d];
-      state[d + 1] = 0;
-      ++d;
-      continue;
-    }
-    if (state[d] == 1) {  // include: two pivots (x then p), Schur complement into Y
-      state[d] = 2;
+      wdet[d + 1] = wdet[d];
+      wcnt[d + 1] = wcnt[d];
+    } else if (st == 1) {
+      // include: two pivots (x then p), Schur complement into Y. With the
+      // 2 x 2 pivot block [[a, b], [b, c']], c = c' - b^2 / a, l_i0 = X_i0 / a
+      // and l_i1 = (X_i1 - l_i0 b) / c, the complement is
+      // Y_ij = X_ij - l_i0 X_0j - l_i1 (X_1j - b X_0j / a): the reciprocals of
+      // the two pivots, no division per entry
+      wstate[d] = 2;
       const T a = X[0];
       const T b01 = X[1];
-      const T c = Ar::sub(X[Dd + 1], Ar::div(Ar::mul(b01, b01), a));
+      const T ra = Ar::recip(a);
+      const T c = Ar::sub(X[Dd + 1], Ar::mul(Ar::mul(b01, b01), ra));
+      const T rc = Ar::recip(c);
       const int Dy = Dd - 2;
       for (int e = lane; e < Dy * Dy; e += 64) {
         const int i = e / Dy + 2, j = e % Dy + 2;
-        const T xi0 = X[i * Dd], x0j = X[j];
-        T v = Ar::fms(Ar::div(xi0, a), x0j, X[i * Dd + j]);
-        const T xi1 = Ar::fms(Ar::div(xi0, a), b01, X[i * Dd + 1]);
-        const T x1j = Ar::fms(Ar::div(b01, a), x0j, X[Dd + j]);
-        v = Ar::fms(Ar::div(xi1, c), x1j, v);
+        const T li0 = Ar::mul(X[i * Dd], ra);
+        const T li1 = Ar::mul(Ar::fms(li0, b01, X[i * Dd + 1]), rc);
+        const T x0j = X[j];
+        const T x1j = Ar::fms(Ar::mul(x0j, ra), b01, X[Dd + j]);
+        T v = Ar::fms(li0, x0j, X[i * Dd + j]);
+        v = Ar::fms(li1, x1j, v);
         Y[e] = v;
       }
       __syncthreads();
       bad |= !Ar::pos(a) || !Ar::pos(c);
-      dets[d + 1] = Ar::mul(dets[d], Ar::mul(a, c));
-      cnts[d + 1] = cnts[d] + 1;
-      state[d + 1] = 0;
-      ++d;
+      wdet[d + 1] = Ar::mul(wdet[d], Ar::mul(a, c));
+      wcnt[d + 1] = wcnt[d] + 1;
+    } else {
+      if (d == 0) break;
+      --d;
       continue;
     }
-    if (d == 0) break;
-    --d;
+    wstate[d + 1] = 0;
+    woff[d + 1] = woff[d] + Dd * Dd;
+    ++d;
   }
   __syncthreads();
 
@@ -687,26 +713,44 @@ static size_t tor_lds_bytes(int nm) {
   return lev * T + (work > task_area ? work : task_area);
 }
 
+template <class Ar, int R, int T2, bool KAPPA, bool RERUN>
+static void launch_tor_rt(int nm, const cplx* d_in, int B, typename Ar::P* part, double* part_abs, dim3 grid,
+                          size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((tor_real_kernel<Ar, R, T2, KAPPA, RERUN>), grid, dim3(64), lds, st, d_in, nm, B, part, part_abs);
+}
+
+// the split's R and T2 are template arguments: T2 = 6 with R = 0..RMAX
+// (nm >= 6), or T2 = nm < 6 with R = 0
 template <class Ar, bool KAPPA, bool RERUN>
 static int launch_tor(int nm, const cplx* d_in, int B, typename Ar::P* part, double* part_abs, hipStream_t st) {
   const TorSplit sp = tor_split(nm, Ar::RMAX);
   const int np = 1 << sp.T1;
   const size_t lds = tor_lds_bytes<Ar>(nm);
   const dim3 grid((unsigned)((size_t)B * np));
-  switch (sp.R) {
-#define TOR_CASE(RR)                                                                                        \
-  case RR:                                                                                                  \
-    if constexpr (RR <= Ar::RMAX)                                                                           \
-      hipLaunchKernelGGL((tor_real_kernel<Ar, RR, KAPPA, RERUN>), grid, dim3(64), lds, st, d_in, nm, B, part, \
-                         part_abs);                                                                         \
+  if (sp.T2 == TOR_T2) {
+    switch (sp.R) {
+#define TOR_CASE(RR)                                                                                             \
+  case RR:                                                                                                       \
+    if constexpr (RR <= Ar::RMAX) launch_tor_rt<Ar, RR, TOR_T2, KAPPA, RERUN>(nm, d_in, B, part, part_abs, grid, lds, st); \
     break;
-    TOR_CASE(0)
-    TOR_CASE(1)
-    TOR_CASE(2)
-    TOR_CASE(3)
-    TOR_CASE(4)
-    TOR_CASE(5)
+      TOR_CASE(0)
+      TOR_CASE(1)
+      TOR_CASE(2)
+      TOR_CASE(3)
+      TOR_CASE(4)
+      TOR_CASE(5)
 #undef TOR_CASE
+      default: return GBSK_ERR_DIM;
+    }
+    return 0;
+  }
+  if (sp.R != 0) return GBSK_ERR_DIM;
+  switch (sp.T2) {
+    case 1: launch_tor_rt<Ar, 0, 1, KAPPA, RERUN>(nm, d_in, B, part, part_abs, grid, lds, st); break;
+    case 2: launch_tor_rt<Ar, 0, 2, KAPPA, RERUN>(nm, d_in, B, part, part_abs, grid, lds, st); break;
+    case 3: launch_tor_rt<Ar, 0, 3, KAPPA, RERUN>(nm, d_in, B, part, part_abs, grid, lds, st); break;
+    case 4: launch_tor_rt<Ar, 0, 4, KAPPA, RERUN>(nm, d_in, B, part, part_abs, grid, lds, st); break;
+    case 5: launch_tor_rt<Ar, 0, 5, KAPPA, RERUN>(nm, d_in, B, part, part_abs, grid, lds, st); break;
     default: return GBSK_ERR_DIM;
   }
   return 0;
