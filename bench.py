@@ -84,6 +84,10 @@ def main() -> int:
     ap.add_argument("--served-requests", type=int, default=40000,
                     help="requests of the served mixed-size workload (0 skips)")
     ap.add_argument("--served-batch", type=int, default=2048, help="requests per batch / window")
+    ap.add_argument("--config3-requests", type=int, default=100000,
+                    help="ragged haf/lhaf requests through the server (0 skips)")
+    ap.add_argument("--config5-requests", type=int, default=40000,
+                    help="sampler-stream requests (Poisson 50k req/s) through the server (0 skips)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (a known-size kernel that "
                          "calibrates hardware byte counters) and exit")
@@ -215,6 +219,9 @@ def main() -> int:
     served = None
     if args.served_requests > 0:
         served = side("served mix", lambda: served_bench(args))
+    streams = None
+    if args.config3_requests > 0 or args.config5_requests > 0:
+        streams = side("config streams", lambda: config_streams(args))
 
     cpu = None
     if not args.no_cpu:
@@ -234,6 +241,7 @@ def main() -> int:
         "per_kernel": per_kernel,
         "parity": parity,
         "served": served,
+        "streams": streams,
     }
     print(json.dumps(line))
     return 0
@@ -434,6 +442,55 @@ def served_bench(args):
         "identical_values": same(sat_vals, plain) and same(poi_vals, plain),
         "size_histogram": {int(k): int(v) for k, v in enumerate(sizes) if v},
     }
+
+
+def config_streams(args):
+    """The ragged-hafnian and served-sampler configurations through the server:
+
+    * ragged haf + lhaf: haf / lhaf uniform, N = 2k with k ~ Geom(0.15) on 1..12,
+      submatrices of U diag(tanh r) U^T (U Haar 32x32, r ~ U[0.5, 1.2]), lhaf
+      diagonal CN(0, 0.3^2), tier auto (fp64, dd when kappa >= 1e8), all offered
+      at once (throughput);
+    * served sampler stream: perm / haf / lhaf / tor uniform, sizes geometric(0.15)
+      on 2..28 (tor: mode count on 1..14), tiers 70% fp64 / 25% dd / 5% certified,
+      Poisson arrivals at 50k req/s (latency)."""
+    from gbskernels_amd import _lib
+    from gbskernels_amd.serving import Server, pack
+    from gbskernels_amd.workloads import served_mix
+
+    def run(items, arr, K):
+        f, t, d, flat, off = pack(items)
+        with Server(slots=3, streams_per_slot=4, max_batch=K) as s:
+            s.replay_packed(f[:min(len(f), K)], t[:min(len(f), K)], d[:min(len(f), K)], flat,
+                            off[:min(len(f), K)], np.zeros(min(len(f), K)))  # warm the slots
+        with Server(slots=3, streams_per_slot=4, max_batch=K) as s:
+            vals, aux, infos = s.replay_packed(f, t, d, flat, off, arr)
+            st = s.stats()
+        t_arr = np.array([i.arrival_ns for i in infos], np.int64)
+        t_done = np.array([i.done_ns for i in infos], np.int64)
+        t0 = t_arr[0] - int(round(arr[0] * 1e9))
+        lat = (t_done - t_arr) * 1e-9
+        fin = np.isfinite(aux[t == _lib.TIERS["certified"]]) if np.any(t == _lib.TIERS["certified"]) else None
+        out = {"requests": len(items), "evals_per_s": len(items) / ((t_done.max() - t0) * 1e-9),
+               "p50_ms": float(np.percentile(lat, 50)) * 1e3, "p99_ms": float(np.percentile(lat, 99)) * 1e3,
+               "batches": st["batches"], "escalated": st["escalated"]}
+        if fin is not None:
+            out["certified_finite"] = float(np.mean(fin))
+        return out
+
+    res = {}
+    n3 = args.config3_requests
+    if n3 > 0:
+        print(f"bench: ragged haf/lhaf {n3} requests", file=sys.stderr, flush=True)
+        items, _ = served_mix(n3, seed=3, funcs=("haf", "lhaf"), nmax=24)
+        res["ragged_haf_lhaf_auto"] = run(items, np.zeros(n3), 4096)
+    n5 = args.config5_requests
+    if n5 > 0:
+        print(f"bench: sampler stream {n5} requests", file=sys.stderr, flush=True)
+        items, arr = served_mix(n5, seed=5, funcs=("perm", "haf", "lhaf", "tor"), nmax=28,
+                                tiers=(("fp64", 0.70), ("dd", 0.25), ("certified", 0.05)))
+        res["sampler_stream_poisson_50k"] = run(items, arr, 4096)
+    return res
 
 
 def cpu_baseline(O, m16, m24, budget_s):
