@@ -62,6 +62,11 @@ int haf_launch_multi(bool loop, int tier, const HafJob* jobs, int njobs, hipStre
   const bool aux = tier == GBSK_KAPPA || tier == GBSK_CERTIFIED || tier == GBSK_DD_CERTIFIED;
   HafJob work[HAF_MAX_JOBS];
   int nw = 0;
+  auto flush = [&]() {
+    const int rc = nw ? run_multi(loop, tier, work, nw, st) : 0;
+    nw = 0;
+    return rc;
+  };
   for (int j = 0; j < njobs; ++j) {
     const HafJob& J = jobs[j];
     if (J.B <= 0) continue;
@@ -73,12 +78,28 @@ int haf_launch_multi(bool loop, int tier, const HafJob* jobs, int njobs, hipStre
       continue;
     }
     if (!J.ws) return GBSK_ERR_WORKSPACE;
-    work[nw++] = J;
+    // a job's share of one launch stays below 2^26 workgroups (at most
+    // HAF_MAX_JOBS jobs per launch: below 2^30); larger buckets are split into
+    // consecutive instance ranges, each with its own slice of the workspace
+    int nch_max = 1;
+    for (int m = 1; m <= J.N / 2; ++m) {
+      const int c = haf_nchunks(J.N / 2, 2 * m);
+      nch_max = c > nch_max ? c : nch_max;
+    }
+    const int64_t max_b = ((int64_t)1 << 26) / nch_max;
+    const size_t per_inst = haf_workspace_bytes(loop, tier, J.N, 1);
+    for (int64_t b0 = 0; b0 < J.B; b0 += max_b) {
+      const int Bs = (int)((J.B - b0) < max_b ? (J.B - b0) : max_b);
+      if (nw == HAF_MAX_JOBS) {
+        const int rc = flush();
+        if (rc) return rc;
+      }
+      work[nw++] = HafJob{J.in + b0 * J.N * J.N, J.out + b0, aux ? J.aux + b0 : nullptr,
+                          (char*)J.ws + (size_t)b0 * per_inst, J.N, Bs};
+    }
   }
-  if (nw) {
-    const int rc = run_multi(loop, tier, work, nw, st);
-    if (rc) return rc;
-  }
+  const int rc = flush();
+  if (rc) return rc;
   return (int)hipGetLastError();
 }
 

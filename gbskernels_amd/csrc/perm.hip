@@ -438,19 +438,30 @@ template <int N, int MODE>
 static int launch_perm_n(const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
   using C = PermCfg<N, perm_is_dd(MODE) ? 1 : 0>;
   constexpr int MPW = 64 / C::LPM, W = C::W;
-  const int64_t groups = ((int64_t)B + MPW - 1) / MPW;
-  const int g8 = groups < 8 ? (int)groups : 8;
-  const int64_t grid = (W == 1) ? groups : ((groups + g8 - 1) / g8) * g8 * W;
-  if (W == 1) {
-    hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, d_in, B, d_out, d_aux, g8);
-  } else {
-    // partials: values [B][W] cplx, then [B][W] of 16 bytes (kappa / bound
-    // doubles, or the dd low parts), then (certified dd) [B][W] bounds
-    cplx* part = reinterpret_cast<cplx*>(ws);
-    double* part_aux = reinterpret_cast<double*>(part + (size_t)B * W);
-    hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, d_in, B, part, part_aux, g8);
-    hipLaunchKernelGGL((perm_reduce_kernel<N, MODE>), dim3((unsigned)((B + 255) / 256)), dim3(256), 0, st, part,
-                       part_aux, B, d_out, d_aux);
+  // at most 2^30 workgroups per launch: large batches run as consecutive
+  // sub-batches (same stream, the workspace reused), matrices of a sub-batch a
+  // multiple of 8 groups so the XCD mapping is unchanged
+  constexpr int64_t max_groups = (((int64_t)1 << 30) / W) / 8 * 8;
+  const int64_t max_b = max_groups * MPW;
+  for (int64_t b0 = 0; b0 < B; b0 += max_b) {
+    const int Bs = (int)((B - b0) < max_b ? (B - b0) : max_b);
+    const cplx* in = d_in + b0 * N * N;
+    cplx* out = d_out + b0;
+    double* aux = d_aux ? d_aux + b0 : nullptr;
+    const int64_t groups = ((int64_t)Bs + MPW - 1) / MPW;
+    const int g8 = groups < 8 ? (int)groups : 8;
+    const int64_t grid = (W == 1) ? groups : ((groups + g8 - 1) / g8) * g8 * W;
+    if (W == 1) {
+      hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, in, Bs, out, aux, g8);
+    } else {
+      // partials: values [Bs][W] cplx, then [Bs][W] of 16 bytes (kappa / bound
+      // doubles, or the dd low parts), then (certified dd) [Bs][W] bounds
+      cplx* part = reinterpret_cast<cplx*>(ws);
+      double* part_aux = reinterpret_cast<double*>(part + (size_t)Bs * W);
+      hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, in, Bs, part, part_aux, g8);
+      hipLaunchKernelGGL((perm_reduce_kernel<N, MODE>), dim3((unsigned)((Bs + 255) / 256)), dim3(256), 0, st, part,
+                         part_aux, Bs, out, aux);
+    }
   }
   return (int)hipGetLastError();
 }

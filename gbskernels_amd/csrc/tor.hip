@@ -858,11 +858,31 @@ int tor_launch(int tier, int dim, const cplx* d_in, int B, cplx* d_out, double* 
     return (int)hipGetLastError();
   }
   if (!ws) return GBSK_ERR_WORKSPACE;
-  if (tier == GBSK_DD) return run_tor_dd(nm, d_in, B, d_out, ws, st);
-  if (tier == GBSK_DD_CERTIFIED) return run_tor_ddcert(nm, d_in, B, d_out, d_aux, ws, st);
-  if (tier == GBSK_CERTIFIED) return run_tor_fp64<RCERT, false>(nm, d_in, B, d_out, d_aux, ws, st);
-  return tier == GBSK_KAPPA ? run_tor_fp64<RF64, true>(nm, d_in, B, d_out, d_aux, ws, st)
-                            : run_tor_fp64<RF64, false>(nm, d_in, B, d_out, nullptr, ws, st);
+  // at most 2^30 workgroups per launch: large batches run as consecutive
+  // sub-batches on the same stream, reusing the workspace
+  const TorSplit s64 = tor_split(nm, RF64::RMAX), sc = tor_split(nm, RC64::RMAX), sdd = tor_split(nm, RDD::RMAX);
+  int t1 = s64.T1 > sc.T1 ? s64.T1 : sc.T1;
+  t1 = sdd.T1 > t1 ? sdd.T1 : t1;
+  const int64_t max_b = ((int64_t)1 << 30) >> t1;
+  for (int64_t b0 = 0; b0 < B; b0 += max_b) {
+    const int Bs = (int)((B - b0) < max_b ? (B - b0) : max_b);
+    const cplx* in = d_in + b0 * dim * dim;
+    cplx* out = d_out + b0;
+    double* aux = d_aux ? d_aux + b0 : nullptr;
+    int rc;
+    if (tier == GBSK_DD)
+      rc = run_tor_dd(nm, in, Bs, out, ws, st);
+    else if (tier == GBSK_DD_CERTIFIED)
+      rc = run_tor_ddcert(nm, in, Bs, out, aux, ws, st);
+    else if (tier == GBSK_CERTIFIED)
+      rc = run_tor_fp64<RCERT, false>(nm, in, Bs, out, aux, ws, st);
+    else if (tier == GBSK_KAPPA)
+      rc = run_tor_fp64<RF64, true>(nm, in, Bs, out, aux, ws, st);
+    else
+      rc = run_tor_fp64<RF64, false>(nm, in, Bs, out, nullptr, ws, st);
+    if (rc) return rc;
+  }
+  return 0;
 }
 
 }  // namespace gbsk

@@ -206,3 +206,17 @@ def test_server_budget_refuses_oversized_request():
             s.wait(big)
         v, _, info = s.wait(small)
         assert info.status == 0 and v == alone("perm", "fp64", rng_complex((4, 4), 2))[0]
+
+
+def test_empty_batches_and_streams():
+    from gbskernels_amd.serving import Batch, Server
+    with Batch() as b:
+        v, a = b.run([])
+        assert v.shape == (0,) and a.shape == (0,)
+        st = b.stats()
+        v1, _ = b.run([("haf", "fp64", np.zeros((0, 0)))])   # haf of the empty matrix
+        assert v1[0] == 1
+    with Server(slots=1) as s:
+        v, a, infos = s.replay([], [])
+        assert v.shape == (0,) and infos == []
+    assert st["items"] == 0
