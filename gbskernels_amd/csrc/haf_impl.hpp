@@ -643,38 +643,65 @@ struct HafReduceJobs {
   int njobs;
 };
 
-// out[b] = sum over phases m = 1..n in order (fixed association)
+// out[b] = sum of the instance's np chunk partials, one wave per instance:
+// lane l sums slots l, l + 64, l + 128, ... in slot order, then a fixed
+// xor-butterfly over the 64 lanes; lane 0 writes. The association depends on np
+// only, so every mode (plain, kappa, certified) produces the same value bits.
+constexpr int HAF_REDUCE_WAVES = 4;
 template <class Ar, int MODE>
-__global__ void haf_reduce_kernel(const HafReduceJobs<typename Ar::T> jobs) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64 * HAF_REDUCE_WAVES) void haf_reduce_kernel(const HafReduceJobs<typename Ar::T> jobs) {
+  using T = typename Ar::T;
+  const int64_t t = (int64_t)blockIdx.x * HAF_REDUCE_WAVES + (threadIdx.x >> 6);  // wave-uniform
+  const int lane = threadIdx.x & 63;
   if (t >= jobs.b_begin[jobs.njobs]) return;
   int jb = 0;
   while (jb + 1 < jobs.njobs && t >= jobs.b_begin[jb + 1]) ++jb;
-  const typename Ar::T* __restrict__ part = jobs.part[jb];
-  const double* __restrict__ part_abs = jobs.part_abs[jb];
-  cplx* __restrict__ out = jobs.out[jb];
-  double* __restrict__ aux = jobs.aux[jb];
   const int n = jobs.n[jb];
-  const int64_t b = t - jobs.b_begin[jb];
   const int np = haf_nparts(n);
-  typename Ar::T s = part[b * np];
-  double a = MODE != HM_PLAIN ? part_abs[b * np] : 0.0;
-  for (int m = 1; m < np; ++m) {
+  const int64_t b = t - jobs.b_begin[jb];
+  const T* __restrict__ part = jobs.part[jb] + b * np;
+  const double* __restrict__ part_abs = jobs.part_abs[jb] + b * np;
+  T s = Ar::zero();
+  double a = 0.0;
+  if (lane < np) {
+    s = part[lane];
+    if (MODE != HM_PLAIN) a = part_abs[lane];
+  }
+  for (int m = lane + 64; m < np; m += 64) {
     if constexpr (MODE == HM_DDCERT) {
       double o;
-      s = cddc_add(s, part[b * np + m], &o);
-      a = a + part_abs[b * np + m] + o;
+      s = cddc_add(s, part[m], &o);
+      a = a + part_abs[m] + o;
     } else {
-      s = Ar::add(s, part[b * np + m]);
-      if (MODE == HM_KAPPA) a += part_abs[b * np + m];
-      if (MODE == HM_CERT) a = cert_round(Ar::to_c(s), a + part_abs[b * np + m]);
+      s = Ar::add(s, part[m]);
+      if (MODE == HM_KAPPA) a += part_abs[m];
+      if (MODE == HM_CERT) a = cert_round(Ar::to_c(s), a + part_abs[m]);
     }
   }
+#pragma unroll
+  for (int k = 1; k < 64; k <<= 1) {
+    const T o = Ar::shfl_xor(s, k);
+    const double ao = MODE != HM_PLAIN ? shfl_xor_d(a, k) : 0.0;
+    if constexpr (MODE == HM_DDCERT) {
+      double r;
+      s = cddc_add(s, o, &r);
+      a = a + ao + r;
+    } else {
+      s = Ar::add(s, o);
+      if (MODE == HM_KAPPA) a += ao;
+      if (MODE == HM_CERT) a = cert_round(Ar::to_c(s), a + ao);
+    }
+  }
+  if (lane != 0) return;
+  cplx* __restrict__ out = jobs.out[jb];
+  double* __restrict__ aux = jobs.aux[jb];
   out[b] = Ar::to_c(s);
   if (MODE == HM_KAPPA) aux[b] = a;
-  if (MODE == HM_CERT) aux[b] = cert_finish(a, 4.0 * np + 8.0);
+  // rounding depth of the bound arithmetic: at most np serial steps plus the six
+  // butterfly levels, each a few roundings deep
+  if (MODE == HM_CERT) aux[b] = cert_finish(a, 4.0 * (np + 6) + 8.0);
   if constexpr (MODE == HM_DDCERT) {
-    const double f = cert_finish(a + cddc_collapse_err(s), 8.0 * np + 16.0);
+    const double f = cert_finish(a + cddc_collapse_err(s), 8.0 * (np + 6) + 16.0);
     aux[b] = (f == f) ? f : INFINITY;
   }
 }
@@ -732,7 +759,8 @@ int run_haf_multi(const HafJob* jobs, int njobs, hipStream_t st) {
   rj.njobs = njobs;
   rj.b_begin[njobs] = bsum;
   launch_phases<Ar, LOOP, MODE>(jobs, njobs, part, part_abs, st, std::make_integer_sequence<int, HAF_MAX_PAIRS>{});
-  hipLaunchKernelGGL((haf_reduce_kernel<Ar, MODE>), dim3((bsum + 255) / 256), dim3(256), 0, st, rj);
+  hipLaunchKernelGGL((haf_reduce_kernel<Ar, MODE>), dim3((bsum + HAF_REDUCE_WAVES - 1) / HAF_REDUCE_WAVES),
+                     dim3(64 * HAF_REDUCE_WAVES), 0, st, rj);
   return (int)hipGetLastError();
 }
 
