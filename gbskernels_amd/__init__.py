@@ -5,7 +5,9 @@ Public surface (mirrors the reference ``gbskernels/__init__.py:54-63,517-558,734
 for ``backend="gpu"``): ``perm/haf/lhaf/tor`` and ``*_batched`` with
 ``precision in {"fp64", "dd", "auto", "certified", "ref"}``, ``return_diagnostics``
 and ``rtol``; :class:`Workspace` for ragged, device-resident batches;
-:func:`gpu_available` and :func:`gpu_backend_kind`.
+:func:`gpu_available` and :func:`gpu_backend_kind`; :func:`lhaf_repeated`, the
+repeated-row loop hafnian of the sampling workload (reference
+``gbskernels/__init__.py:561-632``).
 
 Tiers (semantics of the reference, reference ``gbskernels/__init__.py:1-37``):
 
@@ -35,6 +37,7 @@ __version__ = "0.1.0"
 __all__ = [
     "perm", "perm_batched", "haf", "haf_batched", "lhaf", "lhaf_batched",
     "tor", "tor_batched", "gpu_available", "gpu_backend_kind", "Workspace", "PRECISIONS",
+    "lhaf_repeated", "sieve_term_count",
 ]
 
 PRECISIONS = ("fp64", "ref", "dd", "auto", "certified")
@@ -288,6 +291,64 @@ def tor(O, precision: str = "fp64", backend: str = "gpu", return_diagnostics: bo
 def tor_batched(matrices, precision: str = "fp64", backend: str = "gpu", return_diagnostics: bool = False, rtol=None):
     """Torontonians of a uniform (B, 2n, 2n) stack."""
     return _dispatch_batched("tor", matrices, precision, backend, return_diagnostics, rtol)
+
+
+# --- repeated-row loop hafnian (reference gbskernels/__init__.py:561-632) --------
+
+def sieve_term_count(reps: Any) -> int:
+    """``prod(n_i + 1)``: the sieve's term count for one pattern (reference
+    ``cpu_ref/repeated.py:54-58``)."""
+    n = np.asarray(reps, dtype=np.int64)
+    return int(np.prod(n + 1)) if n.size else 1
+
+
+def lhaf_repeated(A: Any, gamma: Any, reps: Any, backend: str = "gpu", certified: bool = False):
+    """Loop hafnians of ``A`` expanded by repetition patterns.
+
+    ``A`` is the (M, M) base matrix, ``gamma`` the length-M loop weights
+    (``None``: zeros, the ordinary hafnian of the expansion) and ``reps`` a
+    (B, M) table of non-negative counts, or one length-M pattern. Row/column
+    ``i`` is repeated ``reps[b, i]`` times with ``gamma[i]`` on each copy's
+    diagonal; the finite-difference sieve costs ``prod(n_i + 1)`` terms per
+    pattern. Returns (B,) values (a scalar for one pattern); with
+    ``certified=True`` also ``{"tier": "certified-fp64", "abs_error_bound": ...}``
+    with ``|value - exact| <= bound`` and values bit-identical to the plain
+    sieve. Patterns beyond the kernel's domain (M > 32, N > 64, more than 2^40
+    terms) come back NaN; certified ones then raise ``FloatingPointError``."""
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    if A.ndim != 2 or A.shape[0] != A.shape[1]:
+        raise ValueError(f"lhaf_repeated: A must be square (M, M), got {A.shape}")
+    M = A.shape[0]
+    g = np.zeros(M, dtype=np.complex128) if gamma is None else np.ascontiguousarray(gamma, dtype=np.complex128)
+    if g.shape != (M,):
+        raise ValueError(f"lhaf_repeated: gamma must be a length-{M} vector, got {g.shape}")
+    if certified:
+        _check_certified_input(A)
+        _check_certified_input(g)
+    r = np.ascontiguousarray(reps, dtype=np.int32)
+    single = r.ndim == 1
+    if single:
+        r = r[None, :]
+    if r.ndim != 2 or r.shape[1] != M:
+        raise ValueError(f"lhaf_repeated: reps must be (B, {M}) or ({M},), got {np.asarray(reps).shape}")
+    if (r < 0).any():
+        raise ValueError("lhaf_repeated: reps must be non-negative")
+    if backend not in ("cpu", "gpu"):
+        raise ValueError(f"unknown backend {backend!r}; expected 'cpu' or 'gpu'")
+    if backend == "cpu":
+        raise NotImplementedError(
+            "gbskernels_amd is the GPU backend: the CPU sieve is the reference's cpu_ref.repeated")
+    ext = _ext()
+    if certified:
+        v, bnd = ext.lhaf_repeated_certified(A, g, r)
+        valid = bool(np.all(np.isfinite(v.real)) and np.all(np.isfinite(v.imag))
+                     and np.all(np.isfinite(bnd)) and np.all(bnd >= 0.0))
+        if not valid:
+            raise FloatingPointError("certified lhaf_repeated evaluator refused with a non-finite value or invalid bound")
+        diag = {"tier": "certified-fp64", "abs_error_bound": float(bnd[0]) if single else bnd}
+        return (complex(v[0]) if single else v), diag
+    v = ext.lhaf_repeated(A, g, r)
+    return complex(v[0]) if single else v
 
 
 # --- Workspace (reference gbskernels/__init__.py:716-908) ----------------------

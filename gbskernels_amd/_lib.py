@@ -78,6 +78,8 @@ SIGNATURES = {
     "gbs_server_replay": ([_cp, _i, _cp, _cp, _cp, _cp, _cp, _cp, _cp, _cp, _cp], _i),
     "gbs_server_get_stats": ([_cp, ctypes.POINTER(ServerStats)], _i),
     "gbs_server_destroy": ([_cp], _i),
+    "gbs_lhaf_repeated_host": ([_cp, _cp, _i, _cp, _i, _cp], _i),
+    "gbs_lhaf_repeated_cert_host": ([_cp, _cp, _i, _cp, _i, _cp, _cp], _i),
 }
 
 _lib = None

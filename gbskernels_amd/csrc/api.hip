@@ -140,6 +140,43 @@ static int host_call(int func, int tier, const gbs_complex* in, int kdim, int B,
   return run_host(c, func, tier, in, dim, B, out, aux, false);
 }
 
+// repeated-row sieve, one shot: A, gamma, reps and the plan go up in the input
+// buffer (256-byte aligned pieces), chunk partials live in the workspace
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+static int sieve_host(bool cert, const gbs_complex* hA, const gbs_complex* hg, int M, const int* hreps, int B,
+                      gbs_complex* hout, double* hbound) {
+  if (B <= 0) return 0;
+  if (M < 0 || !hout || (cert && !hbound) || (M > 0 && (!hA || !hg || !hreps))) return GBSK_ERR_ARG;
+  std::vector<RepPat> plan;
+  int64_t parts = 0;
+  int rc = sieve_plan(M, hreps, B, plan, &parts);
+  if (rc) return rc;
+  const size_t a_b = (size_t)M * M * sizeof(cplx), g_b = (size_t)M * sizeof(cplx);
+  const size_t r_b = (size_t)B * M * sizeof(int), p_b = (size_t)B * sizeof(RepPat);
+  const size_t o_a = 0, o_g = align256(a_b), o_r = o_g + align256(g_b), o_p = o_r + align256(r_b);
+  const size_t in_b = o_p + align256(p_b), ws_b = sieve_partial_bytes(cert, parts);
+  Ctx& c = global_ctx();
+  std::lock_guard<std::mutex> g(c.mu);
+  rc = c.init();
+  if (!rc) rc = c.in.grow(in_b);
+  if (!rc) rc = c.out.grow((size_t)B * sizeof(cplx));
+  if (!rc) rc = c.aux.grow((size_t)B * sizeof(double));
+  if (!rc) rc = c.ws.grow(ws_b > 0 ? ws_b : 16);
+  char* din = (char*)c.in.p;
+  if (!rc && a_b) rc = (int)hipMemcpyAsync(din + o_a, hA, a_b, hipMemcpyHostToDevice, c.stream);
+  if (!rc && g_b) rc = (int)hipMemcpyAsync(din + o_g, hg, g_b, hipMemcpyHostToDevice, c.stream);
+  if (!rc && r_b) rc = (int)hipMemcpyAsync(din + o_r, hreps, r_b, hipMemcpyHostToDevice, c.stream);
+  if (!rc) rc = (int)hipMemcpyAsync(din + o_p, plan.data(), p_b, hipMemcpyHostToDevice, c.stream);
+  if (!rc)
+    rc = sieve_launch(cert, (const cplx*)(din + o_a), (const cplx*)(din + o_g), M, (const int*)(din + o_r), B, plan,
+                      (const RepPat*)(din + o_p), parts, c.ws.p, (cplx*)c.out.p, (double*)c.aux.p, c.stream);
+  if (!rc) rc = (int)hipMemcpyAsync(hout, c.out.p, (size_t)B * sizeof(cplx), hipMemcpyDeviceToHost, c.stream);
+  if (!rc && cert)
+    rc = (int)hipMemcpyAsync(hbound, c.aux.p, (size_t)B * sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  const int rs = (int)hipStreamSynchronize(c.stream);  // also keeps `plan` alive for its copy
+  return rc ? rc : rs;
+}
+
 }  // namespace gbsk
 
 using namespace gbsk;
@@ -290,5 +327,14 @@ const char* gbs_error_string(int rc) {
 }
 
 int gbs_is_device_build(void) { return 1; }
+
+int gbs_lhaf_repeated_host(const gbs_complex* A, const gbs_complex* gamma, int M, const int* reps, int B,
+                           gbs_complex* out) {
+  return sieve_host(false, A, gamma, M, reps, B, out, nullptr);
+}
+int gbs_lhaf_repeated_cert_host(const gbs_complex* A, const gbs_complex* gamma, int M, const int* reps, int B,
+                                gbs_complex* out, double* bound) {
+  return sieve_host(true, A, gamma, M, reps, B, out, bound);
+}
 
 }  // extern "C"

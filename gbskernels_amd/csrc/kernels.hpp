@@ -3,10 +3,14 @@
 #pragma once
 
 #include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
 
 #include "common.hpp"
 #include "errors.hpp"
 #include "perm.hpp"
+#include "sieve.hpp"
 
 namespace gbsk {
 
@@ -33,5 +37,14 @@ int haf_launch_multi(bool loop, int tier, const HafJob* jobs, int njobs, hipStre
 size_t tor_workspace_bytes(int tier, int dim, int B);
 int tor_launch(int tier, int dim, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st);
 int tor_max_dim(int tier);
+
+// repeated-row loop hafnian (sieve): the host plan from the (B x M) host reps
+// table (one RepPat per pattern, copied to d_plan by the caller), then the chunk
+// and reduction launches; d_part holds sieve_partial_bytes(cert, parts) bytes
+int sieve_plan(int M, const int* reps, int B, std::vector<RepPat>& plan, int64_t* parts);
+size_t sieve_partial_bytes(bool cert, int64_t parts);
+int sieve_launch(bool cert, const cplx* d_A, const cplx* d_g, int M, const int* d_reps, int B,
+                 const std::vector<RepPat>& plan, const RepPat* d_plan, int64_t parts, void* d_part, cplx* d_out,
+                 double* d_bound, hipStream_t st);
 
 }  // namespace gbsk

@@ -85,6 +85,45 @@ def lhaf_dd_certified(a): return _eval_aux("lhaf_dd_certified", a)
 def tor_dd_certified(a): return _eval_aux("tor_dd_certified", a)
 
 
+def _repeated_args(name: str, A, gamma, reps):
+    """Shape checks of bindings/gbskernels_ext.cpp:296-347 (ValueError where the
+    binding raises std::invalid_argument)."""
+    a = np.ascontiguousarray(np.asarray(A, dtype=np.complex128))
+    if a.ndim != 2 or a.shape[0] != a.shape[1]:
+        raise ValueError(f"{name}: A must be (M, M)")
+    M = a.shape[0]
+    g = np.ascontiguousarray(np.asarray(gamma, dtype=np.complex128))
+    if g.ndim != 1 or g.shape[0] != M:
+        raise ValueError(f"{name}: gamma must be (M,)")
+    r = np.ascontiguousarray(np.asarray(reps, dtype=np.int32))
+    if r.ndim != 2 or r.shape[1] != M:
+        raise ValueError(f"{name}: reps must be (B, M)")
+    return a, g, r
+
+
+def lhaf_repeated(A, gamma, reps):
+    """Repeated-row loop hafnians: one (M, M) base matrix and (M,) loop vector,
+    a (B, M) int32 repetition table -> (B,) values (finite-difference sieve)."""
+    a, g, r = _repeated_args("lhaf_repeated", A, gamma, reps)
+    B = r.shape[0]
+    out = np.empty(max(B, 1), np.complex128)[:B]
+    _lib.check(_lib.lib().gbs_lhaf_repeated_host(_lib.ptr(a), _lib.ptr(g), a.shape[0], _lib.ptr(r), B,
+                                                 _lib.ptr(out)), "lhaf_repeated")
+    return out
+
+
+def lhaf_repeated_certified(A, gamma, reps):
+    """Certified sieve: (values, bounds) with |value - exact| <= bound; values
+    bit-identical to lhaf_repeated; over-cap rows give NaN and an inf bound."""
+    a, g, r = _repeated_args("lhaf_repeated_certified", A, gamma, reps)
+    B = r.shape[0]
+    out = np.empty(max(B, 1), np.complex128)[:B]
+    bnd = np.empty(max(B, 1), np.float64)[:B]
+    _lib.check(_lib.lib().gbs_lhaf_repeated_cert_host(_lib.ptr(a), _lib.ptr(g), a.shape[0], _lib.ptr(r), B,
+                                                      _lib.ptr(out), _lib.ptr(bnd)), "lhaf_repeated_certified")
+    return out, bnd
+
+
 class DeviceResult:
     """A (B,) complex128 result left in device memory (reference
     ``bindings/gbskernels_ext.cpp:209-228``). Owns the buffer; ``numpy()`` copies

@@ -186,6 +186,21 @@ int gbs_server_get_stats(gbs_server* s, gbs_server_stats* st);
 /* waits for every submitted request, then stops the dispatcher and frees everything */
 int gbs_server_destroy(gbs_server* s);
 
+/* ---- 7. repeated-row loop hafnian (finite-difference sieve) ---------------
+ * One base matrix A (M x M) and loop vector gamma (M) shared by the call; reps
+ * is a (B x M) int32 table of repetition counts. out[b] is the loop hafnian of A
+ * with row/column i repeated reps[b][i] times and gamma_i on each copy's
+ * diagonal (gamma = 0: the hafnian of the expansion). Per element: N = sum of the
+ * counts; N = 0 gives 1 (bound 0); M > 32, N > 64, a negative count or more than
+ * 2^40 sieve terms prod(n_i + 1) give NaN (bound +inf). The certified entry
+ * point's values are bit-identical to the plain one's. */
+/* core/host_api.cu:411-431 (bindings/gbskernels_ext.cpp:49) */
+int gbs_lhaf_repeated_host(const gbs_complex* A, const gbs_complex* gamma, int M, const int* reps, int B,
+                           gbs_complex* out);
+/* core/host_api.cu:435-458 (bindings/gbskernels_ext.cpp:50) */
+int gbs_lhaf_repeated_cert_host(const gbs_complex* A, const gbs_complex* gamma, int M, const int* reps, int B,
+                                gbs_complex* out, double* bound);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,7 @@
 //   * torontonian, subset determinants ........... cpu_ref/torontonian.py:48-64
 //   * recursive prefix-Cholesky torontonian ...... cpu_ref/tor_recursive.py:46-101
 //   * sum|term| indicators (kappa numerators) .... cpu_ref/diagnostics.py:30-170
+//   * repeated-row loop hafnian (sieve) .......... cpu_ref/repeated.py:61-122
 //   * DD twins of haf / lhaf / tor ............... core/hafnian_dd.cu:34-116,
 //                                                  core/loop_hafnian_dd.cu:40-113,
 //                                                  core/torontonian_dd.cu:23-96
@@ -728,6 +729,94 @@ cplx tor_q(const cplx* O, int n) {
   return qto(total);
 }
 
+// ------------------------------------------------------- repeated-row sieve
+// cpu_ref/repeated.py:61-122 (lhaf_repeated): loop hafnian of A expanded by the
+// repetition counts n (row/column i repeated n_i times, loop weight gamma_i on
+// each copy) by the finite-difference sieve
+//   lhaf = (N! 2^N)^-1 sum_{v <= n} (-1)^|v| prod C(n_i, v_i) S_N(x_v),
+//   x_v = n - 2v, S_N(x) = sum_k C(N, 2k) (2k-1)!! (x^T A x)^k (x . gamma)^(N-2k),
+// visiting v in numpy.ndindex order (last index fastest), keeping v <= n - v
+// lexicographically and doubling the strict half. R = double restates the
+// reference's arithmetic (x @ M @ x as a row vector times x); R = __float128 is
+// this oracle's high-precision twin.
+template <class R>
+struct CX {
+  R re, im;
+};
+template <class R>
+inline CX<R> cx_add(CX<R> a, CX<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <class R>
+inline CX<R> cx_mul(CX<R> a, CX<R> b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+template <class R>
+inline CX<R> cx_rscale(R s, CX<R> a) { return {s * a.re, s * a.im}; }
+
+inline uint64_t binom_u64(int n, int k) {  // exact for n <= 64
+  if (k < 0 || k > n) return 0;
+  if (k > n - k) k = n - k;
+  unsigned __int128 r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (unsigned)(n - k + i) / (unsigned)i;
+  return (uint64_t)r;
+}
+
+template <class R>
+CX<R> sieve_repeated(const cplx* A, const cplx* g, int M, const int* n) {
+  int N = 0;
+  for (int i = 0; i < M; ++i) N += n[i];
+  if (N == 0) return {1, 0};
+  const int kmax = N / 2;
+  // C(N, 2k) (2k-1)!! (an integer; __float128 keeps it exact up to 2^113)
+  std::vector<R> coeff(kmax + 1);
+  for (int k = 0; k <= kmax; ++k) {
+    __float128 df = 1;
+    for (int t = 2 * k - 1; t > 0; t -= 2) df *= t;
+    coeff[k] = (R)((__float128)binom_u64(N, 2 * k) * df);
+  }
+  std::vector<int> v(M, 0);
+  std::vector<R> x(M);
+  std::vector<CX<R>> y(M), mupow(kmax + 1);
+  CX<R> total{0, 0};
+  for (;;) {
+    int cmp = 0;  // v against its mirror n - v, lexicographic from index 0
+    for (int i = 0; i < M && !cmp; ++i) {
+      const int d = v[i] - (n[i] - v[i]);
+      if (d) cmp = d < 0 ? -1 : 1;
+    }
+    if (cmp <= 0) {
+      int vs = 0;
+      for (int i = 0; i < M; ++i) vs += v[i];
+      R c = (vs & 1) ? R(-1) : R(1);
+      if (cmp < 0) c *= 2;
+      for (int i = 0; i < M; ++i) c *= (R)binom_u64(n[i], v[i]);
+      for (int i = 0; i < M; ++i) x[i] = (R)(n[i] - 2 * v[i]);
+      // sig2 = (x @ A) @ x, mu = x @ gamma
+      for (int j = 0; j < M; ++j) {
+        CX<R> s{0, 0};
+        for (int i = 0; i < M; ++i) s = cx_add(s, cx_rscale(x[i], CX<R>{(R)A[i * M + j].re, (R)A[i * M + j].im}));
+        y[j] = s;
+      }
+      CX<R> sig2{0, 0}, mu{0, 0};
+      for (int j = 0; j < M; ++j) sig2 = cx_add(sig2, cx_rscale(x[j], y[j]));
+      for (int i = 0; i < M; ++i) mu = cx_add(mu, cx_rscale(x[i], CX<R>{(R)g[i].re, (R)g[i].im}));
+      mupow[kmax] = (N & 1) ? mu : CX<R>{1, 0};
+      for (int k = kmax - 1; k >= 0; --k) mupow[k] = cx_mul(mupow[k + 1], cx_mul(mu, mu));
+      CX<R> SN{0, 0}, sk{1, 0};
+      for (int k = 0; k <= kmax; ++k) {
+        SN = cx_add(SN, cx_mul(cx_rscale(coeff[k], sk), mupow[k]));
+        sk = cx_mul(sk, sig2);
+      }
+      total = cx_add(total, cx_rscale(c, SN));
+    }
+    int i = M - 1;  // advance, last index fastest
+    while (i >= 0 && v[i] == n[i]) v[i--] = 0;
+    if (i < 0) break;
+    ++v[i];
+  }
+  __float128 f = 1;
+  for (int t = 2; t <= N; ++t) f *= t;
+  const R D = (R)f * (R)ldexpq((__float128)1, N);
+  return {total.re / D, total.im / D};
+}
+
 template <class F>
 void for_batch(int B, F f) {
 #pragma omp parallel for schedule(dynamic, 1)
@@ -831,6 +920,22 @@ void or_tor_dd(const double* in, int n, int B, double* out) {
 void or_tor_q(const double* in, int n, int B, double* out) {
   for_batch(B, [&](int b) {
     cplx v = tor_q(as_c(in) + (size_t)b * 4 * n * n, n);
+    out[2 * b] = v.re;
+    out[2 * b + 1] = v.im;
+  });
+}
+
+// repeated-row sieve: A (M x M), gamma (M), reps (B x M int32); quad: 0 double, 1 __float128
+void or_lhaf_repeated(const double* A, const double* g, int M, const int* reps, int B, int quad, double* out) {
+  for_batch(B, [&](int b) {
+    cplx v;
+    if (quad) {
+      CX<__float128> r = sieve_repeated<__float128>(as_c(A), as_c(g), M, reps + (size_t)b * M);
+      v = {(double)r.re, (double)r.im};
+    } else {
+      CX<double> r = sieve_repeated<double>(as_c(A), as_c(g), M, reps + (size_t)b * M);
+      v = {r.re, r.im};
+    }
     out[2 * b] = v.re;
     out[2 * b + 1] = v.im;
   });

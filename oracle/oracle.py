@@ -36,6 +36,7 @@ _SIGS = {
     "or_tor_recursive": [_dp, ctypes.c_int, ctypes.c_int, _dp],
     "or_tor_dd": [_dp, ctypes.c_int, ctypes.c_int, _dp],
     "or_tor_q": [_dp, ctypes.c_int, ctypes.c_int, _dp],
+    "or_lhaf_repeated": [_dp, _dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int32), ctypes.c_int, ctypes.c_int, _dp],
     "or_gen_bench_complex": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp],
     "or_gen_bench_real": [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp],
 }
@@ -229,3 +230,32 @@ ABS = {"perm": perm_abs, "haf": haf_abs, "lhaf": lhaf_abs, "tor": tor_abs}
 FP64 = {"perm": perm, "haf": haf, "lhaf": lhaf, "tor": tor}
 DD = {"perm": perm_dd, "haf": haf_dd, "lhaf": lhaf_dd, "tor": tor_dd}
 QUAD = {"perm": perm_q, "haf": haf_q, "lhaf": lhaf_q, "tor": tor_q}
+
+
+def lhaf_repeated(A, gamma, reps, quad: bool = False) -> np.ndarray:
+    """Repeated-row loop hafnian by the sieve (cpu_ref/repeated.py:61-122):
+    base ``A`` (M, M), loop weights ``gamma`` (M,) (None -> zeros), counts
+    ``reps`` (B, M) or (M,); returns (B,) complex128 (double or __float128
+    arithmetic)."""
+    a = np.ascontiguousarray(np.asarray(A, np.complex128))
+    M = a.shape[0]
+    g = np.zeros(M, np.complex128) if gamma is None else np.ascontiguousarray(np.asarray(gamma, np.complex128))
+    r = np.ascontiguousarray(np.atleast_2d(np.asarray(reps, np.int32)))
+    if a.shape != (M, M) or g.shape != (M,) or r.shape[1] != M or (r < 0).any():
+        raise ValueError("lhaf_repeated: A (M, M), gamma (M,), reps (B, M) non-negative")
+    out = np.empty(r.shape[0], np.complex128)
+    lib().or_lhaf_repeated(_p(a.view(np.float64)), _p(g.view(np.float64)), M,
+                           r.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), r.shape[0], int(quad),
+                           _p(out.view(np.float64)))
+    return out
+
+
+def expand_repeated(A, gamma, reps) -> np.ndarray:
+    """The expanded matrix the sieve evaluates: row/column i of ``A`` repeated
+    ``reps[i]`` times, diagonal gamma_i on every copy of i."""
+    idx = np.repeat(np.arange(len(reps)), np.asarray(reps, np.int64))
+    E = np.array(np.asarray(A, np.complex128)[np.ix_(idx, idx)])
+    if len(idx):
+        g = np.zeros(len(reps), np.complex128) if gamma is None else np.asarray(gamma, np.complex128)
+        np.fill_diagonal(E, g[idx])
+    return E
