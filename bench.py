@@ -81,6 +81,8 @@ def main() -> int:
     ap.add_argument("--haf", type=int, default=1, help="also time haf/lhaf buckets at N=16,24 (0 skips)")
     ap.add_argument("--tor", type=int, default=1,
                     help="also time the torontonian (fp64, certified, certified-dd) at nm = 8, 12, 16, 20 (0 skips)")
+    ap.add_argument("--repeated", type=int, default=1,
+                    help="also time the repeated-row sieve workload, M = 6, q = 2..6 (0 skips)")
     ap.add_argument("--served-requests", type=int, default=40000,
                     help="requests of the served mixed-size workload (0 skips)")
     ap.add_argument("--served-batch", type=int, default=2048, help="requests per batch / window")
@@ -216,6 +218,9 @@ def main() -> int:
         per_kernel.update(side("haf/lhaf n=16,24", lambda: haf_kernels(L, torch, dev, stream)))
     if args.tor:
         per_kernel.update(side("tor nm=8,12,16,20 and haf config 1", lambda: tor_kernels(L, torch, dev, stream)))
+    repeated = None
+    if args.repeated:
+        repeated = side("repeated-row sieve", repeated_bench)
     served = None
     if args.served_requests > 0:
         served = side("served mix", lambda: served_bench(args))
@@ -242,9 +247,56 @@ def main() -> int:
         "parity": parity,
         "served": served,
         "streams": streams,
+        "repeated_ab": repeated,
     }
     print(json.dumps(line))
     return 0
+
+
+def sieve_flops(reps) -> float:
+    """Canonical flops of one repeated-row loop hafnian by the sieve as the
+    reference kernel writes it (core/repeated.cu:86-132): ceil(T / 2) evaluated
+    terms of 4m^2 + 8m + 22K + 20 real flops (sigma = x^T A x over the m nonzero
+    counts, mu, the mu-power ladder and the S_N loop; K = N / 2)."""
+    n = [int(x) for x in reps if int(x) > 0]
+    T = int(np.prod([x + 1 for x in n])) if n else 1
+    m, K = len(n), sum(n) // 2
+    return (T + 1) // 2 * (4.0 * m * m + 8.0 * m + 22.0 * K + 20.0)
+
+
+def repeated_bench():
+    """The reference's repeated-row sieve benchmark (bench/repeated_ab.py): base
+    A = 0.3 (G + G^T), M = 6, gamma = 0, 2048 patterns (q, ..., q), q = 2..6,
+    timed like the reference through the public path (H2D, launch, D2H), median
+    of 5 after 2 warm-ups; the checksum is compared with the reference's."""
+    import gbskernels_amd as gk
+    from gbskernels_amd.workloads import repeated_ab_base
+    gold = {}
+    gpath = Path(__file__).resolve().parent / "tests" / "golden" / "repeated_ab_checksums.json"
+    if gpath.exists():
+        gold = {r["q"]: r for r in json.loads(gpath.read_text())["rows"]}
+    A, gam = repeated_ab_base()
+    rows = []
+    for q in (2, 3, 4, 5, 6):
+        reps = np.full((2048, A.shape[0]), q, np.int32)
+        for _ in range(2):
+            gk.lhaf_repeated(A, gam, reps[:256])
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            v = gk.lhaf_repeated(A, gam, reps)
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        chk = complex(np.sum(v))
+        row = {"q": q, "batch": 2048, "sieve_terms": (q + 1) ** A.shape[0], "median_ms": t * 1e3,
+               "evals_per_s": 2048 / t, "tflops_canonical": 2048 * sieve_flops(reps[0]) / t / 1e12,
+               "checksum": [chk.real, chk.imag]}
+        if q in gold:
+            ref = complex(*gold[q]["checksum"])
+            row["checksum_rel_vs_reference"] = abs(chk - ref) / abs(ref)
+            row["reference_public_path_evals_per_s"] = gold[q]["public_path_evals_per_s"]
+        rows.append(row)
+    return {"workload": "bench/repeated_ab.py: M = 6, gamma = 0, 2048 x (q,...,q), public path", "rows": rows}
 
 
 def haf_kernels(L, torch, dev, stream):

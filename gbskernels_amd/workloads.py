@@ -112,3 +112,13 @@ def served_mix(n_req: int, seed: int, funcs=("perm", "haf", "lhaf"), nmax: int =
         items.append((f, t, M))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
     return items, arrivals
+
+
+def repeated_ab_base(modes: int = 3, seed: int = 0):
+    """Base matrix of the repeated-row sieve benchmark (reference
+    bench/repeated_ab.py:45-50): M = 2 modes, A = 0.3 (G + G^T) with G complex
+    standard normal from default_rng(seed), loop weights zero."""
+    M = 2 * modes
+    g = np.random.default_rng(seed)
+    A = g.standard_normal((M, M)) + 1j * g.standard_normal((M, M))
+    return np.ascontiguousarray(0.3 * (A + A.T)), np.zeros(M, np.complex128)

@@ -199,3 +199,24 @@ def test_public_api(ext):
     assert isinstance(v1, complex) and np.isfinite(d1["abs_error_bound"])
     with pytest.raises(FloatingPointError):
         gk.lhaf_repeated(A, gam, np.array([40, 25, 0]), certified=True)
+
+
+def test_reference_sieve_checksums(ext, oracle):
+    """bench/repeated_ab.py's workload (M = 6, q = 2..6, 2048 identical
+    patterns): the batch sum against the reference's published checksum, each
+    value enclosed by its certificate around the quad oracle."""
+    import json
+    from conftest import GOLDEN
+    from gbskernels_amd.workloads import repeated_ab_base
+    gold = json.loads((GOLDEN / "repeated_ab_checksums.json").read_text())["rows"]
+    A, gam = repeated_ab_base()
+    for row in gold:
+        reps = np.full((row["batch"], row["M"]), row["q"], np.int32)
+        v, bnd = ext.lhaf_repeated_certified(A, gam, reps)
+        assert np.all(v == v[0]) and np.all(bnd == bnd[0])
+        q = oracle.lhaf_repeated(A, gam, reps[0], quad=True)[0]
+        ref = complex(*row["checksum"]) / row["batch"]
+        assert abs(v[0] - q) <= bnd[0]
+        assert abs(v[0] - ref) <= bnd[0] + abs(ref - q)
+        chk = complex(np.sum(v))
+        assert abs(chk - complex(*row["checksum"])) <= 1e-9 * abs(chk)

@@ -74,3 +74,11 @@ def test_canonical_flop_counts():
         assert bench.tor_flops(nm) == pytest.approx(want, rel=2e-2)
     # 2^(n-1) (8n - 4) at n = 20
     assert math.isclose(bench.perm_flops(20), 2 ** 19 * 156)
+
+
+def test_sieve_canonical_flops():
+    """core/repeated.cu:86-132 per evaluated term: 4m^2 + 8m + 22K + 20."""
+    import bench
+    assert bench.sieve_flops((2,) * 6) == 365 * (4 * 36 + 48 + 22 * 6 + 20)
+    assert bench.sieve_flops((6,) * 6) == 58825 * (4 * 36 + 48 + 22 * 18 + 20)
+    assert bench.sieve_flops((0, 3, 0)) == 2 * (4 + 8 + 22 + 20)

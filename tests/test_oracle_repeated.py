@@ -119,3 +119,20 @@ def test_public_api_argument_checks():
         gk.lhaf_repeated(np.full((3, 3), np.nan), gam, (1, 1, 0), certified=True)
     with pytest.raises(NotImplementedError):
         gk.lhaf_repeated(A, gam, (1, 1, 0), backend="cpu")
+
+
+def test_oracle_matches_reference_sieve_checksums(oracle):
+    """The reference's published sieve checksums (2048 identical values of the
+    pattern (q, ..., q), bench/repeated_ab.py): the quad oracle reproduces each
+    value to the reference fp64 kernel's accuracy."""
+    import json
+    from conftest import GOLDEN
+    from gbskernels_amd.workloads import repeated_ab_base
+    gold = json.loads((GOLDEN / "repeated_ab_checksums.json").read_text())["rows"]
+    A, gam = repeated_ab_base()
+    for row in gold:
+        ref = complex(*row["checksum"]) / row["batch"]
+        q = oracle.lhaf_repeated(A, gam, [row["q"]] * row["M"], quad=True)[0]
+        d = oracle.lhaf_repeated(A, gam, [row["q"]] * row["M"])[0]
+        assert abs(ref - q) <= 1e-9 * abs(q), row["q"]
+        assert abs(d - q) <= 1e-9 * abs(q), row["q"]
