@@ -94,6 +94,28 @@ class GbskError(RuntimeError):
         super().__init__(f"gbskernels: GPU error {rc} ({msg}) in {where}")
 
 
+def _share_torch_hip_runtime() -> None:
+    """PyTorch-ROCm wheels bundle their own HIP and HSA runtimes (same sonames
+    as ROCm's). Two HSA runtimes in one process cannot both open the GPU: torch
+    initialised after a ROCm-runtime library finds no devices. Loading torch's
+    runtime first, by path and without importing torch, makes libgbsk's
+    ``libamdhip64.so.7`` / ``libhsa-runtime64.so.1`` resolve to it, so device
+    pointers, streams and DLPack hand-offs are shared with torch whichever is
+    initialised first. ``GBSK_SYSTEM_HIP=1`` keeps the ROCm runtime instead."""
+    if os.environ.get("GBSK_SYSTEM_HIP") == "1":
+        return
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    for loc in (spec.submodule_search_locations or []) if spec else []:
+        hip = Path(loc) / "lib" / "libamdhip64.so"
+        if hip.exists():
+            ctypes.CDLL(str(hip), mode=ctypes.RTLD_GLOBAL)
+            return
+
+
 def lib():
     """Load ``libgbsk.so`` once; raise if it is absent (no silent fallback)."""
     global _lib
@@ -103,6 +125,7 @@ def lib():
         raise RuntimeError(
             f"gbskernels_amd: the HIP library {LIB_PATH} is not built; run "
             "`python -c 'import __graft_entry__ as g; g.build()'` (hipcc, gfx950)")
+    _share_torch_hip_runtime()
     L = ctypes.CDLL(str(LIB_PATH))
     for name, (args, res) in SIGNATURES.items():
         fn = getattr(L, name)

@@ -124,10 +124,69 @@ def lhaf_repeated_certified(A, gamma, reps):
     return out, bnd
 
 
+# ---- DLPack export of device results (the reference returns an nb::ndarray
+# that exposes __dlpack__, bindings/gbskernels_ext.cpp:209-228) ----------------
+class _DLDevice(ctypes.Structure):
+    _fields_ = [("device_type", ctypes.c_int32), ("device_id", ctypes.c_int32)]
+
+
+class _DLDataType(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_uint8), ("bits", ctypes.c_uint8), ("lanes", ctypes.c_uint16)]
+
+
+class _DLTensor(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("device", _DLDevice), ("ndim", ctypes.c_int32),
+                ("dtype", _DLDataType), ("shape", ctypes.POINTER(ctypes.c_int64)),
+                ("strides", ctypes.POINTER(ctypes.c_int64)), ("byte_offset", ctypes.c_uint64)]
+
+
+class _DLManagedTensor(ctypes.Structure):
+    pass
+
+
+_DLDeleter = ctypes.CFUNCTYPE(None, ctypes.POINTER(_DLManagedTensor))
+_DLManagedTensor._fields_ = [("dl_tensor", _DLTensor), ("manager_ctx", ctypes.c_void_p), ("deleter", _DLDeleter)]
+
+KDL_ROCM = 10      # DLPack device type of HIP device memory
+_KDL_COMPLEX = 5
+_exported = {}     # managed-tensor address -> (struct, shape array, device pointer)
+
+
+def _dl_release(addr: int) -> None:
+    entry = _exported.pop(addr, None)
+    if entry is not None and entry[2]:
+        _lib.lib().gbs_dev_free(entry[2])
+
+
+@_DLDeleter
+def _dl_deleter(mt):
+    _dl_release(ctypes.addressof(mt.contents))
+
+
+_api = ctypes.pythonapi
+_api.PyCapsule_New.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+_api.PyCapsule_New.restype = ctypes.py_object
+_api.PyCapsule_IsValid.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+_api.PyCapsule_IsValid.restype = ctypes.c_int
+_api.PyCapsule_GetPointer.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+_api.PyCapsule_GetPointer.restype = ctypes.c_void_p
+
+
+@ctypes.CFUNCTYPE(None, ctypes.c_void_p)
+def _capsule_destructor(cap):
+    # a capsule nobody consumed keeps the name "dltensor": free its tensor here
+    # (a consumer renames it "used_dltensor" and calls the deleter itself)
+    if _api.PyCapsule_IsValid(cap, b"dltensor"):
+        _dl_release(_api.PyCapsule_GetPointer(cap, b"dltensor"))
+
+
 class DeviceResult:
     """A (B,) complex128 result left in device memory (reference
     ``bindings/gbskernels_ext.cpp:209-228``). Owns the buffer; ``numpy()`` copies
-    it to the host; ``ptr`` is the device address for zero-copy consumers."""
+    it to the host; ``ptr`` is the device address. ``__dlpack__`` hands the
+    buffer zero-copy to a DLPack consumer (``torch.from_dlpack(r)`` gives a
+    complex128 tensor on the GPU) and moves ownership to it; the values are
+    complete when the result is returned, so any consumer stream may read them."""
 
     def __init__(self, ptr: int, n: int):
         self.ptr = ptr
@@ -146,6 +205,27 @@ class DeviceResult:
     def __array__(self, dtype=None, copy=None):
         a = self.numpy()
         return a if dtype is None else a.astype(dtype)
+
+    def __dlpack_device__(self):
+        return (KDL_ROCM, 0)
+
+    def __dlpack__(self, stream=None, max_version=None, dl_device=None, copy=None):
+        if self.ptr is None and self.shape[0]:
+            raise BufferError("DeviceResult: the buffer was already exported or freed")
+        mt = _DLManagedTensor()
+        shape = (ctypes.c_int64 * 1)(self.shape[0])
+        mt.dl_tensor.data = self.ptr
+        mt.dl_tensor.device = _DLDevice(KDL_ROCM, 0)
+        mt.dl_tensor.ndim = 1
+        mt.dl_tensor.dtype = _DLDataType(_KDL_COMPLEX, 128, 1)
+        mt.dl_tensor.shape = shape
+        mt.dl_tensor.strides = None
+        mt.dl_tensor.byte_offset = 0
+        mt.manager_ctx = None
+        mt.deleter = _dl_deleter
+        _exported[ctypes.addressof(mt)] = (mt, shape, self.ptr)
+        self.ptr = None  # the capsule (then its consumer) owns the buffer now
+        return _api.PyCapsule_New(ctypes.addressof(mt), b"dltensor", ctypes.cast(_capsule_destructor, ctypes.c_void_p))
 
     def __del__(self):
         p, self.ptr = getattr(self, "ptr", None), None

@@ -130,6 +130,33 @@ def test_workspace_resident():
         assert np.array_equal(h.numpy(), gk.perm_batched(A, backend="gpu"))
 
 
+def test_resident_dlpack_into_torch():
+    """The resident result goes to PyTorch without a host round trip
+    (DLPack, device type ROCm); the consumer owns and frees the buffer."""
+    import gc
+    import torch
+    from gbskernels_amd.ext import gbskernels_ext as ext
+    A = rng_complex((40, 9, 9), 12)
+    want = gk.perm_batched(A, backend="gpu")
+    with gk.Workspace(backend="gpu") as ws:
+        h = ws.perm_resident(A)
+        assert h.__dlpack_device__() == (ext.KDL_ROCM, 0)
+        t = torch.from_dlpack(h)
+        assert t.is_cuda and t.dtype == torch.complex128 and tuple(t.shape) == (40,)
+        assert np.array_equal(t.cpu().numpy(), want)
+        assert h.ptr is None                      # ownership moved to the tensor
+        with pytest.raises(BufferError):
+            h.__dlpack__()
+        n_live = len(ext._exported)
+        del t
+        gc.collect()
+        torch.cuda.synchronize()
+        assert len(ext._exported) == n_live - 1   # the consumer called the deleter
+        cap = ws.haf_resident(rng_symmetric(3, 6, 2)).__dlpack__()
+        del cap                                   # never consumed: freed by the capsule
+        assert len(ext._exported) == n_live - 1
+
+
 def test_tor_api_real_domain(oracle):
     from physics import threshold_O
     rng = np.random.default_rng(3)
