@@ -37,7 +37,7 @@ __version__ = "0.1.0"
 __all__ = [
     "perm", "perm_batched", "haf", "haf_batched", "lhaf", "lhaf_batched",
     "tor", "tor_batched", "gpu_available", "gpu_backend_kind", "Workspace", "PRECISIONS",
-    "lhaf_repeated", "sieve_term_count",
+    "lhaf_repeated", "sieve_term_count", "tor_single",
 ]
 
 PRECISIONS = ("fp64", "ref", "dd", "auto", "certified")
@@ -349,6 +349,49 @@ def lhaf_repeated(A: Any, gamma: Any, reps: Any, backend: str = "gpu", certified
         return (complex(v[0]) if single else v), diag
     v = ext.lhaf_repeated(A, g, r)
     return complex(v[0]) if single else v
+
+
+# --- single-large torontonian (reference gbskernels/__init__.py:635-691) --------
+
+def tor_single(O: Any, groups: int | None = None, certified: bool = False, dd: bool = False):
+    """Torontonian of ONE real ``(2n, 2n)`` matrix, ``2n <= 64``, split across
+    the whole GPU. The matrix must be finite and exactly symmetric after
+    conversion to binary64; physical-domain inputs (``I - O_S`` SPD) only,
+    off-domain raises ``ValueError``. ``groups`` (default ``min(n, 14)``) is
+    checked for compatibility; the split is this backend's own. ``certified``
+    also returns ``{"tier", "abs_error_bound", "rel_error_bound"}``; ``dd=True``
+    (implies certified) carries the value in double-double."""
+    A = np.asarray(O)
+    if np.iscomplexobj(A):
+        if np.any(np.imag(A) != 0):
+            raise ValueError("tor_single is real-domain only; use tor() for complex O")
+        A = np.real(A)
+    M = np.ascontiguousarray(A, dtype=np.float64)
+    if M.ndim != 2 or M.shape[0] != M.shape[1] or M.shape[0] % 2 != 0:
+        raise ValueError(f"tor_single: O must be square with even dimension, got {M.shape}")
+    if M.shape[0] > 64:
+        raise ValueError(f"tor_single: dimension {M.shape[0]} exceeds the cap 64 (n <= 32 modes)")
+    if not np.all(np.isfinite(M)):
+        raise ValueError("tor_single: O must contain only finite binary64 values")
+    if not np.array_equal(M, M.T):
+        raise ValueError("tor_single: O must be exactly symmetric after conversion to binary64")
+    n = M.shape[0] // 2
+    g = min(n, 14) if groups is None else int(groups)
+    if not 0 <= g <= n:
+        raise ValueError(f"groups must be in [0, {n}]")
+    ext = _ext()
+    if certified or dd:
+        tier = "certified-dd" if dd else "certified-fp64"
+        v, e = (ext.tor_single_ddcertified if dd else ext.tor_single_certified)(M, g)
+        if not (np.isfinite(v) and np.isfinite(e) and e >= 0.0):
+            raise ValueError("tor_single: off the physical domain or uncertifiable "
+                             "(the bound refuses rather than overclaims)")
+        rel = e / abs(v) if v != 0 else float("inf")
+        return v, {"tier": tier, "abs_error_bound": e, "rel_error_bound": rel}
+    v = float(ext.tor_single(M, g))
+    if not np.isfinite(v):
+        raise ValueError("tor_single: input is off the physical domain (I - O_S not SPD)")
+    return v
 
 
 # --- Workspace (reference gbskernels/__init__.py:716-908) ----------------------

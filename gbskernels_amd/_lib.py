@@ -79,6 +79,9 @@ SIGNATURES = {
     "gbs_server_get_stats": ([_cp, ctypes.POINTER(ServerStats)], _i),
     "gbs_server_destroy": ([_cp], _i),
     "gbs_lhaf_repeated_host": ([_cp, _cp, _i, _cp, _i, _cp], _i),
+    "gbs_tor_recursive_single_host": ([_cp, _i, _i, _cp], _i),
+    "gbs_tor_single_certified_host": ([_cp, _i, _i, _cp, _cp], _i),
+    "gbs_tor_single_ddcertified_host": ([_cp, _i, _i, _cp, _cp], _i),
     "gbs_lhaf_repeated_cert_host": ([_cp, _cp, _i, _cp, _i, _cp, _cp], _i),
 }
 

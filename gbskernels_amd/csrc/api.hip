@@ -13,6 +13,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <vector>
 
 #include "../../include/gbsk.h"
 #include "common.hpp"
@@ -177,6 +178,37 @@ static int sieve_host(bool cert, const gbs_complex* hA, const gbs_complex* hg, i
   return rc ? rc : rs;
 }
 
+// single-large torontonian, one shot: the real (2n x 2n) O goes up as complex
+// storage; the subtree count g is validated only (the split is the engine's)
+static int tor_single_host(int tier, const double* hO, int n, int g, double* out, double* bound) {
+  const bool cert = tier == GBSK_CERTIFIED || tier == GBSK_DD_CERTIFIED;
+  if ((!hO && n > 0) || !out || (cert && !bound)) return GBSK_ERR_ARG;
+  if (n < 0 || n > tor_single_max_modes()) return GBSK_ERR_DIM;
+  if (g < 0 || g > n) return GBSK_ERR_ARG;
+  const int dim = 2 * n;
+  std::vector<cplx> h((size_t)dim * dim);
+  for (size_t i = 0; i < h.size(); ++i) h[i] = cplx{hO[i], 0.0};
+  Ctx& c = global_ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  int rc = c.init();
+  const size_t wsb = tor_single_workspace_bytes(tier, n);
+  if (!rc) rc = c.in.grow(h.empty() ? 16 : h.size() * sizeof(cplx));
+  if (!rc) rc = c.out.grow(sizeof(cplx));
+  if (!rc) rc = c.aux.grow(sizeof(double));
+  if (!rc && wsb) rc = c.ws.grow(wsb);
+  if (!rc && !h.empty()) rc = (int)hipMemcpyAsync(c.in.p, h.data(), h.size() * sizeof(cplx), hipMemcpyHostToDevice, c.stream);
+  if (!rc) rc = tor_single_launch(tier, n, (const cplx*)c.in.p, (cplx*)c.out.p, (double*)c.aux.p, c.ws.p, c.stream);
+  cplx v{0.0, 0.0};
+  double e = 0.0;
+  if (!rc) rc = (int)hipMemcpyAsync(&v, c.out.p, sizeof(cplx), hipMemcpyDeviceToHost, c.stream);
+  if (!rc && cert) rc = (int)hipMemcpyAsync(&e, c.aux.p, sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  const int rs = (int)hipStreamSynchronize(c.stream);  // also keeps `h`, `v`, `e` alive for their copies
+  if (rc || rs) return rc ? rc : rs;
+  *out = v.re;
+  if (cert) *bound = e;
+  return 0;
+}
+
 }  // namespace gbsk
 
 using namespace gbsk;
@@ -327,6 +359,16 @@ const char* gbs_error_string(int rc) {
 }
 
 int gbs_is_device_build(void) { return 1; }
+
+int gbs_tor_recursive_single_host(const double* O, int n, int g, double* out) {
+  return tor_single_host(GBSK_FP64, O, n, g, out, nullptr);
+}
+int gbs_tor_single_certified_host(const double* O, int n, int g, double* out, double* bound) {
+  return tor_single_host(GBSK_CERTIFIED, O, n, g, out, bound);
+}
+int gbs_tor_single_ddcertified_host(const double* O, int n, int g, double* out, double* bound) {
+  return tor_single_host(GBSK_DD_CERTIFIED, O, n, g, out, bound);
+}
 
 int gbs_lhaf_repeated_host(const gbs_complex* A, const gbs_complex* gamma, int M, const int* reps, int B,
                            gbs_complex* out) {

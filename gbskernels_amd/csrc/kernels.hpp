@@ -37,6 +37,11 @@ int haf_launch_multi(bool loop, int tier, const HafJob* jobs, int njobs, hipStre
 size_t tor_workspace_bytes(int tier, int dim, int B);
 int tor_launch(int tier, int dim, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st);
 int tor_max_dim(int tier);
+// one real instance of up to tor_single_max_modes() modes (input still complex
+// storage, imaginary parts zero); real path only, NaN / +inf off the domain
+int tor_single_max_modes();
+size_t tor_single_workspace_bytes(int tier, int nm);
+int tor_single_launch(int tier, int nm, const cplx* d_in, cplx* d_out, double* d_aux, void* ws, hipStream_t st);
 
 // repeated-row loop hafnian (sieve): the host plan from the (B x M) host reps
 // table (one RepPat per pattern, copied to d_plan by the caller), then the chunk

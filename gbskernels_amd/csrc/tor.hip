@@ -27,7 +27,8 @@
 
 namespace gbsk {
 
-constexpr int TOR_MAX_MODES = 20;  // dim 40
+constexpr int TOR_MAX_MODES = 32;        // the kernels: dim 64 (one instance at a time at the top)
+constexpr int TOR_BATCH_MAX_MODES = 20;  // batched entry points: dim 40
 constexpr int TOR_T2 = 6;          // cooperative middle modes (64 tasks)
 
 // real arithmetic policies: binary64 and double-double (common.hpp)
@@ -318,6 +319,17 @@ __host__ __device__ inline TorSplit tor_split(int nm, int rmax) {
   return s;
 }
 
+// one large instance: at least 2^10 leading patterns (workgroups) where the
+// mode count allows, the trailing modes per lane up to rmax
+inline TorSplit tor_single_split(int nm, int rmax) {
+  TorSplit s = tor_split(nm, rmax);
+  const int rest = nm - s.T2;
+  const int t1 = rest - rmax > (rest < 10 ? rest : 10) ? rest - rmax : (rest < 10 ? rest : 10);
+  s.T1 = t1;
+  s.R = rest - t1;
+  return s;
+}
+
 // packed lower-triangular index
 __device__ constexpr int tri(int r, int c) { return r * (r + 1) / 2 + c; }
 
@@ -432,14 +444,12 @@ __device__ inline void drop_mode(const T* X, int Dx, int md, T* Y) {
 // RERUN: only workgroups whose partial is NaN (a real-path breakdown or a
 // complex O) run, overwriting it (the complex fallback of the fp64 tier).
 template <class Ar, int R, int T2, bool KAPPA, bool RERUN>
-__global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ mats, int nm, int B,
+__global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ mats, int nm, int T1, int B,
                                                       typename Ar::P* __restrict__ part,
                                                       double* __restrict__ part_abs) {
   using T = typename Ar::T;
   extern __shared__ double lds_raw[];
-  T* lds = reinterpret_cast<T*>(lds_raw);
-  const TorSplit sp = tor_split(nm, Ar::RMAX);  // sp.R == R, sp.T2 == T2 (launch)
-  const int T1 = sp.T1;
+  T* lds = reinterpret_cast<T*>(lds_raw);  // split: T1 (argument) + T2 + R = nm
   const int dim = 2 * nm;
   const int lane = threadIdx.x;
   const int64_t wg = blockIdx.x;
@@ -600,59 +610,104 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
   }
 }
 
+// Reductions: one wave per instance, 4 instances per 256-thread block. Lane l
+// sums partials l, l + 64, l + 128, ... in order, then a fixed butterfly over
+// the 64 lanes; lane 0 writes. The association depends on np only, so plain,
+// kappa and certified (and dd, certified-dd) reductions give the same bits.
+constexpr int TOR_RED_WAVES = 4;
+__device__ inline int64_t tor_red_instance() { return (int64_t)blockIdx.x * TOR_RED_WAVES + (threadIdx.x >> 6); }
+static unsigned tor_red_grid(int B) { return (unsigned)(((int64_t)B + TOR_RED_WAVES - 1) / TOR_RED_WAVES); }
+
+// butterfly of dd values, operands in lane order so partners agree bit for bit
+__device__ inline dd dd_wave_sum(dd s) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+    const dd o{shfl_xor_d(s.hi, m), shfl_xor_d(s.lo, m)};
+    s = ((threadIdx.x & m) == 0) ? dd_add(s, o) : dd_add(o, s);
+  }
+  return s;
+}
+// the same with each addition's certified error (cert.hpp)
+__device__ inline dd cdd_wave_sum(dd s, double& e) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+    const dd o{shfl_xor_d(s.hi, m), shfl_xor_d(s.lo, m)};
+    const double eo = shfl_xor_d(e, m);
+    double r;
+    s = ((threadIdx.x & m) == 0) ? cdd_add(s, o, &r) : cdd_add(o, s, &r);
+    e = e + eo + r;
+  }
+  return s;
+}
+
 template <bool DD, bool KAPPA>
-__global__ void tor_reduce_kernel(const void* __restrict__ partv, const double* __restrict__ part_abs, int np,
-                                  int B, cplx* __restrict__ out, double* __restrict__ aux) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void tor_reduce_kernel(const void* __restrict__ partv,
+                                                         const double* __restrict__ part_abs, int np, int B,
+                                                         cplx* __restrict__ out, double* __restrict__ aux) {
+  const int64_t b = tor_red_instance();
+  const int lane = threadIdx.x & 63;
   if (b >= B) return;
   double a = 0.0;
   if constexpr (DD) {
-    const dd* part = reinterpret_cast<const dd*>(partv);
+    const dd* part = reinterpret_cast<const dd*>(partv) + b * np;
     dd s{0.0, 0.0};
-    for (int i = 0; i < np; ++i) s = dd_add(s, part[b * np + i]);
-    out[b] = cplx{dd_to_d(s), 0.0};
+    for (int i = lane; i < np; i += 64) s = dd_add(s, part[i]);
+    s = dd_wave_sum(s);
+    if (lane == 0) out[b] = cplx{dd_to_d(s), 0.0};
   } else {
-    const cplx* part = reinterpret_cast<const cplx*>(partv);
+    const cplx* part = reinterpret_cast<const cplx*>(partv) + b * np;
     cplx s{0.0, 0.0};
-    for (int i = 0; i < np; ++i) {
-      s = c_add(s, part[b * np + i]);
+    for (int i = lane; i < np; i += 64) {
+      s = c_add(s, part[i]);
       if (KAPPA) a += part_abs[b * np + i];
     }
-    out[b] = s;
+    s = group_sum<64>(s);
+    if (KAPPA) a = group_sum<64>(a);
+    if (lane == 0) out[b] = s;
   }
-  if (KAPPA) aux[b] = a;
+  if (KAPPA && lane == 0) aux[b] = a;
 }
 
-// certified reduce: the fp64 tier's fixed-order sum of the partials (same
-// bits) with the partials' bounds plus one rounding charge per addition
-__global__ void tor_reduce_cert_kernel(const cplx* __restrict__ part, const double* __restrict__ part_e, int np,
-                                       int B, double depth, cplx* __restrict__ out, double* __restrict__ aux) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// certified reduce: the fp64 tier's sum of the partials (same bits) with the
+// partials' bounds plus one rounding charge per addition
+__global__ __launch_bounds__(256) void tor_reduce_cert_kernel(const cplx* __restrict__ part,
+                                                              const double* __restrict__ part_e, int np, int B,
+                                                              double depth, cplx* __restrict__ out,
+                                                              double* __restrict__ aux) {
+  const int64_t b = tor_red_instance();
+  const int lane = threadIdx.x & 63;
   if (b >= B) return;
   cplx s{0.0, 0.0};
   double e = 0.0;
-  for (int i = 0; i < np; ++i) {
+  for (int i = lane; i < np; i += 64) {
     s = c_add(s, part[b * np + i]);
     e = cert_round(s, e + part_e[b * np + i]);
   }
+  cert_group_sum<64>(s, e);
+  if (lane != 0) return;
   out[b] = s;
   const double f = cert_finish(e, depth);
   aux[b] = (f == f) ? f : INFINITY;
 }
 
-// certified-dd reduce: the dd tier's fixed-order dd sum of the partials (same
-// bits) and its collapse to binary64, whose rounding is measured exactly
-__global__ void tor_reduce_ddcert_kernel(const dd* __restrict__ part, const double* __restrict__ part_e, int np,
-                                         int B, double depth, cplx* __restrict__ out, double* __restrict__ aux) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// certified-dd reduce: the dd tier's dd sum of the partials (same bits) and
+// its collapse to binary64, whose rounding is measured exactly
+__global__ __launch_bounds__(256) void tor_reduce_ddcert_kernel(const dd* __restrict__ part,
+                                                                const double* __restrict__ part_e, int np, int B,
+                                                                double depth, cplx* __restrict__ out,
+                                                                double* __restrict__ aux) {
+  const int64_t b = tor_red_instance();
+  const int lane = threadIdx.x & 63;
   if (b >= B) return;
   dd s{0.0, 0.0};
   double e = 0.0;
-  for (int i = 0; i < np; ++i) {
+  for (int i = lane; i < np; i += 64) {
     double r;
     s = cdd_add(s, part[b * np + i], &r);
     e = e + part_e[b * np + i] + r;
   }
+  s = cdd_wave_sum(s, e);
+  if (lane != 0) return;
   const double v = dd_to_d(s);
   e += fabs(two_sum(s.hi, s.lo).lo);  // |v - (s.hi + s.lo)|, exact
   out[b] = cplx{v, 0.0};
@@ -669,22 +724,27 @@ __global__ void tor_mark_kernel(const cplx* __restrict__ out, int np, int B, cpl
   cpart[e] = (v.re != v.re) ? cplx{NAN, NAN} : cplx{0.0, 0.0};
 }
 
-// instances the complex kernel recomputed take its fixed-order sum
-// (certified tier: such an instance's bound is +inf -- the complex path is
-// not certified)
+// instances the complex kernel recomputed take its sum (same association as
+// the real reduce; certified tier: such an instance's bound is +inf -- the
+// complex path is not certified)
 template <bool KAPPA, bool CERT>
-__global__ void tor_merge_kernel(const cplx* __restrict__ cpart, const double* __restrict__ cpart_abs, int np, int B,
-                                 cplx* __restrict__ out, double* __restrict__ aux) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void tor_merge_kernel(const cplx* __restrict__ cpart,
+                                                        const double* __restrict__ cpart_abs, int np, int B,
+                                                        cplx* __restrict__ out, double* __restrict__ aux) {
+  const int64_t b = tor_red_instance();
+  const int lane = threadIdx.x & 63;
   if (b >= B) return;
   const cplx v = out[b];
-  if (!(v.re != v.re)) return;
+  if (!(v.re != v.re)) return;  // wave-uniform (one instance per wave)
   cplx s{0.0, 0.0};
   double a = 0.0;
-  for (int i = 0; i < np; ++i) {
+  for (int i = lane; i < np; i += 64) {
     s = c_add(s, cpart[b * np + i]);
     if (KAPPA) a += cpart_abs[b * np + i];
   }
+  s = group_sum<64>(s);
+  if (KAPPA) a = group_sum<64>(a);
+  if (lane != 0) return;
   out[b] = s;
   if (KAPPA) aux[b] = a;
   if (CERT) aux[b] = INFINITY;
@@ -698,8 +758,7 @@ __global__ void tor_n0_kernel(int B, cplx* out, double* aux, double auxval) {
 }
 
 template <class Ar>
-static size_t tor_lds_bytes(int nm) {
-  const TorSplit sp = tor_split(nm, Ar::RMAX);
+static size_t tor_lds_bytes(int nm, const TorSplit& sp) {
   const int dim = 2 * nm;
   size_t lev = 0;
   for (int d = 0; d <= sp.T2; ++d) {
@@ -714,24 +773,28 @@ static size_t tor_lds_bytes(int nm) {
 }
 
 template <class Ar, int R, int T2, bool KAPPA, bool RERUN>
-static void launch_tor_rt(int nm, const cplx* d_in, int B, typename Ar::P* part, double* part_abs, dim3 grid,
+static void launch_tor_rt(int nm, int T1, const cplx* d_in, int B, typename Ar::P* part, double* part_abs, dim3 grid,
                           size_t lds, hipStream_t st) {
-  hipLaunchKernelGGL((tor_real_kernel<Ar, R, T2, KAPPA, RERUN>), grid, dim3(64), lds, st, d_in, nm, B, part, part_abs);
+  if (lds > 65536)  // dim 64 at the wider tiers: up to gfx950's 160 KiB per workgroup
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&tor_real_kernel<Ar, R, T2, KAPPA, RERUN>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((tor_real_kernel<Ar, R, T2, KAPPA, RERUN>), grid, dim3(64), lds, st, d_in, nm, T1, B, part,
+                     part_abs);
 }
 
 // the split's R and T2 are template arguments: T2 = 6 with R = 0..RMAX
 // (nm >= 6), or T2 = nm < 6 with R = 0
 template <class Ar, bool KAPPA, bool RERUN>
-static int launch_tor(int nm, const cplx* d_in, int B, typename Ar::P* part, double* part_abs, hipStream_t st) {
-  const TorSplit sp = tor_split(nm, Ar::RMAX);
+static int launch_tor(int nm, const TorSplit& sp, const cplx* d_in, int B, typename Ar::P* part, double* part_abs,
+                      hipStream_t st) {
   const int np = 1 << sp.T1;
-  const size_t lds = tor_lds_bytes<Ar>(nm);
+  const size_t lds = tor_lds_bytes<Ar>(nm, sp);
   const dim3 grid((unsigned)((size_t)B * np));
   if (sp.T2 == TOR_T2) {
     switch (sp.R) {
 #define TOR_CASE(RR)                                                                                             \
   case RR:                                                                                                       \
-    if constexpr (RR <= Ar::RMAX) launch_tor_rt<Ar, RR, TOR_T2, KAPPA, RERUN>(nm, d_in, B, part, part_abs, grid, lds, st); \
+    if constexpr (RR <= Ar::RMAX) launch_tor_rt<Ar, RR, TOR_T2, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, lds, st); \
     break;
       TOR_CASE(0)
       TOR_CASE(1)
@@ -746,11 +809,11 @@ static int launch_tor(int nm, const cplx* d_in, int B, typename Ar::P* part, dou
   }
   if (sp.R != 0) return GBSK_ERR_DIM;
   switch (sp.T2) {
-    case 1: launch_tor_rt<Ar, 0, 1, KAPPA, RERUN>(nm, d_in, B, part, part_abs, grid, lds, st); break;
-    case 2: launch_tor_rt<Ar, 0, 2, KAPPA, RERUN>(nm, d_in, B, part, part_abs, grid, lds, st); break;
-    case 3: launch_tor_rt<Ar, 0, 3, KAPPA, RERUN>(nm, d_in, B, part, part_abs, grid, lds, st); break;
-    case 4: launch_tor_rt<Ar, 0, 4, KAPPA, RERUN>(nm, d_in, B, part, part_abs, grid, lds, st); break;
-    case 5: launch_tor_rt<Ar, 0, 5, KAPPA, RERUN>(nm, d_in, B, part, part_abs, grid, lds, st); break;
+    case 1: launch_tor_rt<Ar, 0, 1, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, lds, st); break;
+    case 2: launch_tor_rt<Ar, 0, 2, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, lds, st); break;
+    case 3: launch_tor_rt<Ar, 0, 3, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, lds, st); break;
+    case 4: launch_tor_rt<Ar, 0, 4, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, lds, st); break;
+    case 5: launch_tor_rt<Ar, 0, 5, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, lds, st); break;
     default: return GBSK_ERR_DIM;
   }
   return 0;
@@ -776,21 +839,21 @@ static int run_tor_fp64(int nm, const cplx* d_in, int B, cplx* d_out, double* d_
   // and the complex kernel recomputes whole instances whose sum is NaN
   const int npr = 1 << sr.T1, npc = 1 << sc.T1;
   double* part_abs = reinterpret_cast<double*>(part + (size_t)B * npr);
-  int rc = launch_tor<ArR, KAPPA, false>(nm, d_in, B, part, part_abs, st);
+  int rc = launch_tor<ArR, KAPPA, false>(nm, sr, d_in, B, part, part_abs, st);
   if (rc) return rc;
   if constexpr (CERT)
-    hipLaunchKernelGGL(tor_reduce_cert_kernel, dim3((B + 255) / 256), dim3(256), 0, st, part, part_abs, npr, B,
+    hipLaunchKernelGGL(tor_reduce_cert_kernel, dim3(tor_red_grid(B)), dim3(256), 0, st, part, part_abs, npr, B,
                        tor_cert_depth(nm, npr), d_out, d_aux);
   else
-    hipLaunchKernelGGL((tor_reduce_kernel<false, KAPPA>), dim3((B + 255) / 256), dim3(256), 0, st, part, part_abs,
+    hipLaunchKernelGGL((tor_reduce_kernel<false, KAPPA>), dim3(tor_red_grid(B)), dim3(256), 0, st, part, part_abs,
                        npr, B, d_out, d_aux);
   cplx* cpart = reinterpret_cast<cplx*>(part_abs + (size_t)B * npr);
   double* cpart_abs = reinterpret_cast<double*>(cpart + (size_t)B * npc);
   hipLaunchKernelGGL(tor_mark_kernel, dim3((unsigned)(((size_t)B * npc + 255) / 256)), dim3(256), 0, st, d_out, npc,
                      B, cpart);
-  rc = launch_tor<RC64, KAPPA, true>(nm, d_in, B, cpart, cpart_abs, st);
+  rc = launch_tor<RC64, KAPPA, true>(nm, sc, d_in, B, cpart, cpart_abs, st);
   if (rc) return rc;
-  hipLaunchKernelGGL((tor_merge_kernel<KAPPA, CERT>), dim3((B + 255) / 256), dim3(256), 0, st, cpart, cpart_abs, npc,
+  hipLaunchKernelGGL((tor_merge_kernel<KAPPA, CERT>), dim3(tor_red_grid(B)), dim3(256), 0, st, cpart, cpart_abs, npc,
                      B, d_out, d_aux);
   return (int)hipGetLastError();
 }
@@ -799,9 +862,9 @@ static int run_tor_dd(int nm, const cplx* d_in, int B, cplx* d_out, void* ws, hi
   const TorSplit sp = tor_split(nm, RDD::RMAX);
   const int np = 1 << sp.T1;
   dd* part = reinterpret_cast<dd*>(ws);
-  int rc = launch_tor<RDD, false, false>(nm, d_in, B, part, nullptr, st);
+  int rc = launch_tor<RDD, false, false>(nm, sp, d_in, B, part, nullptr, st);
   if (rc) return rc;
-  hipLaunchKernelGGL((tor_reduce_kernel<true, false>), dim3((B + 255) / 256), dim3(256), 0, st, part, nullptr, np, B,
+  hipLaunchKernelGGL((tor_reduce_kernel<true, false>), dim3(tor_red_grid(B)), dim3(256), 0, st, part, nullptr, np, B,
                      d_out, nullptr);
   return (int)hipGetLastError();
 }
@@ -813,19 +876,19 @@ static int run_tor_ddcert(int nm, const cplx* d_in, int B, cplx* d_out, double* 
   const int np = 1 << sp.T1;
   dd* part = reinterpret_cast<dd*>(ws);
   double* part_e = reinterpret_cast<double*>(part + (size_t)B * np);
-  int rc = launch_tor<RCDD, false, false>(nm, d_in, B, part, part_e, st);
+  int rc = launch_tor<RCDD, false, false>(nm, sp, d_in, B, part, part_e, st);
   if (rc) return rc;
   // bound rounding depth: per mode <= 4 divisions / square roots and 8
   // products of ~12 bound operations each, the 2^R leaf sums, the butterfly
   // and the np-term reduce
   const double depth = 160.0 * nm + 64.0 + 96.0 + 4.0 * np + 64.0;
-  hipLaunchKernelGGL(tor_reduce_ddcert_kernel, dim3((B + 255) / 256), dim3(256), 0, st, part, part_e, np, B, depth,
+  hipLaunchKernelGGL(tor_reduce_ddcert_kernel, dim3(tor_red_grid(B)), dim3(256), 0, st, part, part_e, np, B, depth,
                      d_out, d_aux);
   return (int)hipGetLastError();
 }
 
 size_t tor_workspace_bytes(int tier, int dim, int B) {
-  if (dim < 2 || (dim & 1) || dim > 2 * TOR_MAX_MODES || B <= 0) return 0;
+  if (dim < 2 || (dim & 1) || dim > 2 * TOR_BATCH_MAX_MODES || B <= 0) return 0;
   if (tier == GBSK_DD || tier == GBSK_DD_CERTIFIED) {
     const TorSplit sp = tor_split(dim / 2, RDD::RMAX);
     return (size_t)B * ((size_t)1 << sp.T1) * (sizeof(dd) + sizeof(double));
@@ -841,14 +904,14 @@ int tor_max_dim(int tier) {
     case GBSK_DD:
     case GBSK_KAPPA:
     case GBSK_CERTIFIED:
-    case GBSK_DD_CERTIFIED: return 2 * TOR_MAX_MODES;
+    case GBSK_DD_CERTIFIED: return 2 * TOR_BATCH_MAX_MODES;
   }
   return 0;
 }
 
 int tor_launch(int tier, int dim, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
   if (B <= 0) return 0;
-  if (dim < 0 || (dim & 1) || dim > 2 * TOR_MAX_MODES) return GBSK_ERR_DIM;
+  if (dim < 0 || (dim & 1) || dim > 2 * TOR_BATCH_MAX_MODES) return GBSK_ERR_DIM;
   if (tier < GBSK_FP64 || tier > GBSK_DD_CERTIFIED) return GBSK_ERR_FUNC;
   const int nm = dim / 2;
   if (nm == 0) {  // tor of the empty matrix: exactly 1 (kappa 1, bound 0)
@@ -883,6 +946,70 @@ int tor_launch(int tier, int dim, const cplx* d_in, int B, cplx* d_out, double* 
     if (rc) return rc;
   }
   return 0;
+}
+
+// ------------------------------------------------ one large real instance
+// The single-large torontonian (reference core/tor_recursive.cu:148-678 via
+// core/host_api.cu:262-407): the same kernels on one instance, dim <= 64, with
+// a wider leading split (tor_single_split) so that one evaluation fills the
+// GPU. Real path only: off the physical domain the value is NaN and a bound is
+// +inf (no complex fallback).
+int tor_single_max_modes() { return TOR_MAX_MODES; }
+
+static int tor_single_rmax(int tier) { return (tier == GBSK_DD || tier == GBSK_DD_CERTIFIED) ? RDD::RMAX : RF64::RMAX; }
+
+size_t tor_single_workspace_bytes(int tier, int nm) {
+  if (nm < 1 || nm > TOR_MAX_MODES) return 0;
+  const TorSplit sp = tor_single_split(nm, tor_single_rmax(tier));
+  const size_t el = (tier == GBSK_DD || tier == GBSK_DD_CERTIFIED) ? sizeof(dd) : sizeof(cplx);
+  return ((size_t)1 << sp.T1) * (el + sizeof(double));
+}
+
+int tor_single_launch(int tier, int nm, const cplx* d_in, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
+  if (nm < 0 || nm > TOR_MAX_MODES) return GBSK_ERR_DIM;
+  if (tier != GBSK_FP64 && tier != GBSK_DD && tier != GBSK_CERTIFIED && tier != GBSK_DD_CERTIFIED)
+    return GBSK_ERR_FUNC;
+  const bool cert = tier == GBSK_CERTIFIED || tier == GBSK_DD_CERTIFIED;
+  if (cert && !d_aux) return GBSK_ERR_ARG;
+  if (nm == 0) {
+    hipLaunchKernelGGL(tor_n0_kernel, dim3(1), dim3(64), 0, st, 1, d_out, cert ? d_aux : nullptr, 0.0);
+    return (int)hipGetLastError();
+  }
+  if (!ws) return GBSK_ERR_WORKSPACE;
+  const TorSplit sp = tor_single_split(nm, tor_single_rmax(tier));
+  const int np = 1 << sp.T1;
+  int rc;
+  if (tier == GBSK_FP64 || tier == GBSK_CERTIFIED) {
+    cplx* part = reinterpret_cast<cplx*>(ws);
+    double* part_abs = reinterpret_cast<double*>(part + np);
+    if (tier == GBSK_FP64) {
+      rc = launch_tor<RF64, false, false>(nm, sp, d_in, 1, part, nullptr, st);
+      if (!rc)
+        hipLaunchKernelGGL((tor_reduce_kernel<false, false>), dim3(1), dim3(256), 0, st, part, nullptr, np, 1, d_out,
+                           nullptr);
+    } else {
+      rc = launch_tor<RCERT, false, false>(nm, sp, d_in, 1, part, part_abs, st);
+      if (!rc)
+        hipLaunchKernelGGL(tor_reduce_cert_kernel, dim3(1), dim3(256), 0, st, part, part_abs, np, 1,
+                           tor_cert_depth(nm, np), d_out, d_aux);
+    }
+  } else {
+    dd* part = reinterpret_cast<dd*>(ws);
+    double* part_e = reinterpret_cast<double*>(part + np);
+    if (tier == GBSK_DD) {
+      rc = launch_tor<RDD, false, false>(nm, sp, d_in, 1, part, nullptr, st);
+      if (!rc)
+        hipLaunchKernelGGL((tor_reduce_kernel<true, false>), dim3(1), dim3(256), 0, st, part, nullptr, np, 1, d_out,
+                           nullptr);
+    } else {
+      rc = launch_tor<RCDD, false, false>(nm, sp, d_in, 1, part, part_e, st);
+      const double depth = 160.0 * nm + 64.0 + 96.0 + 4.0 * np + 64.0;
+      if (!rc)
+        hipLaunchKernelGGL(tor_reduce_ddcert_kernel, dim3(1), dim3(256), 0, st, part, part_e, np, 1, depth, d_out,
+                           d_aux);
+    }
+  }
+  return rc ? rc : (int)hipGetLastError();
 }
 
 }  // namespace gbsk

@@ -124,6 +124,42 @@ def lhaf_repeated_certified(A, gamma, reps):
     return out, bnd
 
 
+def _tor_single_arg(name: str, O) -> np.ndarray:
+    """Checks of bindings/gbskernels_ext.cpp:257-295: real (2n, 2n), 2n <= 64."""
+    M = np.ascontiguousarray(np.asarray(O, dtype=np.float64))
+    if M.ndim != 2 or M.shape[0] != M.shape[1] or M.shape[0] % 2 != 0 or M.shape[0] > 64:
+        raise ValueError(f"{name}: O must be (2n, 2n) real, 2n <= 64")
+    return M
+
+
+def tor_single(O, g):
+    """Single-large torontonian of one real (2n, 2n) O (2n <= 64) across the
+    whole GPU; NaN off the physical (SPD) domain."""
+    M = _tor_single_arg("tor_single", O)
+    out = ctypes.c_double(0.0)
+    _lib.check(_lib.lib().gbs_tor_recursive_single_host(_lib.ptr(M), M.shape[0] // 2, int(g),
+                                                        ctypes.addressof(out)), "tor_single")
+    return out.value
+
+
+def _tor_single_cert(name: str, fn: str, O, g):
+    M = _tor_single_arg(name, O)
+    out, bnd = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    _lib.check(getattr(_lib.lib(), fn)(_lib.ptr(M), M.shape[0] // 2, int(g), ctypes.addressof(out),
+                                       ctypes.addressof(bnd)), name)
+    return out.value, bnd.value
+
+
+def tor_single_certified(O, g):
+    """Certified single-large torontonian -> (value, bound); NaN / inf off the domain."""
+    return _tor_single_cert("tor_single_certified", "gbs_tor_single_certified_host", O, g)
+
+
+def tor_single_ddcertified(O, g):
+    """Certified double-double single-large torontonian -> (value, bound)."""
+    return _tor_single_cert("tor_single_ddcertified", "gbs_tor_single_ddcertified_host", O, g)
+
+
 # ---- DLPack export of device results (the reference returns an nb::ndarray
 # that exposes __dlpack__, bindings/gbskernels_ext.cpp:209-228) ----------------
 class _DLDevice(ctypes.Structure):

@@ -201,6 +201,19 @@ int gbs_lhaf_repeated_host(const gbs_complex* A, const gbs_complex* gamma, int M
 int gbs_lhaf_repeated_cert_host(const gbs_complex* A, const gbs_complex* gamma, int M, const int* reps, int B,
                                 gbs_complex* out, double* bound);
 
+/* ---- 8. single-large torontonian -------------------------------------------
+ * One real (2n x 2n) O, row-major doubles, xxpp order, n <= 32 modes (dim 64),
+ * split across the whole GPU. Real physical domain only: if I - O_S is not SPD
+ * for some subset the value is NaN (bound +inf). g is the reference's subtree
+ * exponent, accepted for compatibility and checked (0 <= g <= n); the work split
+ * is this library's own and the value does not depend on g. */
+/* core/host_api.cu:262-289 (bindings/gbskernels_ext.cpp:257-268) */
+int gbs_tor_recursive_single_host(const double* O, int n, int g, double* out);
+/* core/host_api.cu:294-344 (bindings/gbskernels_ext.cpp:271-282) */
+int gbs_tor_single_certified_host(const double* O, int n, int g, double* out, double* bound);
+/* core/host_api.cu:350-404 (bindings/gbskernels_ext.cpp:285-295) */
+int gbs_tor_single_ddcertified_host(const double* O, int n, int g, double* out, double* bound);
+
 #ifdef __cplusplus
 }
 #endif
