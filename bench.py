@@ -83,6 +83,8 @@ def main() -> int:
                     help="also time the torontonian (fp64, certified, certified-dd) at nm = 8, 12, 16, 20 (0 skips)")
     ap.add_argument("--repeated", type=int, default=1,
                     help="also time the repeated-row sieve workload, M = 6, q = 2..6 (0 skips)")
+    ap.add_argument("--tor-single", type=int, default=1,
+                    help="also time the single-large torontonian at 16..32 modes (0 skips)")
     ap.add_argument("--served-requests", type=int, default=40000,
                     help="requests of the served mixed-size workload (0 skips)")
     ap.add_argument("--served-batch", type=int, default=2048, help="requests per batch / window")
@@ -221,6 +223,9 @@ def main() -> int:
     repeated = None
     if args.repeated:
         repeated = side("repeated-row sieve", repeated_bench)
+    tor_single = None
+    if args.tor_single:
+        tor_single = side("single-large torontonian", tor_single_bench)
     served = None
     if args.served_requests > 0:
         served = side("served mix", lambda: served_bench(args))
@@ -248,6 +253,7 @@ def main() -> int:
         "served": served,
         "streams": streams,
         "repeated_ab": repeated,
+        "tor_single": tor_single,
     }
     print(json.dumps(line))
     return 0
@@ -297,6 +303,44 @@ def repeated_bench():
             row["reference_public_path_evals_per_s"] = gold[q]["public_path_evals_per_s"]
         rows.append(row)
     return {"workload": "bench/repeated_ab.py: M = 6, gamma = 0, 2048 x (q,...,q), public path", "rows": rows}
+
+
+def tor_single_bench():
+    """One large torontonian per call, the reference's own timing workload
+    (core/bench_kernels.cu:368-410): O = 0.5 (G + G^T) + 0.15 I with G = 0.02
+    U(-1, 1) (numpy default_rng(4321 + n) here), n = 16, 20, 24, 26 modes and
+    32 (the cap), timed through the public path, median of 7 after a warm-up;
+    certified and certified-dd at n = 16, 20. The fp64 value of these inputs is
+    rounding-dominated (exact ~0.18^n); the certified-dd value is the accurate one."""
+    from gbskernels_amd.ext import gbskernels_ext as ext
+    ref = {}
+    gpath = Path(__file__).resolve().parent / "tests" / "golden" / "tor_single_reference.json"
+    if gpath.exists():
+        ref = {r["matrix_dim"]: r["evals_per_sec_median"] for r in json.loads(gpath.read_text())["rows"]}
+    rows = []
+    for n in (16, 20, 24, 26, 32):
+        rng = np.random.default_rng(4321 + n)
+        G = 0.02 * rng.uniform(-1.0, 1.0, (2 * n, 2 * n))
+        O = 0.5 * (G + G.T) + 0.15 * np.eye(2 * n)
+        tiers = [("fp64", lambda: ext.tor_single(O, 14))]
+        if n <= 20:
+            tiers += [("certified", lambda: ext.tor_single_certified(O, 14)),
+                      ("certified_dd", lambda: ext.tor_single_ddcertified(O, 14))]
+        for tier, fn in tiers:
+            reps = 7 if n <= 26 else 3
+            fn()
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                val = fn()
+                ts.append(time.perf_counter() - t0)
+            t = float(np.median(ts))
+            row = {"modes": n, "matrix_dim": 2 * n, "tier": tier, "median_ms": t * 1e3, "evals_per_s": 1.0 / t,
+                   "tflops_canonical": tor_flops(n) / t / 1e12, "value": val}
+            if tier == "fp64" and 2 * n in ref:
+                row["reference_evals_per_s"] = ref[2 * n]
+            rows.append(row)
+    return {"workload": "core/bench_kernels.cu:368-410 inputs (numpy generator), public path", "rows": rows}
 
 
 def haf_kernels(L, torch, dev, stream):
