@@ -85,6 +85,8 @@ def main() -> int:
                     help="also time the repeated-row sieve workload, M = 6, q = 2..6 (0 skips)")
     ap.add_argument("--tor-single", type=int, default=1,
                     help="also time the single-large torontonian at 16..32 modes (0 skips)")
+    ap.add_argument("--sampler", type=int, default=1,
+                    help="also time the resident GBS sampler on the reference's sampler configurations (0 skips)")
     ap.add_argument("--served-requests", type=int, default=40000,
                     help="requests of the served mixed-size workload (0 skips)")
     ap.add_argument("--served-batch", type=int, default=2048, help="requests per batch / window")
@@ -226,6 +228,9 @@ def main() -> int:
     tor_single = None
     if args.tor_single:
         tor_single = side("single-large torontonian", tor_single_bench)
+    sampler = None
+    if args.sampler:
+        sampler = side("resident sampler", sampler_bench)
     served = None
     if args.served_requests > 0:
         served = side("served mix", lambda: served_bench(args))
@@ -254,6 +259,7 @@ def main() -> int:
         "streams": streams,
         "repeated_ab": repeated,
         "tor_single": tor_single,
+        "sampler": sampler,
     }
     print(json.dumps(line))
     return 0
@@ -341,6 +347,42 @@ def tor_single_bench():
                 row["reference_evals_per_s"] = ref[2 * n]
             rows.append(row)
     return {"workload": "core/bench_kernels.cu:368-410 inputs (numpy generator), public path", "rows": rows}
+
+
+def sampler_bench():
+    """End-to-end samples/s of the resident chain-rule sampler on the reference's
+    sampler-throughput states (bench/sampler_throughput.py:38-49: r ~ U[0.3, 0.6]
+    per mode, Haar interferometer, default_rng(0)); median of 7 after a warm-up,
+    host reduced-A computation included. The (modes, cutoff, samples) cells are
+    the reference's resident rows and its repeated-row sieve rows; its published
+    4090 figures for each are quoted beside."""
+    from gbskernels_amd.sampling import sample_resident
+    ref = {(3, 3, 2000): ("gpu-resident", 3.321e4), (2, 4, 2000): ("gpu-resident", 1.463e4),
+           (5, 2, 2000): ("gpu-resident", 8024.0), (6, 6, 300): ("gpu+sieve", 135.3),
+           (8, 5, 300): ("gpu+sieve", 23.88), (10, 4, 300): ("gpu+nosieve", 6.631),
+           (5, 8, 300): ("gpu+sieve", 628.1)}
+    rows = []
+    for (m, c, n), (path, rate) in ref.items():
+        g = np.random.default_rng(0)
+        r = g.uniform(0.3, 0.6, m)
+        z = (g.standard_normal((m, m)) + 1j * g.standard_normal((m, m))) / np.sqrt(2.0)
+        U, rr = np.linalg.qr(z)
+        ph = np.diagonal(rr).copy()
+        U = U * (ph / np.abs(ph))
+        S = np.block([[U.real, -U.imag], [U.imag, U.real]]) @ np.diag(np.concatenate([np.exp(-r), np.exp(r)]))
+        cov = S @ S.T
+        sample_resident(cov, n, cutoff=c, seed=0)
+        ts = []
+        for rep in range(7):
+            t0 = time.perf_counter()
+            s = sample_resident(cov, n, cutoff=c, seed=rep)
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        rows.append({"modes": m, "cutoff": c, "num_samples": n, "samples_per_s": n / t, "median_ms": t * 1e3,
+                     "mean_photons": float(s.sum(axis=1).mean()),
+                     "reference_4090": {"path": path, "samples_per_s": rate}})
+    return {"workload": "bench/sampler_throughput.py states, resident chain (sieve conditionals, Philox draws)",
+            "rows": rows}
 
 
 def haf_kernels(L, torch, dev, stream):

@@ -82,6 +82,8 @@ SIGNATURES = {
     "gbs_tor_recursive_single_host": ([_cp, _i, _i, _cp], _i),
     "gbs_tor_single_certified_host": ([_cp, _i, _i, _cp, _cp], _i),
     "gbs_tor_single_ddcertified_host": ([_cp, _i, _i, _cp, _cp], _i),
+    "gbs_sampler_sample": ([_cp, _i, _cp, _i, _i, _i, _i, _cp, ctypes.c_ulonglong, _cp], _i),
+    "gbs_philox4x32_10": ([_cp, _cp, _cp], None),
     "gbs_lhaf_repeated_cert_host": ([_cp, _cp, _i, _cp, _i, _cp, _cp], _i),
 }
 

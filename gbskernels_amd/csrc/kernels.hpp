@@ -51,5 +51,12 @@ size_t sieve_partial_bytes(bool cert, int64_t parts);
 int sieve_launch(bool cert, const cplx* d_A, const cplx* d_g, int M, const int* d_reps, int B,
                  const std::vector<RepPat>& plan, const RepPat* d_plan, int64_t parts, void* d_part, cplx* d_out,
                  double* d_bound, hipStream_t st);
+// the same from a device reps table, planned on the device (no host round
+// trip); mm >= every pattern's nonzero-count m; buffers: B RepPat, one int64,
+// sieve_device_partial_bytes(cert, B)
+size_t sieve_device_partial_bytes(bool cert, int B);
+int sieve_launch_device(bool cert, const cplx* d_A, const cplx* d_g, int M, const int* d_reps, int B, int mm,
+                        RepPat* d_plan, int64_t* d_total, void* d_part, cplx* d_out, double* d_bound,
+                        hipStream_t st);
 
 }  // namespace gbsk

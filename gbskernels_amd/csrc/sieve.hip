@@ -84,7 +84,7 @@ static size_t sieve_lds_bytes(int mm, bool cert) {
 }
 
 template <bool CERT>
-__global__ __launch_bounds__(64) void sieve_chunk_kernel(const SieveArgs a) {
+__device__ __forceinline__ void sieve_chunk_body(const SieveArgs& a, const int p, const int chunk) {
   extern __shared__ __align__(16) unsigned char dyn[];
   __shared__ cplx gs[REP_MAX_M];
   __shared__ double gN[REP_MAX_M];
@@ -99,16 +99,7 @@ __global__ __launch_bounds__(64) void sieve_chunk_kernel(const SieveArgs a) {
   unsigned char* dig = dyn + (size_t)P2 * (sizeof(cplx) + (CERT ? sizeof(double) : 0));
 
   const int lane = threadIdx.x;
-  const int64_t wg = a.wg0 + blockIdx.x;
-  int lo = a.p0, hi = a.p1 - 1;  // last pattern whose first workgroup is <= wg
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (a.pats[mid].wg_begin <= wg) lo = mid;
-    else hi = mid - 1;
-  }
-  const int p = lo;
   const RepPat P = a.pats[p];
-  const int chunk = (int)(wg - P.wg_begin);
   const int M = a.M, m = P.m, q = P.q, TI = P.TI, N = P.N, K = N / 2;
 
   // nonzero counts in ascending index order
@@ -306,6 +297,75 @@ __global__ __launch_bounds__(64) void sieve_chunk_kernel(const SieveArgs a) {
   }
 }
 
+// the last pattern of [lo, hi] whose first workgroup is <= wg
+__device__ inline int sieve_find_pattern(const RepPat* pats, int lo, int hi, int64_t wg) {
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pats[mid].wg_begin <= wg) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// host-planned launch: workgroup blockIdx.x of patterns [p0, p1)
+template <bool CERT>
+__global__ __launch_bounds__(64) void sieve_chunk_kernel(const SieveArgs a) {
+  const int64_t wg = a.wg0 + blockIdx.x;
+  const int p = sieve_find_pattern(a.pats, a.p0, a.p1 - 1, wg);
+  sieve_chunk_body<CERT>(a, p, (int)(wg - a.pats[p].wg_begin));
+}
+
+// device-planned launch: a fixed grid walks the *total work items the plan
+// kernels counted (every workgroup exits once its index passes the total)
+template <bool CERT>
+__global__ __launch_bounds__(64) void sieve_chunk_loop_kernel(const SieveArgs a, const int64_t* __restrict__ total) {
+  const int64_t tot = *total;
+  for (int64_t wg = blockIdx.x; wg < tot; wg += gridDim.x) {
+    const int p = sieve_find_pattern(a.pats, a.p0, a.p1 - 1, wg);
+    sieve_chunk_body<CERT>(a, p, (int)(wg - a.pats[p].wg_begin));
+    __syncthreads();
+  }
+}
+
+// device plan: one thread per pattern; partial slots at a fixed stride chunk_cap
+__global__ void sieve_plan_kernel(const int* __restrict__ reps, int M, int B, int chunk_cap, RepPat* __restrict__ pats) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  RepPat P;
+  sieve_plan_one(reps + (size_t)b * M, M, chunk_cap, P);
+  P.part = (int64_t)b * chunk_cap;
+  P.wg_begin = 0;
+  pats[b] = P;
+}
+
+// exclusive scan of the patterns' workgroup counts (one workgroup of 1024
+// threads, contiguous segments per thread) and their total
+__global__ __launch_bounds__(1024) void sieve_scan_kernel(RepPat* __restrict__ pats, int B, int64_t* __restrict__ total) {
+  __shared__ int64_t seg[1024];
+  const int t = threadIdx.x;
+  const int per = (B + 1023) / 1024;
+  const int b0 = t * per, b1 = min(B, b0 + per);
+  int64_t s = 0;
+  for (int b = b0; b < b1; ++b) s += pats[b].chunks;
+  seg[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int64_t run = 0;
+    for (int i = 0; i < 1024; ++i) {
+      const int64_t v = seg[i];
+      seg[i] = run;
+      run += v;
+    }
+    *total = run;
+  }
+  __syncthreads();
+  int64_t run = seg[t];
+  for (int b = b0; b < b1; ++b) {
+    pats[b].wg_begin = run;
+    run += pats[b].chunks;
+  }
+}
+
 // one wave per pattern: chunk partials in order (lane-strided, then a fixed
 // butterfly), then the division by N! 2^N
 template <bool CERT>
@@ -349,59 +409,15 @@ __global__ __launch_bounds__(256) void sieve_reduce_kernel(const RepPat* __restr
 
 int sieve_plan(int M, const int* reps, int B, std::vector<RepPat>& plan, int64_t* parts) {
   plan.assign(B > 0 ? (size_t)B : 0, RepPat{});
-  int64_t wg = 0, pt = 0;
+  int64_t wg = 0;
   for (int b = 0; b < B; ++b) {
     RepPat& P = plan[b];
+    sieve_plan_one(reps + (size_t)b * M, M, REP_CHUNKS_MAX, P);
     P.wg_begin = wg;
-    P.part = pt;
-    P.status = REP_EVAL;
-    const int* n = reps + (size_t)b * M;
-    int N = 0, m = 0;
-    double Td = 1.0;
-    bool bad = M > REP_MAX_M;
-    for (int i = 0; i < M && !bad; ++i) {
-      if (n[i] < 0 || n[i] > REP_MAX_N) bad = true;
-      N += n[i];
-      m += n[i] > 0;
-      Td *= (double)(n[i] + 1);
-    }
-    if (bad || N > REP_MAX_N || Td > REP_MAX_TERMS) {
-      P.status = REP_REFUSE;
-      continue;
-    }
-    if (N == 0) {
-      P.status = REP_ONE;
-      continue;
-    }
-    int nzc[REP_MAX_M], c = 0;
-    for (int i = 0; i < M; ++i)
-      if (n[i] > 0) nzc[c++] = n[i];
-    const int64_t T = (int64_t)Td, Th = (T + 1) / 2;
-    int q = 0, TI = 1;
-    while (q < m) {
-      const int64_t nxt = (int64_t)TI * (nzc[q] + 1);
-      if (nxt > REP_TI_MAX || (Th + nxt - 1) / nxt < 64) break;
-      TI = (int)nxt;
-      ++q;
-    }
-    const int64_t TOh = (Th + TI - 1) / TI;
-    int64_t J = std::min<int64_t>((TOh + 63) / 64, std::max(1, 256 / TI));
-    J = std::max<int64_t>(J, 1);
-    J = std::max<int64_t>(J, (TOh + 64LL * REP_CHUNKS_MAX - 1) / (64LL * REP_CHUNKS_MAX));
-    const int64_t chunks = (TOh + 64 * J - 1) / (64 * J);
-    P.T = T;
-    P.TOh = TOh;
-    P.m = m;
-    P.N = N;
-    P.q = q;
-    P.TI = TI;
-    P.J = (int)J;
-    P.chunks = (int)chunks;
-    P.depth = 4.0 * m + 8.0 * (N / 2) + 256.0 + 4.0 * (double)J * TI + 4.0 * (double)chunks;
-    wg += chunks;
-    pt += chunks;
+    P.part = wg;
+    wg += P.chunks;
   }
-  *parts = pt;
+  *parts = wg;
   return 0;
 }
 
@@ -455,6 +471,50 @@ int sieve_launch(bool cert, const cplx* d_A, const cplx* d_g, int M, const int* 
                        d_bound);
   else
     hipLaunchKernelGGL(sieve_reduce_kernel<false>, dim3(grid), dim3(256), 0, st, d_plan, B, a.part, a.part_e, d_out,
+                       d_bound);
+  return (int)hipGetLastError();
+}
+
+// Device-planned sieve over a device reps table (B x M): plan, scan and a
+// fixed grid of chunk workgroups, then the reduction; no host round trip.
+// d_plan: B RepPat; d_total: one int64; d_part: B * REP_DEV_CHUNKS partials
+// (plus as many doubles for the certified tier); mm >= every pattern's m.
+size_t sieve_device_partial_bytes(bool cert, int B) {
+  return (size_t)B * REP_DEV_CHUNKS * (sizeof(cplx) + (cert ? sizeof(double) : 0));
+}
+
+int sieve_launch_device(bool cert, const cplx* d_A, const cplx* d_g, int M, const int* d_reps, int B, int mm,
+                        RepPat* d_plan, int64_t* d_total, void* d_part, cplx* d_out, double* d_bound,
+                        hipStream_t st) {
+  if (B <= 0) return 0;
+  if (M > REP_MAX_M || mm < 1 || mm > REP_MAX_M) return GBSK_ERR_DIM;
+  if (cert && !d_bound) return GBSK_ERR_ARG;
+  hipLaunchKernelGGL(sieve_plan_kernel, dim3((B + 255) / 256), dim3(256), 0, st, d_reps, M, B, REP_DEV_CHUNKS, d_plan);
+  hipLaunchKernelGGL(sieve_scan_kernel, dim3(1), dim3(1024), 0, st, d_plan, B, d_total);
+  SieveArgs a{};
+  a.A = d_A;
+  a.g = d_g;
+  a.reps = d_reps;
+  a.pats = d_plan;
+  a.part = reinterpret_cast<cplx*>(d_part);
+  a.part_e = reinterpret_cast<double*>(reinterpret_cast<char*>(d_part) + (size_t)B * REP_DEV_CHUNKS * sizeof(cplx));
+  a.M = M;
+  a.p0 = 0;
+  a.p1 = B;
+  a.wg0 = 0;
+  a.mm = mm;
+  const size_t lds = sieve_lds_bytes(mm, cert);
+  const unsigned grid = 4096;  // 16 workgroups per CU; each loops over the counted work
+  if (cert)
+    hipLaunchKernelGGL(sieve_chunk_loop_kernel<true>, dim3(grid), dim3(64), lds, st, a, (const int64_t*)d_total);
+  else
+    hipLaunchKernelGGL(sieve_chunk_loop_kernel<false>, dim3(grid), dim3(64), lds, st, a, (const int64_t*)d_total);
+  const unsigned rgrid = (unsigned)(((int64_t)B + 3) / 4);
+  if (cert)
+    hipLaunchKernelGGL(sieve_reduce_kernel<true>, dim3(rgrid), dim3(256), 0, st, d_plan, B, a.part, a.part_e, d_out,
+                       d_bound);
+  else
+    hipLaunchKernelGGL(sieve_reduce_kernel<false>, dim3(rgrid), dim3(256), 0, st, d_plan, B, a.part, a.part_e, d_out,
                        d_bound);
   return (int)hipGetLastError();
 }

@@ -160,6 +160,22 @@ def tor_single_ddcertified(O, g):
     return _tor_single_cert("tor_single_ddcertified", "gbs_tor_single_ddcertified_host", O, g)
 
 
+def sample_resident(ak, off, M, num_draws, cutoff, maxn, invfac, seed):
+    """Resident chain-rule GBS sampler (bindings/gbskernels_ext.cpp:153-167):
+    ``ak`` the M reduced A-matrices concatenated (complex128), ``off`` their
+    int32 offsets, ``invfac[j] = 1/j!``; returns (num_draws, M) int32 samples."""
+    a = np.ascontiguousarray(np.asarray(ak, dtype=np.complex128).reshape(-1))
+    o = np.ascontiguousarray(np.asarray(off, dtype=np.int32).reshape(-1))
+    f = np.ascontiguousarray(np.asarray(invfac, dtype=np.float64).reshape(-1))
+    M, D, C = int(M), int(num_draws), int(cutoff)
+    if o.shape[0] < M or f.shape[0] < C + 1:
+        raise ValueError("sample_resident: off needs M entries and invfac cutoff + 1")
+    out = np.zeros((D, M), np.int32)
+    _lib.check(_lib.lib().gbs_sampler_sample(_lib.ptr(a), a.shape[0], _lib.ptr(o), M, D, C, int(maxn), _lib.ptr(f),
+                                             int(seed) & 0xFFFFFFFFFFFFFFFF, _lib.ptr(out)), "sample_resident")
+    return out
+
+
 # ---- DLPack export of device results (the reference returns an nb::ndarray
 # that exposes __dlpack__, bindings/gbskernels_ext.cpp:209-228) ----------------
 class _DLDevice(ctypes.Structure):

@@ -214,6 +214,22 @@ int gbs_tor_single_certified_host(const double* O, int n, int g, double* out, do
 /* core/host_api.cu:350-404 (bindings/gbskernels_ext.cpp:285-295) */
 int gbs_tor_single_ddcertified_host(const double* O, int n, int g, double* out, double* bound);
 
+/* ---- 9. resident chain-rule GBS sampler -------------------------------------
+ * Ak: the M reduced A-matrices A_1..A_M concatenated (A_k is 2k x 2k row-major,
+ * starting at Ak + off[k - 1]); invfac[j] = 1 / j!, j = 0..cutoff. Draws
+ * num_draws photon patterns mode by mode, P(j | prefix) proportional to
+ * max(Re haf(A_k[prefix, j]), 0) / j! over j <= cutoff, with every conditional
+ * hafnian evaluated on the device by the repeated-row sieve and uniforms from
+ * Philox4x32-10 at counter (draw, mode) under key = seed. out: num_draws x M
+ * int32, row-major. M <= 16; maxn is accepted for compatibility (unused).
+ * Returns GBSK_ERR_DIM if a conditional fell outside the sieve's domain. */
+/* core/sampler_session.cu:96-128 (bindings/gbskernels_ext.cpp:153-167) */
+int gbs_sampler_sample(const gbs_complex* Ak, int ak_total, const int* off, int M, int num_draws, int cutoff,
+                       int maxn, const double* invfac, unsigned long long seed, int* out);
+/* the generator the sampler draws from (host evaluation; no reference
+ * counterpart): out = Philox4x32-10(ctr[4], key[2]) */
+void gbs_philox4x32_10(const unsigned* ctr, const unsigned* key, unsigned* out);
+
 #ifdef __cplusplus
 }
 #endif

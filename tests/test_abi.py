@@ -12,14 +12,15 @@ ROOT = Path(__file__).resolve().parent.parent
 
 def header_symbols():
     text = (ROOT / "include" / "gbsk.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|size_t|long long|const char\*)\s+(gbs_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|size_t|long long|const char\*)\s+(gbs_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_the_binding_surface():
     syms = header_symbols()
-    # the reference binding's extern "C" set (bindings/gbskernels_ext.cpp:24-68), minus
-    # the out-of-scope single-large / repeated / sampler entry points
-    for name in ["gbs_perm_host", "gbs_perm_dd_host", "gbs_haf_host", "gbs_haf_dd_host", "gbs_lhaf_host",
+    # the reference binding's extern "C" set (bindings/gbskernels_ext.cpp:24-68)
+    for name in ["gbs_tor_recursive_single_host", "gbs_tor_single_certified_host",
+                 "gbs_tor_single_ddcertified_host", "gbs_lhaf_repeated_host", "gbs_lhaf_repeated_cert_host",
+                 "gbs_sampler_sample", "gbs_perm_host", "gbs_perm_dd_host", "gbs_haf_host", "gbs_haf_dd_host", "gbs_lhaf_host",
                  "gbs_lhaf_dd_host", "gbs_tor_host", "gbs_tor_dd_host", "gbs_perm_kappa_host",
                  "gbs_haf_kappa_host", "gbs_lhaf_kappa_host", "gbs_tor_kappa_host", "gbs_perm_certified_host",
                  "gbs_haf_certified_host", "gbs_lhaf_certified_host", "gbs_tor_certified_host",
@@ -59,7 +60,10 @@ def test_ext_module_surface():
     assert ext.__host_shim__ is False
     for name in ["perm", "perm_dd", "haf", "haf_dd", "lhaf", "lhaf_dd", "tor", "tor_dd", "perm_kappa",
                  "haf_kappa", "lhaf_kappa", "tor_kappa", "perm_certified", "haf_certified", "lhaf_certified",
-                 "tor_certified", "perm_dd_certified", "haf_dd_certified", "Session"]:
+                 "tor_certified", "perm_dd_certified", "haf_dd_certified", "Session",
+                 # optional attributes the reference probes with hasattr
+                 "lhaf_dd_certified", "tor_dd_certified", "lhaf_repeated", "lhaf_repeated_certified",
+                 "tor_single", "tor_single_certified", "tor_single_ddcertified", "sample_resident"]:
         assert hasattr(ext, name), name
     for m in ["perm", "haf", "lhaf", "tor", "perm_dd", "haf_dd", "lhaf_dd", "tor_dd", "perm_resident",
               "haf_resident", "lhaf_resident", "tor_resident", "close", "reallocs"]:
