@@ -184,14 +184,14 @@ static int plan(int n, const int* func, const int* tier, const int* dim, std::ve
     std::stable_sort(out.begin(), out.end(), [](const Bucket& a, const Bucket& c) { return a.cost > c.cost; });
   else
     std::stable_sort(out.begin(), out.end(), [](const Bucket& a, const Bucket& c) { return a.cost < c.cost; });
-  // group hafnian buckets: same function and tier, sizes below / from 14
-  // (small and large ones complete separately), at most HAF_MAX_JOBS each
+  // group hafnian buckets: same function and tier, at most HAF_MAX_JOBS each
+  // (one launch per subset size for the whole group)
   {
     std::map<long long, int> open;
     for (int k = 0; k < (int)out.size(); ++k) {
       Bucket& bk = out[k];
       if (bk.func != GBSK_HAF && bk.func != GBSK_LHAF) continue;
-      const long long key = ((long long)bk.func << 8) | ((long long)bk.tier << 1) | (bk.dim >= 14 ? 1 : 0);
+      const long long key = ((long long)bk.func << 8) | (long long)bk.tier;
       auto it = open.find(key);
       if (it != open.end() && (int)out[it->second].members.size() < HAF_MAX_JOBS) {
         bk.leader = it->second;
@@ -220,9 +220,15 @@ static int plan(int n, const int* func, const int* tier, const int* dim, std::ve
       load[k] += cost;
     }
     bk.in_off = ie;
-    bk.res_off = re;
     ie += bk.items.size() * (size_t)bk.dim * bk.dim;
-    re += (bk.items.size() * (sizeof(cplx) + sizeof(double)) + 15) & ~(size_t)15;
+    // results: a group's members back to back (one copy brings the group home)
+    if (bk.leader < 0 || bk.leader == kb) {
+      const std::vector<int> one(1, kb);
+      for (int m : (bk.leader == kb ? bk.members : one)) {
+        out[m].res_off = re;
+        re += (out[m].items.size() * (sizeof(cplx) + sizeof(double)) + 15) & ~(size_t)15;
+      }
+    }
     bk.ws_bytes = gbs_workspace_bytes(bk.func, bk.tier, bk.dim, (int)bk.items.size());
     bk.ws_off = ws;
     ws += (bk.ws_bytes + 255) & ~(size_t)255;
@@ -274,16 +280,16 @@ static int enqueue_device(gbs_batch* b, const std::vector<Bucket>& bks, size_t i
                            (char*)b->d_ws + bk.ws_off, bk.ws_bytes, (void*)s);
     }
     if (!rc && b->per_bucket) {
-      // each bucket's values and auxiliaries come back in one copy on the
-      // launching stream, then its event fires (a group's buckets together)
+      // a bucket's (or a whole group's, laid out back to back) values and
+      // auxiliaries come back in one copy on the launching stream, then the
+      // events of its buckets fire
       const std::vector<int> one(1, (int)k);
-      for (int m : (bk.leader == (int)k ? bk.members : one)) {
-        const Bucket& x = bks[m];
-        if (!rc)
-          rc = (int)hipMemcpyAsync(b->h_res + x.res_off, res_of(x), x.items.size() * (sizeof(cplx) + sizeof(double)),
-                                   hipMemcpyDeviceToHost, s);
+      const std::vector<int>& mem = bk.leader == (int)k ? bk.members : one;
+      const Bucket& last = bks[mem.back()];
+      const size_t beg = bk.res_off, end = last.res_off + last.items.size() * (sizeof(cplx) + sizeof(double));
+      rc = (int)hipMemcpyAsync(b->h_res + beg, (char*)b->d_res + beg, end - beg, hipMemcpyDeviceToHost, s);
+      for (int m : mem)
         if (!rc) rc = (int)hipEventRecord(b->bucket_ev[m], s);
-      }
     }
   }
   if (b->per_bucket) return rc;
