@@ -38,6 +38,7 @@ struct RF64 {
   static constexpr bool CERT = false;
   // real SPD: 1/sqrt(det) = product of the reciprocal Cholesky diagonals
   static constexpr bool ISD = true;
+  static constexpr bool SYM = true;  // symmetric input: packed Cholesky (general LU otherwise)
   __device__ static T zero() { return 0.0; }
   __device__ static T one() { return 1.0; }
   __device__ static T from(double x) { return x; }
@@ -64,6 +65,7 @@ struct RDD {
   static constexpr int RMAX = 3;
   static constexpr bool CERT = false;
   static constexpr bool ISD = true;
+  static constexpr bool SYM = true;
   __device__ static T zero() { return dd{0.0, 0.0}; }
   __device__ static T one() { return dd{1.0, 0.0}; }
   __device__ static T from(double x) { return dd{x, 0.0}; }
@@ -93,9 +95,12 @@ struct RDD {
   }
 };
 
-// complex symmetric O (no conjugation): L L^T factorisation without pivoting,
-// principal square root of each subset determinant (the np.sqrt convention of
-// cpu_ref/torontonian.py:60); a zero pivot is a breakdown (NaN)
+// any complex O (complex symmetric, Hermitian as in the complex-basis
+// O = I - Q^-1, or general): LU without pivoting (the middle tree's Schur
+// complements use both off-diagonal pivot entries, the per-lane walk a prefix
+// LU of the full task matrix), principal square root of each subset
+// determinant (the np.sqrt convention of cpu_ref/torontonian.py:60); a zero
+// pivot is a breakdown (NaN)
 __device__ inline cplx c_div(cplx a, cplx b) {  // Smith's algorithm
   if (fabs(b.re) >= fabs(b.im)) {
     const double r = b.im / b.re, den = fma(b.im, r, b.re);
@@ -118,6 +123,8 @@ struct RC64 {
   // complex: the principal sqrt of the subset determinant (not the product of
   // the pivots' principal roots, which can differ in sign)
   static constexpr bool ISD = false;
+  // any complex O (complex symmetric, Hermitian, general): LU, no symmetry assumed
+  static constexpr bool SYM = false;
   __device__ static T zero() { return cplx{0.0, 0.0}; }
   __device__ static T one() { return cplx{1.0, 0.0}; }
   __device__ static T from(double x) { return cplx{x, x}; }
@@ -163,6 +170,7 @@ struct RCERT {
   static constexpr int RMAX = RF64::RMAX;  // the fp64 tier's split, for identical values
   static constexpr bool CERT = true;
   static constexpr bool ISD = true;
+  static constexpr bool SYM = true;
   __device__ static T zero() { return cv{0.0, 0.0}; }
   __device__ static T one() { return cv{1.0, 0.0}; }
   __device__ static T one_minus(cplx o) {
@@ -243,6 +251,7 @@ struct RCDD {
   static constexpr int RMAX = RDD::RMAX;  // the dd tier's split, for identical values
   static constexpr bool CERT = true;
   static constexpr bool ISD = true;
+  static constexpr bool SYM = true;
   __device__ static T zero() { return cdd{dd{0.0, 0.0}, 0.0}; }
   __device__ static T one() { return cdd{dd{1.0, 0.0}, 0.0}; }
   __device__ static T one_minus(cplx o) { return cdd{two_sum(1.0, -o.re), 0.0}; }  // exact
@@ -398,16 +407,79 @@ struct Bottom<Ar, R, R, MASK, C> {
   }
 };
 
+// Non-symmetric (complex) policy: the same walk with a prefix LU (Doolittle,
+// unit L) of the full task matrix: appending index r gives U[c][r], L[r][c]
+// (c < r) and the pivot U[r][r]; the determinant is the product of pivots.
+template <class Ar, int R, int D, unsigned MASK, int C>
+struct BottomLU {
+  using T = typename Ar::T;
+  static __device__ __forceinline__ void run(const T* __restrict__ M, T (&L)[2 * R][2 * R], T (&U)[2 * R][2 * R],
+                                             T det, int count, int nm, bool& bad, T& acc, double& aacc) {
+    BottomLU<Ar, R, D + 1, MASK, C>::run(M, L, U, det, count, nm, bad, acc, aacc);
+    T piv = Ar::one();
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int r = 2 * C + t;
+      const int gr = 2 * D + t;
+#pragma unroll
+      for (int c = 0; c < r; ++c) {
+        int pos = c >> 1, mode = D, k = 0;
+#pragma unroll
+        for (int j = 0; j < D; ++j)
+          if ((MASK >> j) & 1u) {
+            if (k == pos) mode = j;
+            ++k;
+          }
+        const int gc = 2 * mode + (c & 1);
+        T u = M[gc * (2 * R) + gr], l = M[gr * (2 * R) + gc];
+#pragma unroll
+        for (int w = 0; w < c; ++w) {
+          u = Ar::fms(L[c][w], U[w][r], u);
+          l = Ar::fms(L[r][w], U[w][c], l);
+        }
+        U[c][r] = u;
+        L[r][c] = Ar::div(l, U[c][c]);
+      }
+      T p = M[gr * (2 * R) + gr];
+#pragma unroll
+      for (int w = 0; w < r; ++w) p = Ar::fms(L[r][w], U[w][r], p);
+      bad |= !Ar::pos(p);
+      U[r][r] = p;
+      piv = Ar::mul(piv, p);
+    }
+    BottomLU<Ar, R, D + 1, MASK | (1u << D), C + 1>::run(M, L, U, Ar::mul(det, piv), count + 1, nm, bad, acc, aacc);
+  }
+};
+
+template <class Ar, int R, unsigned MASK, int C>
+struct BottomLU<Ar, R, R, MASK, C> {
+  using T = typename Ar::T;
+  static __device__ __forceinline__ void run(const T* __restrict__, T (&)[2 * R][2 * R], T (&)[2 * R][2 * R], T det,
+                                             int count, int nm, bool&, T& acc, double& aacc) {
+    tor_leaf<Ar>(det, count, nm, acc, aacc);
+  }
+};
+
+// task slot entries: packed lower triangle (symmetric policies) or full matrix
+template <class Ar, int R>
+__host__ __device__ constexpr int tor_slot() {
+  return Ar::SYM ? (2 * R) * (2 * R + 1) / 2 : (2 * R) * (2 * R);
+}
+
 template <class Ar, int R>
 __device__ inline void bottom_walk(const typename Ar::T* M, typename Ar::T det, int count, int nm, bool& bad,
                                    typename Ar::T& acc, double& aacc) {
-  typename Ar::T L[2 * R > 0 ? 2 * R : 1][2 * R > 0 ? 2 * R : 1];
+  constexpr int W = 2 * R > 0 ? 2 * R : 1;
   // ISD: the walk carries 1/sqrt of the determinant (upstream pivots here)
   const typename Ar::T d0 = Ar::ISD ? Ar::recip(Ar::sqrt_(det)) : det;
   if constexpr (R == 0) {
     tor_leaf<Ar>(d0, count, nm, acc, aacc);
-  } else {
+  } else if constexpr (Ar::SYM) {
+    typename Ar::T L[W][W];
     Bottom<Ar, R, 0, 0u, 0>::run(M, L, d0, count, nm, bad, acc, aacc);
+  } else {
+    typename Ar::T L[W][W], U[W][W];
+    BottomLU<Ar, R, 0, 0u, 0>::run(M, L, U, d0, count, nm, bad, acc, aacc);
   }
 }
 
@@ -468,7 +540,7 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
     const int Dd = 2 * (nm - T1 - d);
     levtot += Dd * Dd;
   }
-  constexpr int SLOT = (2 * R) * (2 * R + 1) / 2;
+  constexpr int SLOT = tor_slot<Ar, R>();
   T* W = lev + levtot;
   T* slots = W;
   T* slot_det = slots + 64 * SLOT;
@@ -530,9 +602,15 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
     if (d == T2) {
       // leaf: the task is this level's matrix (the trailing R modes)
       for (int e = lane; e < SLOT; e += 64) {
-        int r = 0;
-        while (tri(r + 1, 0) <= e) ++r;
-        const int c = e - tri(r, 0);
+        int r, c;
+        if constexpr (Ar::SYM) {  // packed lower triangle
+          r = 0;
+          while (tri(r + 1, 0) <= e) ++r;
+          c = e - tri(r, 0);
+        } else {  // full matrix
+          r = e / (2 * R);
+          c = e % (2 * R);
+        }
         slots[ntask * SLOT + e] = X[r * Dd + c];
       }
       if (lane == 0) {
@@ -560,8 +638,9 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
       wstate[d] = 2;
       const T a = X[0];
       const T b01 = X[1];
+      const T b10 = Ar::SYM ? b01 : X[Dd];  // [[a, b01], [b10, c']]
       const T ra = Ar::recip(a);
-      const T c = Ar::sub(X[Dd + 1], Ar::mul(Ar::mul(b01, b01), ra));
+      const T c = Ar::sub(X[Dd + 1], Ar::mul(Ar::mul(b10, b01), ra));
       const T rc = Ar::recip(c);
       const int Dy = Dd - 2;
       for (int e = lane; e < Dy * Dy; e += 64) {
@@ -569,7 +648,7 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
         const T li0 = Ar::mul(X[i * Dd], ra);
         const T li1 = Ar::mul(Ar::fms(li0, b01, X[i * Dd + 1]), rc);
         const T x0j = X[j];
-        const T x1j = Ar::fms(Ar::mul(x0j, ra), b01, X[Dd + j]);
+        const T x1j = Ar::fms(Ar::mul(x0j, ra), b10, X[Dd + j]);
         T v = Ar::fms(li0, x0j, X[i * Dd + j]);
         v = Ar::fms(li1, x1j, v);
         Y[e] = v;
@@ -765,7 +844,7 @@ static size_t tor_lds_bytes(int nm, const TorSplit& sp) {
     const size_t Dd = 2 * (nm - sp.T1 - d);
     lev += Dd * Dd;
   }
-  const size_t slot = (size_t)(2 * sp.R) * (2 * sp.R + 1) / 2;
+  const size_t slot = Ar::SYM ? (size_t)(2 * sp.R) * (2 * sp.R + 1) / 2 : (size_t)(2 * sp.R) * (2 * sp.R);
   const size_t T = sizeof(typename Ar::T);
   const size_t task_area = 64 * slot * T + 64 * T + 64 * 4;
   const size_t work = (size_t)dim * dim * T;
@@ -820,7 +899,7 @@ static int launch_tor(int nm, const TorSplit& sp, const cplx* d_in, int B, typen
 }
 
 // fp64 tier: real-domain kernel; workgroups that found a complex entry or a
-// non-positive pivot are recomputed by the complex-symmetric kernel, whose split
+// non-positive pivot are recomputed by the complex (LU) kernel, whose split
 // is chosen so that both kernels partition the instance identically (same T1).
 // rounding depth of the certified bound: per mode a Schur level (<= 16 bound
 // operations), per lane the 2^R leaf sums, the butterfly and the np-term reduce

@@ -91,6 +91,33 @@ def test_tor_complex_symmetric_vs_oracle(ext, oracle, nm):
     assert rel_err(a, absn).max() < 1e-10
 
 
+@pytest.mark.parametrize("nm", [1, 2, 4, 6, 8])
+def test_tor_hermitian_and_general_complex_vs_oracle(ext, oracle, nm):
+    """Complex O that is not symmetric -- the complex-basis O = I - Q^-1 of a
+    physical state (Hermitian) and a general perturbation of it -- takes the LU
+    fallback and agrees with the oracle's partial-pivot LU (principal branch);
+    the Hermitian one's value is the real-basis torontonian's."""
+    from gbskernels_amd.gbs import threshold_O_xxpp
+    from gbskernels_amd.sampling import qmat
+    print(f"tor hermitian nm={nm}", flush=True)
+    g = np.random.default_rng(900 + nm)
+    r = g.uniform(0.2, 0.6, nm)
+    z = (g.standard_normal((nm, nm)) + 1j * g.standard_normal((nm, nm))) / np.sqrt(2.0)
+    U, rr = np.linalg.qr(z)
+    U = U * (np.diagonal(rr) / np.abs(np.diagonal(rr)))
+    S = np.block([[U.real, -U.imag], [U.imag, U.real]]) @ np.diag(np.concatenate([np.exp(-r), np.exp(r)]))
+    cov = 0.7 * (S @ S.T) + 0.3 * np.eye(2 * nm)      # lossy: a mixed state
+    O = np.eye(2 * nm) - np.linalg.inv(qmat(cov))
+    assert not np.allclose(O, O.T)
+    Y = O + 0.02 * (g.standard_normal(O.shape) + 1j * g.standard_normal(O.shape))
+    stack = np.stack([O, Y])
+    got = ext.tor(stack)
+    want, absn = oracle.tor(stack, want_abs=True)
+    assert np.all(np.abs(got - want) <= 2 * tor_bound(absn, nm))
+    Ox, _ = threshold_O_xxpp(cov)
+    assert abs(got[0] - ext.tor_single(Ox, 0)) <= 4 * tor_bound(absn[:1], nm)[0]
+
+
 @pytest.mark.parametrize("nm", [3, 8, 12])
 def test_tor_kappa_value_bitwise_and_abs(ext, oracle, nm):
     O = physical_stack(nm, 8, 300 + nm)
