@@ -39,6 +39,9 @@ struct RF64 {
   // real SPD: 1/sqrt(det) = product of the reciprocal Cholesky diagonals
   static constexpr bool ISD = true;
   static constexpr bool SYM = true;  // symmetric input: packed Cholesky (general LU otherwise)
+  // each lane keeps its task's matrix in VGPRs (no LDS task slots): a smaller
+  // LDS footprint and more workgroups per CU
+  static constexpr bool TASK_REGS = true;
   __device__ static T zero() { return 0.0; }
   __device__ static T one() { return 1.0; }
   __device__ static T from(double x) { return x; }
@@ -66,6 +69,7 @@ struct RDD {
   static constexpr bool CERT = false;
   static constexpr bool ISD = true;
   static constexpr bool SYM = true;
+  static constexpr bool TASK_REGS = false;
   __device__ static T zero() { return dd{0.0, 0.0}; }
   __device__ static T one() { return dd{1.0, 0.0}; }
   __device__ static T from(double x) { return dd{x, 0.0}; }
@@ -125,6 +129,7 @@ struct RC64 {
   static constexpr bool ISD = false;
   // any complex O (complex symmetric, Hermitian, general): LU, no symmetry assumed
   static constexpr bool SYM = false;
+  static constexpr bool TASK_REGS = false;
   __device__ static T zero() { return cplx{0.0, 0.0}; }
   __device__ static T one() { return cplx{1.0, 0.0}; }
   __device__ static T from(double x) { return cplx{x, x}; }
@@ -171,6 +176,7 @@ struct RCERT {
   static constexpr bool CERT = true;
   static constexpr bool ISD = true;
   static constexpr bool SYM = true;
+  static constexpr bool TASK_REGS = false;
   __device__ static T zero() { return cv{0.0, 0.0}; }
   __device__ static T one() { return cv{1.0, 0.0}; }
   __device__ static T one_minus(cplx o) {
@@ -252,6 +258,7 @@ struct RCDD {
   static constexpr bool CERT = true;
   static constexpr bool ISD = true;
   static constexpr bool SYM = true;
+  static constexpr bool TASK_REGS = false;
   __device__ static T zero() { return cdd{dd{0.0, 0.0}, 0.0}; }
   __device__ static T one() { return cdd{dd{1.0, 0.0}, 0.0}; }
   __device__ static T one_minus(cplx o) { return cdd{two_sum(1.0, -o.re), 0.0}; }  // exact
@@ -339,8 +346,14 @@ inline TorSplit tor_single_split(int nm, int rmax) {
   return s;
 }
 
-// packed lower-triangular index
+// packed lower-triangular index, and its inverse
 __device__ constexpr int tri(int r, int c) { return r * (r + 1) / 2 + c; }
+__device__ constexpr int tri_row(int e) {
+  int r = 0;
+  while ((r + 1) * (r + 2) / 2 <= e) ++r;
+  return r;
+}
+__device__ constexpr int tri_col(int e) { return e - tri(tri_row(e), 0); }
 
 // ----------------------------------------------------------- per-lane bottom
 // Prefix-Cholesky walk over the R trailing modes of one task. M is the task's
@@ -541,7 +554,11 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
     levtot += Dd * Dd;
   }
   constexpr int SLOT = tor_slot<Ar, R>();
-  T* W = lev + levtot;
+  constexpr bool TREG = Ar::TASK_REGS;
+  // TREG: the work matrix and the level stack share the region from 0 (level 0
+  // is staged through registers); otherwise the work matrix, later the task
+  // slots, follow the level stack
+  T* W = TREG ? lds : lev + levtot;
   T* slots = W;
   T* slot_det = slots + 64 * SLOT;
   int* slot_cnt = reinterpret_cast<int*>(slot_det + 64);
@@ -576,9 +593,25 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
   // level 0 = W restricted to modes T1..nm-1
   {
     const int D0 = 2 * (nm - T1);
-    for (int e = lane; e < D0 * D0; e += 64) {
-      const int i = e / D0, j = e % D0;
-      lev[e] = W[(2 * T1 + i) * dim + 2 * T1 + j];
+    if constexpr (TREG) {  // in place: read everything, then write
+      constexpr int PER = (2 * (TOR_T2 + R)) * (2 * (TOR_T2 + R)) / 64 + 1;
+      T tmp[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int e = lane + 64 * k;
+        if (e < D0 * D0) tmp[k] = W[(2 * T1 + e / D0) * dim + 2 * T1 + e % D0];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int e = lane + 64 * k;
+        if (e < D0 * D0) lev[e] = tmp[k];
+      }
+    } else {
+      for (int e = lane; e < D0 * D0; e += 64) {
+        const int i = e / D0, j = e % D0;
+        lev[e] = W[(2 * T1 + i) * dim + 2 * T1 + j];
+      }
     }
     __syncthreads();
   }
@@ -596,9 +629,25 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
   woff[0] = 0;
   int ntask = 0;
   int d = 0;
+  T Mreg[TREG && SLOT > 0 ? SLOT : 1];  // TREG: this lane's task (packed lower triangle)
+  T my_det = Ar::one();
+  int my_cnt = 0;
   for (;;) {
     const int Dd = 2 * (nm - T1 - d);
     T* X = lev + woff[d];
+    if (d == T2 && TREG) {
+      // leaf: task ntask goes to lane ntask's registers
+      if (lane == ntask) {
+#pragma unroll
+        for (int e = 0; e < SLOT; ++e) Mreg[e] = X[tri_row(e) * Dd + tri_col(e)];
+        my_det = wdet[d];
+        my_cnt = wcnt[d];
+      }
+      ++ntask;
+      if (d == 0) break;
+      --d;
+      continue;
+    }
     if (d == T2) {
       // leaf: the task is this level's matrix (the trailing R modes)
       for (int e = lane; e < SLOT; e += 64) {
@@ -672,7 +721,10 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
   T acc = Ar::zero();
   double aacc = 0.0;
   if (lane < ntask) {
-    bottom_walk<Ar, R>(slots + lane * SLOT, slot_det[lane], slot_cnt[lane], nm, bad, acc, aacc);
+    if constexpr (TREG)
+      bottom_walk<Ar, R>(Mreg, my_det, my_cnt, nm, bad, acc, aacc);
+    else
+      bottom_walk<Ar, R>(slots + lane * SLOT, slot_det[lane], slot_cnt[lane], nm, bad, acc, aacc);
   }
   if (bad) bad_flag = 1;
   acc = Ar::sum64(acc);
@@ -846,6 +898,8 @@ static size_t tor_lds_bytes(int nm, const TorSplit& sp) {
   }
   const size_t slot = Ar::SYM ? (size_t)(2 * sp.R) * (2 * sp.R + 1) / 2 : (size_t)(2 * sp.R) * (2 * sp.R);
   const size_t T = sizeof(typename Ar::T);
+  // task matrices in VGPRs: the level stack overlays the work matrix
+  if (Ar::TASK_REGS) return (lev > (size_t)dim * dim ? lev : (size_t)dim * dim) * T + 16;
   const size_t task_area = 64 * slot * T + 64 * T + 64 * 4;
   const size_t work = (size_t)dim * dim * T;
   return lev * T + (work > task_area ? work : task_area);
