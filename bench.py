@@ -497,7 +497,7 @@ def served_bench(args):
       escalated items (what Workspace does per bucket), inputs pre-gathered per
       bucket outside the timed region;
     * served, saturated: every request offered at t = 0 to the continuous-
-      batching server (max K per batch, 3 slots x 4 streams);
+      batching server (max K per batch, 4 slots x 4 streams);
     * served, Poisson 50k req/s: the open-loop stream; p50 / p99 of
       completion - arrival.
     Values must agree bit for bit between the paths."""
@@ -556,7 +556,7 @@ def served_bench(args):
 
     def serve(arr_s):
         m = len(arr_s)
-        with Server(slots=3, streams_per_slot=4, max_batch=K) as s:
+        with Server(slots=4, streams_per_slot=4, max_batch=K) as s:
             vals, aux, infos = s.replay_packed(func[:m], auto[:m], dim[:m], flat, off[:m], arr_s)
             st = s.stats()
         t_arr = np.array([i.arrival_ns for i in infos], np.int64)
@@ -608,10 +608,10 @@ def config_streams(args):
 
     def run(items, arr, K):
         f, t, d, flat, off = pack(items)
-        with Server(slots=3, streams_per_slot=4, max_batch=K) as s:
+        with Server(slots=4, streams_per_slot=4, max_batch=K) as s:
             s.replay_packed(f[:min(len(f), K)], t[:min(len(f), K)], d[:min(len(f), K)], flat,
                             off[:min(len(f), K)], np.zeros(min(len(f), K)))  # warm the slots
-        with Server(slots=3, streams_per_slot=4, max_batch=K) as s:
+        with Server(slots=4, streams_per_slot=4, max_batch=K) as s:
             vals, aux, infos = s.replay_packed(f, t, d, flat, off, arr)
             st = s.stats()
         t_arr = np.array([i.arrival_ns for i in infos], np.int64)

@@ -154,7 +154,7 @@ class Server:
     ``sum|term| / |value| >= kappa_max``.
     """
 
-    def __init__(self, budget_bytes: int = 0, slots: int = 3, streams_per_slot: int = 4, max_batch: int = 0,
+    def __init__(self, budget_bytes: int = 0, slots: int = 4, streams_per_slot: int = 4, max_batch: int = 0,
                  kappa_max: float = 1e8):
         cfg = _lib.ServerConfig(budget_bytes, slots, streams_per_slot, max_batch, kappa_max)
         h = ctypes.c_void_p()
