@@ -554,11 +554,12 @@ def served_bench(args):
 
     auto = np.full(n_req, _lib.GBSK_AUTO, np.int32)
 
-    def serve(arr_s):
+    def serve(s, arr_s):
         m = len(arr_s)
-        with Server(slots=4, streams_per_slot=4, max_batch=K) as s:
-            vals, aux, infos = s.replay_packed(func[:m], auto[:m], dim[:m], flat, off[:m], arr_s)
-            st = s.stats()
+        st0 = s.stats()
+        vals, aux, infos = s.replay_packed(func[:m], auto[:m], dim[:m], flat, off[:m], arr_s)
+        st = {k: v - st0[k] for k, v in s.stats().items()}
+        st["max_in_flight"] = s.stats()["max_in_flight"]
         t_arr = np.array([i.arrival_ns for i in infos], np.int64)
         t_done = np.array([i.done_ns for i in infos], np.int64)
         t0 = t_arr[0] - int(round(arr_s[0] * 1e9))
@@ -567,12 +568,15 @@ def served_bench(args):
                       "p50_ms": float(np.percentile(lat, 50)) * 1e3, "p99_ms": float(np.percentile(lat, 99)) * 1e3,
                       "batches": st["batches"], "escalated": st["escalated"], "max_in_flight": st["max_in_flight"]}
 
-    serve(np.zeros(min(n_req, 2 * K)))  # warm the slots' buffers
-    sats = [serve(np.zeros(n_req)) for _ in range(reps)]
+    # one long-lived server, as in service: the untimed first pass loads the
+    # kernels and sizes the slots' buffers
+    with Server(slots=4, streams_per_slot=4, max_batch=K) as s:
+        serve(s, np.zeros(n_req))
+        sats = [serve(s, np.zeros(n_req)) for _ in range(reps)]
+        poi_vals, poi = serve(s, arrivals)
     rates = [x[1]["evals_per_s"] for x in sats]
     sat_vals, sat = sats[int(np.argsort(rates)[len(rates) // 2])]  # the median run
     sat["evals_per_s_runs"] = rates
-    poi_vals, poi = serve(arrivals)
     same = lambda x, y: bool(np.array_equal(x.view(np.uint64), y.view(np.uint64)))
     plain_rate = n_req / t_plain
     sizes = np.bincount(dim, minlength=25)
@@ -609,11 +613,10 @@ def config_streams(args):
     def run(items, arr, K):
         f, t, d, flat, off = pack(items)
         with Server(slots=4, streams_per_slot=4, max_batch=K) as s:
-            s.replay_packed(f[:min(len(f), K)], t[:min(len(f), K)], d[:min(len(f), K)], flat,
-                            off[:min(len(f), K)], np.zeros(min(len(f), K)))  # warm the slots
-        with Server(slots=4, streams_per_slot=4, max_batch=K) as s:
+            s.replay_packed(f, t, d, flat, off, np.zeros(len(f)))  # untimed: kernels, slot buffers
+            st0 = s.stats()
             vals, aux, infos = s.replay_packed(f, t, d, flat, off, arr)
-            st = s.stats()
+            st = {k: v - st0[k] for k, v in s.stats().items()}
         t_arr = np.array([i.arrival_ns for i in infos], np.int64)
         t_done = np.array([i.done_ns for i in infos], np.int64)
         t0 = t_arr[0] - int(round(arr[0] * 1e9))

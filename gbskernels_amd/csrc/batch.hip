@@ -102,6 +102,7 @@ struct gbs_batch {
   size_t hcap_in = 0, hcap_res = 0;
   size_t res_total = 0;
   size_t budget = 0;
+  bool slack = true;  // no memory budget: buffers may grow past a batch's need
   bool use_graphs = false;
   std::vector<GraphEntry> graphs;  // small LRU keyed by bucket signature
   unsigned long long gen = 0;
@@ -116,9 +117,14 @@ struct gbs_batch {
 
 namespace gbsk {
 
-static int grow_dev(void** p, size_t* cap, size_t need, unsigned long long* grows) {
+// Without a memory budget, buffers grow to at least twice their old capacity:
+// hipFree waits for the whole device, so a server slot that reallocated on
+// every larger batch would stall the other slots' streams each time. Under a
+// budget they grow to exactly what a batch needs.
+static int grow_dev(void** p, size_t* cap, size_t need, bool slack, unsigned long long* grows) {
   if (need <= *cap) return 0;
   if (*p) (void)hipFree(*p);
+  if (slack && need < 2 * *cap) need = 2 * *cap;
   *p = nullptr;
   *cap = 0;
   if (hipMalloc(p, need) != hipSuccess) {
@@ -131,9 +137,10 @@ static int grow_dev(void** p, size_t* cap, size_t need, unsigned long long* grow
 }
 
 template <class T>
-static int grow_pinned(T** p, size_t* cap, size_t need_elems, unsigned long long* grows) {
+static int grow_pinned(T** p, size_t* cap, size_t need_elems, bool slack, unsigned long long* grows) {
   if (need_elems <= *cap) return 0;
   if (*p) (void)hipHostFree(*p);
+  if (slack && need_elems < 2 * *cap) need_elems = 2 * *cap;
   *p = nullptr;
   *cap = 0;
   if (hipHostMalloc((void**)p, need_elems * sizeof(T), hipHostMallocDefault) != hipSuccess) {
@@ -321,11 +328,12 @@ static int batch_submit(gbs_batch* b, int n, const int* func, const int* tier, c
   // grow-only buffers; any reallocation changes addresses a captured graph
   // refers to, so cached graphs are dropped
   const unsigned long long g0 = b->grows;
-  rc = grow_dev(&b->d_in, &b->cap_in, in_elems * sizeof(cplx) + 16, &b->grows);
-  if (!rc) rc = grow_dev(&b->d_res, &b->cap_res, res_total + 16, &b->grows);
-  if (!rc) rc = grow_dev(&b->d_ws, &b->cap_ws, ws_total + 256, &b->grows);
-  if (!rc) rc = grow_pinned(&b->h_in, &b->hcap_in, in_elems + 1, &b->grows);
-  if (!rc) rc = grow_pinned(&b->h_res, &b->hcap_res, res_total + 16, &b->grows);
+  const bool slack = b->slack;
+  rc = grow_dev(&b->d_in, &b->cap_in, in_elems * sizeof(cplx) + 16, slack, &b->grows);
+  if (!rc) rc = grow_dev(&b->d_res, &b->cap_res, res_total + 16, slack, &b->grows);
+  if (!rc) rc = grow_dev(&b->d_ws, &b->cap_ws, ws_total + 256, slack, &b->grows);
+  if (!rc) rc = grow_pinned(&b->h_in, &b->hcap_in, in_elems + 1, slack, &b->grows);
+  if (!rc) rc = grow_pinned(&b->h_res, &b->hcap_res, res_total + 16, slack, &b->grows);
   if (rc) return rc;
   if (b->grows != g0) drop_graphs(b);
   const std::vector<Bucket>& bks = b->cur;
@@ -693,6 +701,7 @@ int gbs_batch_create(gbs_batch** out, size_t budget_bytes, int nstreams, int use
   if (!b) return GBSK_ERR_ALLOC;
   b->nstreams = nstreams < 1 ? 1 : (nstreams > 16 ? 16 : nstreams);
   b->budget = budget_bytes;
+  b->slack = budget_bytes == 0;
   b->use_graphs = use_graphs != 0;
   b->streams.resize(b->nstreams, nullptr);
   b->joins.resize(b->nstreams, nullptr);
@@ -769,7 +778,10 @@ int gbs_server_create(gbs_server** out, const gbs_server_config* cfg) {
   s->slots.resize((size_t)c.slots);
   for (size_t k = 0; !rc && k < s->slots.size(); ++k) {
     rc = gbs_batch_create(&s->slots[k].b, 0, c.streams_per_slot, 0);
-    if (!rc) s->slots[k].b->per_bucket = true;
+    if (!rc) {
+      s->slots[k].b->per_bucket = true;
+      s->slots[k].b->slack = c.budget_bytes == 0;  // a server budget bounds what slots hold
+    }
   }
   if (!rc) {
     try {
