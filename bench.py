@@ -87,6 +87,9 @@ def main() -> int:
                     help="also time the single-large torontonian at 16..32 modes (0 skips)")
     ap.add_argument("--sampler", type=int, default=1,
                     help="also time the resident GBS sampler on the reference's sampler configurations (0 skips)")
+    ap.add_argument("--accuracy", type=int, default=1,
+                    help="also measure relative error against the mpmath ref tier, physical and "
+                         "cancelling inputs, every tier (0 skips)")
     ap.add_argument("--served-requests", type=int, default=40000,
                     help="requests of the served mixed-size workload (0 skips)")
     ap.add_argument("--served-batch", type=int, default=2048, help="requests per batch / window")
@@ -170,21 +173,6 @@ def main() -> int:
         dd16 = dev_out[16].cpu().numpy().view(np.complex128).copy()
         dd24 = dev_out[24].cpu().numpy().view(np.complex128).copy()
 
-    # parity gate on a sample (oracle = CPU restatement of cpu_ref's Glynn walk)
-    from oracle import oracle as O
-    s16 = np.arange(0, 65536, 4099)[:16]
-    ref16 = O.perm_dd(m16[s16])
-    ref24 = O.perm_dd(m24[:2])
-    rel = lambda a, b: float(np.max(np.abs(a - b) / np.abs(b)))
-    parity = {"n16_fp64_vs_oracle_dd": rel(out16[s16], ref16), "n24_fp64_vs_oracle_dd": rel(out24[:2], ref24),
-              "samples": [len(s16), 2]}
-    if dd16 is not None:
-        parity["n16_dd_vs_oracle_dd"] = rel(dd16[s16], ref16)
-        parity["n24_dd_vs_oracle_dd"] = rel(dd24[:2], ref24)
-    if max(parity["n16_fp64_vs_oracle_dd"], parity["n24_fp64_vs_oracle_dd"]) > 1e-10:
-        print(f"bench: PARITY FAILURE {parity}", file=sys.stderr)
-        return 1
-
     dom_n = 24
     dom_flops = perm_flops(dom_n) * m24.shape[0]
     dom_s = kern_ms[dom_n] * 1e-3
@@ -238,9 +226,17 @@ def main() -> int:
     if args.config3_requests > 0 or args.config5_requests > 0:
         streams = side("config streams", lambda: config_streams(args))
 
-    cpu = None
+    accuracy = None
+    if args.accuracy:
+        accuracy = side("accuracy vs mpmath", accuracy_bench)
+
+    cpu, parity = None, None
     if not args.no_cpu:
-        cpu = cpu_baseline(O, m16, m24, args.cpu_seconds)
+        print("bench: CPU baseline and CPU agreement gate", file=sys.stderr, flush=True)
+        cpu, parity = cpu_baseline(m16, m24, args.cpu_seconds, out16, out24, dd16, dd24)
+        if max(parity["n16_fp64_vs_oracle_dd"], parity["n24_fp64_vs_oracle_dd"]) > 1e-10:
+            print(f"bench: PARITY FAILURE {parity}", file=sys.stderr)
+            return 1
 
     line = {
         "metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
@@ -255,6 +251,7 @@ def main() -> int:
                 "ms_per_step": dd_wall / args.dd_steps * 1e3, "steps": args.dd_steps} if dd_wall else None),
         "per_kernel": per_kernel,
         "parity": parity,
+        "accuracy": accuracy,
         "served": served,
         "streams": streams,
         "repeated_ab": repeated,
@@ -383,6 +380,86 @@ def sampler_bench():
                      "reference_4090": {"path": path, "samples_per_s": rate}})
     return {"workload": "bench/sampler_throughput.py states, resident chain (sieve conditionals, Philox draws)",
             "rows": rows}
+
+
+def accuracy_bench():
+    """Relative error |v - exact| / |exact| of every GPU tier against the mpmath
+    ref tier at 80 digits (gbskernels_amd's own ref tier: Ryser, matching
+    recursion, subset determinants; the error itself is taken in mpmath, so
+    double-double errors below 1e-16 show), through the public API and the
+    extension module:
+
+    * physical: configuration-style inputs (perm: rows of a Haar 64x64 unitary;
+      haf / lhaf: submatrices of U diag(tanh r) U^T, lhaf diagonal CN(0, 0.3^2);
+      tor: threshold O of lossy squeezed pairs), the max over K instances per
+      size, sizes where mpmath finishes in seconds;
+    * cancelling: value = delta x an O(1) factor with O(1) summands,
+      delta = 1e-2 .. 1e-14 (workloads.cancellation), with the tier the auto
+      ladder chose.
+    Certified tiers: fraction of instances whose bound encloses the mpmath
+    value, and the largest bound / |exact|. The reference's own published
+    errors (4090, one physical instance per size) are quoted beside."""
+    import mpmath as mp
+    import gbskernels_amd as gk
+    from gbskernels_amd import _ref
+    from gbskernels_amd.ext import gbskernels_ext as ext
+    from gbskernels_amd.workloads import cancellation, gbs_adjacency, haar_unitary, haf_subset, perm_rows, \
+        threshold_O
+
+    published = {("perm", 4): (1.7e-16, 4.3e-17), ("perm", 8): (2.3e-15, 7.7e-18), ("perm", 12): (7.4e-15, 3.3e-18),
+                 ("haf", 4): (1.0e-16, 1.0e-16), ("haf", 8): (1.2e-14, 2.1e-15), ("haf", 12): (5.5e-15, 6.1e-17),
+                 ("lhaf", 4): (1.5e-16, 7.1e-17), ("lhaf", 8): (9.3e-15, 4.5e-16),
+                 ("lhaf", 10): (1.9e-14, 8.6e-17)}
+    rng = np.random.default_rng(11)
+    U64, A32 = haar_unitary(64, rng), gbs_adjacency(rng)
+    gen = {"perm": lambda n: perm_rows(U64, n, rng), "haf": lambda n: haf_subset(A32, n, rng, False),
+           "lhaf": lambda n: haf_subset(A32, n, rng, True),
+           "tor": lambda n: threshold_O(n, rng).astype(np.complex128)}
+    sizes = {"perm": ((4, 4), (8, 4), (12, 3)), "haf": ((4, 4), (8, 4), (12, 3)),
+             "lhaf": ((4, 4), (8, 4), (10, 3), (12, 2)), "tor": ((2, 4), (4, 4), (6, 4), (8, 3))}
+
+    def errs(vals, exact):
+        with mp.workdps(80):
+            return [float(abs(mp.mpc(complex(v)) - e) / abs(e)) for v, e in zip(vals, exact)]
+
+    def tiers(func, mats, exact):
+        stack = np.ascontiguousarray(np.stack(mats).astype(np.complex128))
+        row = {}
+        for prec in ("fp64", "dd"):
+            v = getattr(gk, f"{func}_batched")(stack, precision=prec, backend="gpu")
+            row[f"{prec}_max_rel"] = max(errs(v, exact))
+        v, diags = getattr(gk, f"{func}_batched")(stack, precision="auto", backend="gpu", return_diagnostics=True)
+        row["auto_max_rel"] = max(errs(v, exact))
+        row["auto_tiers"] = sorted({d["tier"] for d in diags})
+        for name in ("certified", "dd_certified"):
+            v, b = getattr(ext, f"{func}_{name}")(stack)
+            with mp.workdps(80):
+                enc = [bool(np.isfinite(bb) and abs(mp.mpc(complex(vv)) - e) <= mp.mpf(float(bb)))
+                       for vv, bb, e in zip(v, b, exact)]
+                width = max(float(mp.mpf(float(bb)) / abs(e)) for bb, e in zip(b, exact))
+            row[f"{name}_enclosed"] = f"{sum(enc)}/{len(enc)}"
+            row[f"{name}_max_bound_rel"] = width
+        return row
+
+    physical = []
+    for func, rows in sizes.items():
+        for n, K in rows:
+            mats = [gen[func](n) for _ in range(K)]
+            exact = [_ref.REF[func](m, dps=80) for m in mats]
+            row = {"func": func, "n" if func != "tor" else "modes": n, "instances": K}
+            row.update(tiers(func, mats, exact))
+            if (func, n) in published:
+                row["reference_4090_fp64_dd"] = list(published[(func, n)])
+            physical.append(row)
+    cancelling = []
+    for func in ("perm", "haf", "lhaf", "tor"):
+        for delta in (1e-2, 1e-6, 1e-10, 1e-14):
+            mats = [cancellation(func, delta, rng) for _ in range(2)]
+            exact = [_ref.REF[func](m, dps=80) for m in mats]
+            row = {"func": func, "delta": delta}
+            row.update(tiers(func, mats, exact))
+            cancelling.append(row)
+    return {"dps": 80, "oracle": "gbskernels_amd ref tier (mpmath)", "physical": physical, "cancelling": cancelling}
 
 
 def haf_kernels(L, torch, dev, stream):
@@ -644,9 +721,22 @@ def config_streams(args):
     return res
 
 
-def cpu_baseline(O, m16, m24, budget_s):
+def cpu_baseline(m16, m24, budget_s, out16, out24, dd16, dd24):
     """Oracle Glynn (fp64) on the host cores, OpenMP over instances, on a bounded
-    sample; extrapolated to the per-step mix (65536 n=16 + 1024 n=24)."""
+    sample; extrapolated to the per-step mix (65536 n=16 + 1024 n=24). Also the
+    GPU-vs-CPU agreement gate: the oracle's double-double Glynn walk on a sample
+    of the same inputs against the GPU's fp64 and dd outputs. Returns
+    (baseline, agreement)."""
+    from oracle import oracle as O
+    s16 = np.arange(0, 65536, 4099)[:16]
+    ref16 = O.perm_dd(m16[s16])
+    ref24 = O.perm_dd(m24[:2])
+    rel = lambda a, b: float(np.max(np.abs(a - b) / np.abs(b)))
+    parity = {"n16_fp64_vs_oracle_dd": rel(out16[s16], ref16), "n24_fp64_vs_oracle_dd": rel(out24[:2], ref24),
+              "samples": [len(s16), 2]}
+    if dd16 is not None:
+        parity["n16_dd_vs_oracle_dd"] = rel(dd16[s16], ref16)
+        parity["n24_dd_vs_oracle_dd"] = rel(dd24[:2], ref24)
     cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     O.set_threads(cores)
     O.perm(m16[:cores])  # warm
@@ -665,7 +755,7 @@ def cpu_baseline(O, m16, m24, budget_s):
     return {"value": (65536 + 1024) / step_s, "unit": "evals/s", "cores": cores, "kind": "port",
             "sample": f"oracle Glynn fp64 (cpu_ref restated in C++), {k16} n=16 + {k24} n=24 evals "
                       f"of the same workload, OpenMP {cores} threads; per-eval times "
-                      f"{t16 * 1e3:.3f} ms (n=16), {t24:.3f} s (n=24) extrapolated to the step mix"}
+                      f"{t16 * 1e3:.3f} ms (n=16), {t24:.3f} s (n=24) extrapolated to the step mix"}, parity
 
 
 if __name__ == "__main__":

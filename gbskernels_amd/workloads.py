@@ -63,7 +63,11 @@ def perm_rows(U: np.ndarray, n: int, rng: np.random.Generator) -> np.ndarray:
 def gbs_adjacency(rng: np.random.Generator, m: int = 32) -> np.ndarray:
     U = haar_unitary(m, rng)
     r = rng.uniform(0.5, 1.2, m)
-    return (U * np.tanh(r)) @ U.T
+    A = (U * np.tanh(r)) @ U.T
+    # exactly symmetric: the product's rounding leaves A[i, j] and A[j, i] a few
+    # ulps apart, which moves the hafnian by ~1e-16 relative depending on which
+    # triangle a formula reads
+    return (A + A.T) / 2.0
 
 
 def haf_subset(A: np.ndarray, N: int, rng: np.random.Generator, loop: bool) -> np.ndarray:
@@ -72,6 +76,45 @@ def haf_subset(A: np.ndarray, N: int, rng: np.random.Generator, loop: bool) -> n
     if loop:
         M[np.diag_indices(N)] = 0.3 * (rng.standard_normal(N) + 1j * rng.standard_normal(N)) / np.sqrt(2.0)
     return M
+
+
+def cancellation(func: str, delta: float, rng: np.random.Generator) -> np.ndarray:
+    """A matrix whose value is delta times an O(1) factor while its summands stay
+    O(1) (a tunable-cancellation family, like the reference's adversarial
+    sweep, bench/_inputs.py:60-123):
+
+    * perm: [[1, 2], [-1, 2 + delta]] (+) R, perm = delta perm(R), R 4x4 complex;
+    * haf: a 4x4 block whose matchings 2*2 + 1*1 + 1*(delta - 5) sum to delta,
+      (+) a symmetric 4x4 R;
+    * lhaf: [[1, 3], [3, delta - 3]] (lhaf = delta) (+) a symmetric 4x4 R;
+    * tor: one mode, O = a I_2 with a = delta / (1 + delta): tor = a / (1 - a) =
+      delta, as the difference det(I - O)^(-1/2) - 1 of O(1) terms.
+
+    Values are computed from the matrix as rounded (exact values come from the
+    ref tier, not from these formulas).
+    """
+    R = rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4))
+    if func == "perm":
+        A = np.zeros((6, 6), complex)
+        A[:2, :2] = [[1.0, 2.0], [-1.0, 2.0 + delta]]
+        A[2:, 2:] = R
+        return A
+    if func in ("haf", "lhaf"):
+        R = R + R.T
+        if func == "haf":
+            np.fill_diagonal(R, 0.0)
+            Bk = np.array([[0, 2, 1, 1], [2, 0, delta - 5.0, 1], [1, delta - 5.0, 0, 2], [1, 1, 2, 0]], complex)
+            A = np.zeros((8, 8), complex)
+            A[:4, :4], A[4:, 4:] = Bk, R
+            return A
+        A = np.zeros((6, 6), complex)
+        A[:2, :2] = [[1.0, 3.0], [3.0, delta - 3.0]]
+        A[2:, 2:] = R
+        return A
+    if func == "tor":
+        a = delta / (1.0 + delta)
+        return np.array([[a, 0.0], [0.0, a]], complex)
+    raise ValueError(func)
 
 
 def geometric_size(rng: np.random.Generator, p: float, lo: int, hi: int) -> int:
