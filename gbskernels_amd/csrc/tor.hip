@@ -497,16 +497,19 @@ __device__ inline void bottom_walk(const typename Ar::T* M, typename Ar::T det, 
 }
 
 // ----------------------------------------------------- cooperative helpers
-// Eliminate pivot row p of the D x D symmetric matrix X (full storage, LDS) in
-// place: X -= X[:,p] X[p,:] / X[p][p] on rows/cols != p. Returns the pivot.
+// Eliminate pivot p of the D x D matrix X (full storage, LDS) in place on the
+// trailing block: X[i][j] -= (X[i][p] / X[p][p]) X[p][j] for i, j > p. Pivots
+// are taken in order, so rows and columns up to p are never read again. Returns
+// the pivot.
 template <class Ar>
 __device__ inline typename Ar::T eliminate(typename Ar::T* X, int D, int p) {
   const typename Ar::T piv = X[p * D + p];
   const typename Ar::T rp = Ar::recip(piv);
+  const int n = D - p - 1;
   __syncthreads();
-  for (int e = threadIdx.x; e < D * D; e += 64) {
-    const int i = e / D, j = e % D;
-    if (i != p && j != p) X[e] = Ar::fms(Ar::mul(X[i * D + p], rp), X[p * D + j], X[e]);
+  for (int e = threadIdx.x; e < n * n; e += 64) {
+    const int i = p + 1 + e / n, j = p + 1 + e % n;
+    X[i * D + j] = Ar::fms(Ar::mul(X[i * D + p], rp), X[p * D + j], X[i * D + j]);
   }
   __syncthreads();
   return piv;
@@ -566,11 +569,27 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
   if (lane == 0) bad_flag = 0;
   __syncthreads();
 
-  // A = I - O, mode-major: W[(2i+s)][(2j+t)] = delta - O[i + s*nm][j + t*nm]
+  // Active modes: the included leading modes in mode order, then the trailing
+  // modes. Only their rows and columns enter a pivot or the Schur complement,
+  // so W holds just them (D x D). The workgroup with every leading mode
+  // included loads all of O, so a complex entry anywhere still flags the
+  // instance.
+  __shared__ int amode[TOR_MAX_MODES];
+  const int k1 = __popc(p1);
+  const int D = 2 * (k1 + nm - T1);
+  if (lane == 0) {
+    int q = 0;
+    for (int j = 0; j < T1; ++j)
+      if ((p1 >> j) & 1u) amode[q++] = j;
+    for (int j = T1; j < nm; ++j) amode[q++] = j;
+  }
+  __syncthreads();
+  // A = I - O, mode-major: W[(2i+s)][(2j+t)] = delta - O[mi + s*nm][mj + t*nm]
+  // for active modes mi = amode[i], mj = amode[j]
   const cplx* O = mats + b * dim * dim;
-  for (int e = lane; e < dim * dim; e += 64) {
-    const int r = e / dim, c = e % dim;
-    const int gr = (r >> 1) + (r & 1) * nm, gc = (c >> 1) + (c & 1) * nm;
+  for (int e = lane; e < D * D; e += 64) {
+    const int r = e / D, c = e % D;
+    const int gr = amode[r >> 1] + (r & 1) * nm, gc = amode[c >> 1] + (c & 1) * nm;
     const cplx o = O[gr * dim + gc];
     W[e] = (r == c) ? Ar::one_minus(o) : Ar::neg(o);
     if (!Ar::domain(o)) bad_flag = 1;  // complex O: not the real kernels' domain
@@ -579,27 +598,24 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
 
   // leading modes: eliminate the included ones (mode order, x then p pivot)
   T det = Ar::one();
-  int count = 0;
+  const int count = k1;
   bool bad = false;
-  for (int j = 0; j < T1; ++j) {
-    if ((p1 >> j) & 1u) {
-      const T a = eliminate<Ar>(W, dim, 2 * j);
-      const T c = eliminate<Ar>(W, dim, 2 * j + 1);
-      bad |= !Ar::pos(a) || !Ar::pos(c);
-      det = Ar::mul(det, Ar::mul(a, c));
-      ++count;
-    }
+  for (int q = 0; q < k1; ++q) {
+    const T a = eliminate<Ar>(W, D, 2 * q);
+    const T c = eliminate<Ar>(W, D, 2 * q + 1);
+    bad |= !Ar::pos(a) || !Ar::pos(c);
+    det = Ar::mul(det, Ar::mul(a, c));
   }
-  // level 0 = W restricted to modes T1..nm-1
+  // level 0 = W restricted to the trailing modes T1..nm-1 (from row 2 k1)
   {
-    const int D0 = 2 * (nm - T1);
+    const int D0 = 2 * (nm - T1), o0 = 2 * k1;
     if constexpr (TREG) {  // in place: read everything, then write
       constexpr int PER = (2 * (TOR_T2 + R)) * (2 * (TOR_T2 + R)) / 64 + 1;
       T tmp[PER];
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int e = lane + 64 * k;
-        if (e < D0 * D0) tmp[k] = W[(2 * T1 + e / D0) * dim + 2 * T1 + e % D0];
+        if (e < D0 * D0) tmp[k] = W[(o0 + e / D0) * D + o0 + e % D0];
       }
       __syncthreads();
 #pragma unroll
@@ -610,7 +626,7 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
     } else {
       for (int e = lane; e < D0 * D0; e += 64) {
         const int i = e / D0, j = e % D0;
-        lev[e] = W[(2 * T1 + i) * dim + 2 * T1 + j];
+        lev[e] = W[(o0 + i) * D + o0 + j];
       }
     }
     __syncthreads();
@@ -656,9 +672,10 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
           r = 0;
           while (tri(r + 1, 0) <= e) ++r;
           c = e - tri(r, 0);
-        } else {  // full matrix
-          r = e / (2 * R);
-          c = e % (2 * R);
+        } else {  // full matrix (SLOT = 0 when R = 0: the loop does not run)
+          constexpr int W2 = 2 * R > 0 ? 2 * R : 1;
+          r = e / W2;
+          c = e % W2;
         }
         slots[ntask * SLOT + e] = X[r * Dd + c];
       }
