@@ -354,6 +354,13 @@ __device__ constexpr int tri_row(int e) {
   return r;
 }
 __device__ constexpr int tri_col(int e) { return e - tri(tri_row(e), 0); }
+// run-time inverse of tri for the cooperative loops (e < 2^20)
+__device__ inline void tri_decode(int e, int& r, int& c) {
+  r = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+  if (tri(r + 1, 0) <= e) ++r;
+  if (tri(r, 0) > e) --r;
+  c = e - tri(r, 0);
+}
 
 // ----------------------------------------------------------- per-lane bottom
 // Prefix-Cholesky walk over the R trailing modes of one task. M is the task's
@@ -515,14 +522,25 @@ __device__ inline typename Ar::T eliminate(typename Ar::T* X, int D, int p) {
   return piv;
 }
 
-// Y (Dy x Dy) = X (Dx x Dx) with mode `md` (rows 2md, 2md+1) removed.
-template <class T>
-__device__ inline void drop_mode(const T* X, int Dx, int md, T* Y) {
+// Level matrices of the middle tree: packed lower triangles for symmetric
+// policies (a Schur complement of a symmetric matrix is symmetric: half the
+// storage and half the update), full D x D otherwise.
+template <bool SYM>
+__device__ constexpr int lev_size(int D) { return SYM ? D * (D + 1) / 2 : D * D; }
+
+// Y = X (Dx x Dx) with its first mode (rows and columns 0, 1) removed.
+template <bool SYM, class T>
+__device__ inline void drop_first_mode(const T* X, int Dx, T* Y) {
   const int Dy = Dx - 2;
-  for (int e = threadIdx.x; e < Dy * Dy; e += 64) {
-    int i = e / Dy, j = e % Dy;
-    const int si = i < 2 * md ? i : i + 2, sj = j < 2 * md ? j : j + 2;
-    Y[e] = X[si * Dx + sj];
+  for (int e = threadIdx.x; e < lev_size<SYM>(Dy); e += 64) {
+    if constexpr (SYM) {
+      int i, j;
+      tri_decode(e, i, j);
+      Y[e] = X[tri(i + 2, j + 2)];
+    } else {
+      const int i = e / Dy, j = e % Dy;
+      Y[e] = X[(i + 2) * Dx + j + 2];
+    }
   }
   __syncthreads();
 }
@@ -552,10 +570,7 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
   T* lev = lds;
   int levtot = 0;
 #pragma unroll
-  for (int d = 0; d <= T2; ++d) {
-    const int Dd = 2 * (nm - T1 - d);
-    levtot += Dd * Dd;
-  }
+  for (int d = 0; d <= T2; ++d) levtot += lev_size<Ar::SYM>(2 * (nm - T1 - d));
   constexpr int SLOT = tor_slot<Ar, R>();
   constexpr bool TREG = Ar::TASK_REGS;
   // TREG: the work matrix and the level stack share the region from 0 (level 0
@@ -606,28 +621,37 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
     bad |= !Ar::pos(a) || !Ar::pos(c);
     det = Ar::mul(det, Ar::mul(a, c));
   }
-  // level 0 = W restricted to the trailing modes T1..nm-1 (from row 2 k1)
+  // level 0 = W restricted to the trailing modes T1..nm-1 (from row 2 k1);
+  // symmetric policies keep its lower triangle
   {
-    const int D0 = 2 * (nm - T1), o0 = 2 * k1;
+    const int D0 = 2 * (nm - T1), o0 = 2 * k1, n0 = lev_size<Ar::SYM>(D0);
+    auto src = [&](int e) -> int {
+      int i, j;
+      if constexpr (Ar::SYM) {
+        tri_decode(e, i, j);
+      } else {
+        i = e / D0;
+        j = e % D0;
+      }
+      return (o0 + i) * D + o0 + j;
+    };
     if constexpr (TREG) {  // in place: read everything, then write
-      constexpr int PER = (2 * (TOR_T2 + R)) * (2 * (TOR_T2 + R)) / 64 + 1;
+      constexpr int D0MAX = 2 * (TOR_T2 + R);
+      constexpr int PER = lev_size<Ar::SYM>(D0MAX) / 64 + 1;
       T tmp[PER];
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int e = lane + 64 * k;
-        if (e < D0 * D0) tmp[k] = W[(o0 + e / D0) * D + o0 + e % D0];
+        if (e < n0) tmp[k] = W[src(e)];
       }
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int e = lane + 64 * k;
-        if (e < D0 * D0) lev[e] = tmp[k];
+        if (e < n0) lev[e] = tmp[k];
       }
     } else {
-      for (int e = lane; e < D0 * D0; e += 64) {
-        const int i = e / D0, j = e % D0;
-        lev[e] = W[(o0 + i) * D + o0 + j];
-      }
+      for (int e = lane; e < n0; e += 64) lev[e] = W[src(e)];
     }
     __syncthreads();
   }
@@ -655,7 +679,7 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
       // leaf: task ntask goes to lane ntask's registers
       if (lane == ntask) {
 #pragma unroll
-        for (int e = 0; e < SLOT; ++e) Mreg[e] = X[tri_row(e) * Dd + tri_col(e)];
+        for (int e = 0; e < SLOT; ++e) Mreg[e] = X[e];  // the leaf level is the packed task
         my_det = wdet[d];
         my_cnt = wcnt[d];
       }
@@ -666,19 +690,8 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
     }
     if (d == T2) {
       // leaf: the task is this level's matrix (the trailing R modes)
-      for (int e = lane; e < SLOT; e += 64) {
-        int r, c;
-        if constexpr (Ar::SYM) {  // packed lower triangle
-          r = 0;
-          while (tri(r + 1, 0) <= e) ++r;
-          c = e - tri(r, 0);
-        } else {  // full matrix (SLOT = 0 when R = 0: the loop does not run)
-          constexpr int W2 = 2 * R > 0 ? 2 * R : 1;
-          r = e / W2;
-          c = e % W2;
-        }
-        slots[ntask * SLOT + e] = X[r * Dd + c];
-      }
+      // (the leaf level has the task's layout: packed lower or full 2R x 2R)
+      for (int e = lane; e < SLOT; e += 64) slots[ntask * SLOT + e] = X[e];
       if (lane == 0) {
         slot_det[ntask] = wdet[d];
         slot_cnt[ntask] = wcnt[d];
@@ -688,11 +701,11 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
       --d;
       continue;
     }
-    T* Y = X + Dd * Dd;
+    T* Y = X + lev_size<Ar::SYM>(Dd);
     const int st = wstate[d];
     if (st == 0) {  // exclude middle mode d: drop its rows
       wstate[d] = 1;
-      drop_mode(X, Dd, 0, Y);
+      drop_first_mode<Ar::SYM>(X, Dd, Y);
       wdet[d + 1] = wdet[d];
       wcnt[d + 1] = wcnt[d];
     } else if (st == 1) {
@@ -701,21 +714,41 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
       // and l_i1 = (X_i1 - l_i0 b) / c, the complement is
       // Y_ij = X_ij - l_i0 X_0j - l_i1 (X_1j - b X_0j / a): the reciprocals of
       // the two pivots, no division per entry
+      // Symmetric policies (packed lower): X_0j = X_j0, X_1j = X_j1, and only
+      // the lower triangle of Y is formed.
       wstate[d] = 2;
       const T a = X[0];
-      const T b01 = X[1];
-      const T b10 = Ar::SYM ? b01 : X[Dd];  // [[a, b01], [b10, c']]
+      const T b01 = X[1];                                   // packed: X_10
+      const T b10 = Ar::SYM ? b01 : X[Dd];                  // [[a, b01], [b10, c']]
       const T ra = Ar::recip(a);
-      const T c = Ar::sub(X[Dd + 1], Ar::mul(Ar::mul(b10, b01), ra));
+      const T c = Ar::sub(Ar::SYM ? X[2] : X[Dd + 1], Ar::mul(Ar::mul(b10, b01), ra));
       const T rc = Ar::recip(c);
       const int Dy = Dd - 2;
-      for (int e = lane; e < Dy * Dy; e += 64) {
-        const int i = e / Dy + 2, j = e % Dy + 2;
-        const T li0 = Ar::mul(X[i * Dd], ra);
-        const T li1 = Ar::mul(Ar::fms(li0, b01, X[i * Dd + 1]), rc);
-        const T x0j = X[j];
-        const T x1j = Ar::fms(Ar::mul(x0j, ra), b10, X[Dd + j]);
-        T v = Ar::fms(li0, x0j, X[i * Dd + j]);
+      for (int e = lane; e < lev_size<Ar::SYM>(Dy); e += 64) {
+        int i, j, xi0, xi1, x0j_, x1j_, xij;
+        if constexpr (Ar::SYM) {
+          tri_decode(e, i, j);
+          i += 2;
+          j += 2;
+          xi0 = tri(i, 0);
+          xi1 = tri(i, 1);
+          x0j_ = tri(j, 0);
+          x1j_ = tri(j, 1);
+          xij = tri(i, j);
+        } else {
+          i = e / Dy + 2;
+          j = e % Dy + 2;
+          xi0 = i * Dd;
+          xi1 = i * Dd + 1;
+          x0j_ = j;
+          x1j_ = Dd + j;
+          xij = i * Dd + j;
+        }
+        const T li0 = Ar::mul(X[xi0], ra);
+        const T li1 = Ar::mul(Ar::fms(li0, b01, X[xi1]), rc);
+        const T x0j = X[x0j_];
+        const T x1j = Ar::fms(Ar::mul(x0j, ra), b10, X[x1j_]);
+        T v = Ar::fms(li0, x0j, X[xij]);
         v = Ar::fms(li1, x1j, v);
         Y[e] = v;
       }
@@ -729,7 +762,7 @@ __global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ m
       continue;
     }
     wstate[d + 1] = 0;
-    woff[d + 1] = woff[d] + Dd * Dd;
+    woff[d + 1] = woff[d] + lev_size<Ar::SYM>(Dd);
     ++d;
   }
   __syncthreads();
@@ -911,7 +944,7 @@ static size_t tor_lds_bytes(int nm, const TorSplit& sp) {
   size_t lev = 0;
   for (int d = 0; d <= sp.T2; ++d) {
     const size_t Dd = 2 * (nm - sp.T1 - d);
-    lev += Dd * Dd;
+    lev += Ar::SYM ? Dd * (Dd + 1) / 2 : Dd * Dd;  // the kernel's lev_size
   }
   const size_t slot = Ar::SYM ? (size_t)(2 * sp.R) * (2 * sp.R + 1) / 2 : (size_t)(2 * sp.R) * (2 * sp.R);
   const size_t T = sizeof(typename Ar::T);
