@@ -10,13 +10,14 @@
 // MI355X design (this file's own). With A = I - O re-ordered mode-major
 // (mode i -> rows 2i, 2i+1), one instance is split three ways:
 //  * T1 leading modes: their 2^T1 include patterns are separate workgroups;
-//    each eliminates its included modes from A (right-looking Cholesky,
-//    cooperative over the wave's lanes) to get the Schur complement on the
-//    remaining modes;
+//    each loads A on its active modes only (included leading, then trailing)
+//    and eliminates the included ones (right-looking Cholesky, cooperative
+//    over the wave's lanes) to get the Schur complement on the trailing modes;
 //  * T2 <= 6 middle modes: the wave walks their include/exclude tree together,
-//    one Schur complement per tree level in LDS, every update spread over the
-//    64 lanes; each of the 2^T2 leaves becomes one lane's task (its Schur
-//    complement on the last R modes, the running determinant, the count);
+//    one Schur complement per tree level in LDS (packed lower triangle when
+//    symmetric), every update spread over the 64 lanes; each of the 2^T2
+//    leaves becomes one lane's task (its Schur complement on the last R modes,
+//    the running determinant, the count);
 //  * R <= 5 trailing modes: each lane walks its task's 2^R leaves alone with a
 //    fully unrolled prefix-Cholesky (reference algorithm) whose factor lives in
 //    VGPRs.
