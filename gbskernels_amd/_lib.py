@@ -84,6 +84,9 @@ SIGNATURES = {
     "gbs_tor_single_ddcertified_host": ([_cp, _i, _i, _cp, _cp], _i),
     "gbs_sampler_sample": ([_cp, _i, _cp, _i, _i, _i, _i, _cp, ctypes.c_ulonglong, _cp], _i),
     "gbs_philox4x32_10": ([_cp, _cp, _cp], None),
+    "gbs_perm_get_lane_log2": ([_i, _i, _cp], _i),
+    "gbs_perm_set_lane_log2": ([_i, _i, _i], _i),
+    "gbs_perm_split": ([_i, _i, _cp, _cp], _i),
     "gbs_lhaf_repeated_cert_host": ([_cp, _cp, _i, _cp, _i, _cp, _cp], _i),
 }
 

@@ -44,17 +44,17 @@ static int launch(int func, int tier, const cplx* d_in, int dim, int B, cplx* d_
   switch (func) {
     case GBSK_PERM:
       switch (tier) {
-        case GBSK_FP64: return perm_launch(dim, PERM_FP64, d_in, B, d_out, nullptr, ws, st);
-        case GBSK_DD: return perm_launch(dim, PERM_DD, d_in, B, d_out, nullptr, ws, st);
+        case GBSK_FP64: return perm_launch(dim, PERM_FP64, d_in, B, d_out, nullptr, ws, ws_size, st);
+        case GBSK_DD: return perm_launch(dim, PERM_DD, d_in, B, d_out, nullptr, ws, ws_size, st);
         case GBSK_KAPPA:
           if (!d_aux) return GBSK_ERR_ARG;
-          return perm_launch(dim, PERM_KAPPA, d_in, B, d_out, d_aux, ws, st);
+          return perm_launch(dim, PERM_KAPPA, d_in, B, d_out, d_aux, ws, ws_size, st);
         case GBSK_CERTIFIED:
           if (!d_aux) return GBSK_ERR_ARG;
-          return perm_launch(dim, PERM_CERT, d_in, B, d_out, d_aux, ws, st);
+          return perm_launch(dim, PERM_CERT, d_in, B, d_out, d_aux, ws, ws_size, st);
         case GBSK_DD_CERTIFIED:
           if (!d_aux) return GBSK_ERR_ARG;
-          return perm_launch(dim, PERM_DDCERT, d_in, B, d_out, d_aux, ws, st);
+          return perm_launch(dim, PERM_DDCERT, d_in, B, d_out, d_aux, ws, ws_size, st);
         default: return GBSK_ERR_FUNC;
       }
     case GBSK_HAF:
@@ -334,6 +334,24 @@ int gbs_eval_device(int func, int tier, const gbs_complex* d_in, int dim, int B,
   if (cap == 0) return GBSK_ERR_FUNC;  // (func, tier) not built
   if (dim > cap) return GBSK_ERR_DIM;
   return launch(func, tier, (const cplx*)d_in, dim, B, (cplx*)d_out, d_aux, d_ws, ws_size, (hipStream_t)stream);
+}
+
+int gbs_perm_get_lane_log2(int family, int n, int* log2_terms) {
+  if (!log2_terms) return GBSK_ERR_ARG;
+  const int v = perm_lane_log2_get(family, n);
+  if (v == GBSK_ERR_ARG) return v;
+  *log2_terms = v;
+  return 0;
+}
+
+int gbs_perm_set_lane_log2(int family, int n, int log2_terms) { return perm_lane_log2_set(family, n, log2_terms); }
+
+int gbs_perm_split(int family, int n, int* lane_log2, int* workgroups) {
+  if ((family != 0 && family != 1) || n < 1 || n > PERM_MAX_N || !lane_log2 || !workgroups) return GBSK_ERR_ARG;
+  const PermSplit sp = perm_split(n, family);
+  *lane_log2 = sp.M;
+  *workgroups = sp.W;
+  return 0;
 }
 
 int gbs_max_dim(int func, int tier) {

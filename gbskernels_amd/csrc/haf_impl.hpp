@@ -293,7 +293,6 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const HafPhaseJobs<typena
     unrank<M>(active ? q : 0, n, p);
     // lane r owns row r of C = B_S X: C[r][2u] = A[R][2p_u + 1], C[r][2u+1] = A[R][2p_u]
     const int R = 2 * p[r >> 1] + (r & 1);
-    const int gs = lane_ok ? g : 0;
     T P[S];
     double eP[ANYC ? S : 1];
 #pragma unroll

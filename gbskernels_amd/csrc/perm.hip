@@ -20,15 +20,32 @@
 #include "cert.hpp"
 #include "common.hpp"
 #include "perm.hpp"
+#include "perm_tune.hpp"
 
 namespace gbsk {
 
+// ------------------------------------------------------------- split table
+// Read with relaxed atomics by every launch; written only by the tuning entry
+// point (documented: not while evaluations are in flight).
+static int g_lane_log2[2][PERM_MAX_N + 1];
+[[maybe_unused]] static const bool g_lane_log2_init = [] {
+  for (int f = 0; f < 2; ++f)
+    for (int n = 0; n <= PERM_MAX_N; ++n) g_lane_log2[f][n] = kPermLaneLog2[f][n];
+  return true;
+}();
+
+template <int N, int DD>
+static PermSplit split_of() {
+  using C = PermCfg<N, DD>;
+  const int t = __atomic_load_n(&g_lane_log2[DD][N], __ATOMIC_RELAXED);
+  const int M = t < C::MLIM ? t : C::MLIM;
+  return PermSplit{M, 1 << (C::MLIM - M)};
+}
+
 // rounding depth of the certificate arithmetic (cert.hpp): per lane 8 * 2^M + 16 N,
 // plus the butterfly and the chunk sum
-template <int N>
-__device__ constexpr double perm_cert_depth() {
-  using C = PermCfg<N, 0>;
-  return 8.0 * (double)(1u << C::M) + 16.0 * N + 4.0 * C::W + 64.0;
+__device__ inline double perm_cert_depth(int N, int M, int W) {
+  return 8.0 * (double)(1u << M) + 16.0 * N + 4.0 * W + 64.0;
 }
 
 // Per-lane Gray range [start, start + 2^M): initial row sums at gray(start),
@@ -295,21 +312,20 @@ __device__ inline void cddc_group_sum(ddc& v, double& e) {
 
 // rounding depth of the certified-dd bound: the row bound's 2^M-step
 // accumulation, the product chains, the butterfly and the chunk sum
-template <int N>
-__device__ constexpr double perm_ddcert_depth() {
-  using C = PermCfg<N, 1>;
-  return 8.0 * (double)(1u << C::M) + 16.0 * N + 8.0 * C::W + 128.0;
+__device__ inline double perm_ddcert_depth(int N, int M, int W) {
+  return 8.0 * (double)(1u << M) + 16.0 * N + 8.0 * W + 128.0;
 }
 
 // One workgroup = one wave = MPW matrices x LPM lanes (or one chunk of one
 // matrix when the matrix is split over W workgroups).
 //   out      : final values (W == 1) or chunk partials [B][W] (W > 1)
 //   aux      : kappa sum|term| (final or partial), may be null in plain mode
+//   M, W     : the run-time split (log2 terms per lane, workgroups per matrix)
 template <int N, int MODE>
 __global__ __launch_bounds__(64) void perm_kernel(const cplx* __restrict__ mats, int B, cplx* __restrict__ out,
-                                                  double* __restrict__ aux, int g8) {
+                                                  double* __restrict__ aux, int g8, int M, int W) {
   using C = PermCfg<N, perm_is_dd(MODE) ? 1 : 0>;
-  constexpr int LPM = C::LPM, MPW = 64 / LPM, W = C::W, M = C::M;
+  constexpr int LPM = C::LPM, MPW = 64 / LPM;
   __shared__ cplx AT[MPW][N * N];
   const int lane = threadIdx.x;
   const int gidx = lane / LPM, sub = lane % LPM;
@@ -319,7 +335,7 @@ __global__ __launch_bounds__(64) void perm_kernel(const cplx* __restrict__ mats,
   const int64_t wg = blockIdx.x;
   int64_t grp;
   int chunk;
-  if constexpr (W == 1) {
+  if (W == 1) {
     grp = wg;
     chunk = 0;
   } else {
@@ -352,12 +368,12 @@ __global__ __launch_bounds__(64) void perm_kernel(const cplx* __restrict__ mats,
       acc = group_sum<LPM>(acc);
     }
     if (sub == 0 && b < B) {
-      if constexpr (W == 1) {
+      if (W == 1) {
         const double scale = 1.0 / (double)(1ull << (N - 1));
         const ddc sc = ddc_mul_d(acc, scale);  // exact (power of two)
         out[b] = ddc_to_c(sc);
         if constexpr (DDCERT)
-          aux[b] = cert_finish(fma(ea, scale, cddc_collapse_err(sc) + 4.0 * CERT_UFL), perm_ddcert_depth<N>());
+          aux[b] = cert_finish(fma(ea, scale, cddc_collapse_err(sc) + 4.0 * CERT_UFL), perm_ddcert_depth(N, M, W));
       } else {
         // chunk partials stay double-double: hi in out, lo in aux-as-cplx,
         // then (certified-dd) the bounds after the B * W lo entries
@@ -379,11 +395,11 @@ __global__ __launch_bounds__(64) void perm_kernel(const cplx* __restrict__ mats,
       if (KAPPA) aacc = group_sum<LPM>(aacc);
     }
     if (sub == 0 && b < B) {
-      if constexpr (W == 1) {
+      if (W == 1) {
         const double scale = 1.0 / (double)(1ull << (N - 1));
         out[b] = c_scale(acc, scale);
         if (KAPPA) aux[b] = aacc * scale;
-        if (CERT) aux[b] = cert_finish(aacc * scale + CERT_UFL, perm_cert_depth<N>());
+        if (CERT) aux[b] = cert_finish(aacc * scale + CERT_UFL, perm_cert_depth(N, M, W));
       } else {
         out[b * W + chunk] = acc;
         if (KAPPA || CERT) aux[b * W + chunk] = aacc;
@@ -395,9 +411,7 @@ __global__ __launch_bounds__(64) void perm_kernel(const cplx* __restrict__ mats,
 // Fixed-order (chunk 0, 1, ..., W-1) reduction of chunk partials.
 template <int N, int MODE>
 __global__ void perm_reduce_kernel(const cplx* __restrict__ part, const double* __restrict__ part_aux, int B,
-                                   cplx* __restrict__ out, double* __restrict__ aux) {
-  using C = PermCfg<N, perm_is_dd(MODE) ? 1 : 0>;
-  constexpr int W = C::W;
+                                   cplx* __restrict__ out, double* __restrict__ aux, int M, int W) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   const double scale = 1.0 / (double)(1ull << (N - 1));
@@ -419,7 +433,7 @@ __global__ void perm_reduce_kernel(const cplx* __restrict__ part, const double* 
     const ddc sc = ddc_mul_d(s, scale);
     out[b] = ddc_to_c(sc);
     if constexpr (MODE == PERM_DDCERT)
-      aux[b] = cert_finish(fma(e, scale, cddc_collapse_err(sc) + 4.0 * CERT_UFL), perm_ddcert_depth<N>());
+      aux[b] = cert_finish(fma(e, scale, cddc_collapse_err(sc) + 4.0 * CERT_UFL), perm_ddcert_depth(N, M, W));
   } else {
     cplx s{0.0, 0.0};
     double a = 0.0;
@@ -430,18 +444,25 @@ __global__ void perm_reduce_kernel(const cplx* __restrict__ part, const double* 
     }
     out[b] = c_scale(s, scale);
     if (MODE == PERM_KAPPA) aux[b] = a * scale;
-    if (MODE == PERM_CERT) aux[b] = cert_finish(a * scale + CERT_UFL, perm_cert_depth<N>());
+    if (MODE == PERM_CERT) aux[b] = cert_finish(a * scale + CERT_UFL, perm_cert_depth(N, M, W));
   }
 }
 
+static size_t perm_chunk_bytes(int mode) { return mode == PERM_DDCERT ? 16 + 16 + 8 : 16 + 16; }
+
 template <int N, int MODE>
-static int launch_perm_n(const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
+static int launch_perm_n(const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, size_t ws_size,
+                         hipStream_t st) {
   using C = PermCfg<N, perm_is_dd(MODE) ? 1 : 0>;
-  constexpr int MPW = 64 / C::LPM, W = C::W;
+  constexpr int MPW = 64 / C::LPM;
+  // the split is read once: the workspace check and the launches agree
+  const PermSplit sp = split_of<N, perm_is_dd(MODE) ? 1 : 0>();
+  const int M = sp.M, W = sp.W;
+  if (W > 1 && ws_size < (size_t)B * W * perm_chunk_bytes(MODE)) return GBSK_ERR_WORKSPACE;
   // at most 2^30 workgroups per launch: large batches run as consecutive
   // sub-batches (same stream, the workspace reused), matrices of a sub-batch a
   // multiple of 8 groups so the XCD mapping is unchanged
-  constexpr int64_t max_groups = (((int64_t)1 << 30) / W) / 8 * 8;
+  const int64_t max_groups = (((int64_t)1 << 30) / W) / 8 * 8;
   const int64_t max_b = max_groups * MPW;
   for (int64_t b0 = 0; b0 < B; b0 += max_b) {
     const int Bs = (int)((B - b0) < max_b ? (B - b0) : max_b);
@@ -452,15 +473,16 @@ static int launch_perm_n(const cplx* d_in, int B, cplx* d_out, double* d_aux, vo
     const int g8 = groups < 8 ? (int)groups : 8;
     const int64_t grid = (W == 1) ? groups : ((groups + g8 - 1) / g8) * g8 * W;
     if (W == 1) {
-      hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, in, Bs, out, aux, g8);
+      hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, in, Bs, out, aux, g8, M, W);
     } else {
       // partials: values [Bs][W] cplx, then [Bs][W] of 16 bytes (kappa / bound
       // doubles, or the dd low parts), then (certified dd) [Bs][W] bounds
       cplx* part = reinterpret_cast<cplx*>(ws);
       double* part_aux = reinterpret_cast<double*>(part + (size_t)Bs * W);
-      hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, in, Bs, part, part_aux, g8);
+      hipLaunchKernelGGL((perm_kernel<N, MODE>), dim3((unsigned)grid), dim3(64), 0, st, in, Bs, part, part_aux, g8,
+                         M, W);
       hipLaunchKernelGGL((perm_reduce_kernel<N, MODE>), dim3((unsigned)((Bs + 255) / 256)), dim3(256), 0, st, part,
-                         part_aux, Bs, out, aux);
+                         part_aux, Bs, out, aux, M, W);
     }
   }
   return (int)hipGetLastError();
@@ -477,23 +499,17 @@ __global__ void perm_n0_kernel(int B, cplx* out, double* aux, double auxval) {
 
 
 template <int MODE, int... Ns>
-static int dispatch_perm(int n, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st,
-                         std::integer_sequence<int, Ns...>) {
+static int dispatch_perm(int n, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, size_t ws_size,
+                         hipStream_t st, std::integer_sequence<int, Ns...>) {
   int rc = GBSK_ERR_DIM;
-  ((n == Ns ? (rc = launch_perm_n<Ns, MODE>(d_in, B, d_out, d_aux, ws, st), 0) : 0), ...);
+  ((n == Ns ? (rc = launch_perm_n<Ns, MODE>(d_in, B, d_out, d_aux, ws, ws_size, st), 0) : 0), ...);
   return rc;
 }
 
 template <int... Ns>
-static size_t ws_perm(int n, int mode, int B, std::integer_sequence<int, Ns...>) {
-  size_t r = 0;
-  const size_t per_chunk = mode == PERM_DDCERT ? 16 + 16 + 8 : 16 + 16;
-  ((n == Ns ? (r = (perm_is_dd(mode) ? PermCfg<Ns, 1>::W : PermCfg<Ns, 0>::W) > 1
-                       ? (size_t)B * (perm_is_dd(mode) ? PermCfg<Ns, 1>::W : PermCfg<Ns, 0>::W) * per_chunk
-                       : 0,
-               0)
-            : 0),
-   ...);
+static PermSplit split_dispatch(int n, int family, std::integer_sequence<int, Ns...>) {
+  PermSplit r{0, 1};
+  ((n == Ns ? (r = family ? split_of<Ns, 1>() : split_of<Ns, 0>(), 0) : 0), ...);
   return r;
 }
 
@@ -503,12 +519,33 @@ constexpr std::integer_sequence<int, (Is + 1)...> shift1(std::integer_sequence<i
 }
 using PermSizes = decltype(shift1(std::make_integer_sequence<int, PERM_MAX_N>{}));
 
-size_t perm_workspace_bytes(int n, int mode, int B) {
-  if (n < 1 || n > PERM_MAX_N || B <= 0) return 0;
-  return ws_perm(n, mode, B, PermSizes{});
+PermSplit perm_split(int n, int family) {
+  if (n < 1 || n > PERM_MAX_N) return PermSplit{0, 1};
+  return split_dispatch(n, family ? 1 : 0, PermSizes{});
 }
 
-int perm_launch(int n, int mode, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
+int perm_lane_log2_get(int family, int n) {
+  if ((family != 0 && family != 1) || n < 0 || n > PERM_MAX_N) return GBSK_ERR_ARG;
+  return __atomic_load_n(&g_lane_log2[family][n], __ATOMIC_RELAXED);
+}
+
+// at least 2^LOGMIN terms per lane (16 fp64, 4 dd: the per-lane start costs n^2
+// updates) and at most 2^27 (every n <= 28 then runs one workgroup per matrix)
+int perm_lane_log2_set(int family, int n, int log2_terms) {
+  if ((family != 0 && family != 1) || n < 0 || n > PERM_MAX_N) return GBSK_ERR_ARG;
+  if (log2_terms < (family ? 2 : 4) || log2_terms > 27) return GBSK_ERR_ARG;
+  __atomic_store_n(&g_lane_log2[family][n], log2_terms, __ATOMIC_RELAXED);
+  return 0;
+}
+
+size_t perm_workspace_bytes(int n, int mode, int B) {
+  if (n < 1 || n > PERM_MAX_N || B <= 0) return 0;
+  const PermSplit sp = perm_split(n, perm_is_dd(mode) ? 1 : 0);
+  return sp.W > 1 ? (size_t)B * sp.W * perm_chunk_bytes(mode) : 0;
+}
+
+int perm_launch(int n, int mode, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, size_t ws_size,
+                hipStream_t st) {
   if (B <= 0) return 0;
   if (n < 0 || n > PERM_MAX_N) return GBSK_ERR_DIM;
   if (n == 0) {
@@ -518,11 +555,11 @@ int perm_launch(int n, int mode, const cplx* d_in, int B, cplx* d_out, double* d
     return (int)hipGetLastError();
   }
   switch (mode) {
-    case PERM_FP64: return dispatch_perm<PERM_FP64>(n, d_in, B, d_out, d_aux, ws, st, PermSizes{});
-    case PERM_KAPPA: return dispatch_perm<PERM_KAPPA>(n, d_in, B, d_out, d_aux, ws, st, PermSizes{});
-    case PERM_DD: return dispatch_perm<PERM_DD>(n, d_in, B, d_out, d_aux, ws, st, PermSizes{});
-    case PERM_CERT: return dispatch_perm<PERM_CERT>(n, d_in, B, d_out, d_aux, ws, st, PermSizes{});
-    case PERM_DDCERT: return dispatch_perm<PERM_DDCERT>(n, d_in, B, d_out, d_aux, ws, st, PermSizes{});
+    case PERM_FP64: return dispatch_perm<PERM_FP64>(n, d_in, B, d_out, d_aux, ws, ws_size, st, PermSizes{});
+    case PERM_KAPPA: return dispatch_perm<PERM_KAPPA>(n, d_in, B, d_out, d_aux, ws, ws_size, st, PermSizes{});
+    case PERM_DD: return dispatch_perm<PERM_DD>(n, d_in, B, d_out, d_aux, ws, ws_size, st, PermSizes{});
+    case PERM_CERT: return dispatch_perm<PERM_CERT>(n, d_in, B, d_out, d_aux, ws, ws_size, st, PermSizes{});
+    case PERM_DDCERT: return dispatch_perm<PERM_DDCERT>(n, d_in, B, d_out, d_aux, ws, ws_size, st, PermSizes{});
   }
   return GBSK_ERR_ARG;
 }

@@ -230,6 +230,22 @@ int gbs_sampler_sample(const gbs_complex* Ak, int ak_total, const int* off, int 
  * counterpart): out = Philox4x32-10(ctr[4], key[2]) */
 void gbs_philox4x32_10(const unsigned* ctr, const unsigned* key, unsigned* out);
 
+/* ---- 10. tuning: the permanent's Gray-code chunk shape (no reference
+ * counterpart; it replaces the reference's fixed crossovers PERM_COOP_CROSSOVER
+ * / PERM_COOP_GROUPS, core/host_api.cu:65-72) ---------------------------------
+ * One n x n permanent walks 2^(n-1) Gray terms: LPM lanes (fixed per n) times W
+ * workgroups times 2^k terms per lane. The table entry t gives k = min(t, the
+ * one-workgroup limit) and W = 2^(limit - k). family 0: fp64 / kappa /
+ * certified (one split, so their values stay bit-identical); family 1: dd /
+ * certified-dd. A value's last bits depend on the split (its summation order),
+ * never on the batch. The built-in table comes from gbskernels_amd/autotune.py;
+ * setting an entry is for tuning runs and must not overlap evaluations.
+ * log2_terms: 4..27 (family 0), 2..27 (family 1). */
+int gbs_perm_get_lane_log2(int family, int n, int* log2_terms);
+int gbs_perm_set_lane_log2(int family, int n, int log2_terms);
+/* the split a launch of n x n permanents uses now (n = 1..28) */
+int gbs_perm_split(int family, int n, int* lane_log2, int* workgroups);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,35 @@ def test_pure_queries():
     assert L.gbs_workspace_bytes(0, 0, 24, 1000) > 0
 
 
+def test_perm_split_table_queries():
+    """The permanent's chunk-shape table (host-side state, no GPU call)."""
+    from gbskernels_amd import _lib
+    L = _lib.lib()
+    k, w, v = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    for family in (0, 1):
+        for n in (1, 8, 16, 24, 28):
+            assert L.gbs_perm_split(family, n, ctypes.byref(k), ctypes.byref(w)) == 0
+            # 2^(n-1) Gray terms = lanes per matrix x workgroups x terms per lane
+            lanes = 1 << min(max(n - 1 - (2 if family else 4), 0), 6)
+            assert lanes * w.value * (1 << k.value) == 1 << (n - 1)
+    assert L.gbs_perm_get_lane_log2(0, 24, ctypes.byref(v)) == 0
+    old = v.value
+    try:
+        assert L.gbs_perm_set_lane_log2(0, 24, 12) == 0
+        assert L.gbs_perm_split(0, 24, ctypes.byref(k), ctypes.byref(w)) == 0
+        assert (k.value, w.value) == (12, 32)
+        # a larger chunk needs less workspace (fewer partials)
+        assert L.gbs_workspace_bytes(0, 0, 24, 100) == 100 * 32 * 32
+    finally:
+        assert L.gbs_perm_set_lane_log2(0, 24, old) == 0
+    ERR_ARG = 10001
+    assert L.gbs_perm_set_lane_log2(0, 24, 3) == ERR_ARG      # below 2^4 terms per lane
+    assert L.gbs_perm_set_lane_log2(1, 24, 1) == ERR_ARG
+    assert L.gbs_perm_set_lane_log2(2, 24, 9) == ERR_ARG      # no such family
+    assert L.gbs_perm_set_lane_log2(0, 29, 9) == ERR_ARG
+    assert L.gbs_perm_split(0, 0, ctypes.byref(k), ctypes.byref(w)) == ERR_ARG
+
+
 def test_ext_module_surface():
     import importlib
     import sys

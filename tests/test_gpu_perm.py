@@ -235,3 +235,39 @@ def test_perm_dd_certified_resolves_cancellation(ext, oracle):
     assert bound[0] / abs(exact) < 1e-14
     v0, b0 = ext.perm_dd_certified(np.zeros((3, 0, 0), complex))
     assert np.all(v0 == 1) and np.all(b0 == 0)
+
+
+# ------------------------------------------------------- tuned chunk shape
+@pytest.mark.parametrize("family,n,k", [(0, 20, 6), (0, 20, 13), (0, 24, 12), (1, 16, 3), (1, 18, 10)])
+def test_perm_split_entry_changes_only_rounding(ext, oracle, family, n, k):
+    """Any entry of the chunk-shape table gives the permanent to rounding, with
+    the family's tiers still bit-identical (fp64 = kappa value = certified value;
+    dd = certified-dd value) and the certificates enclosing the oracle."""
+    import ctypes
+    from gbskernels_amd import _lib
+    L = _lib.lib()
+    print(f"perm split family={family} n={n} k={k}", flush=True)
+    A = rng_complex((3, n, n), 1200 + n, 0.5)
+    ref = oracle.perm_dd(A)
+    old = ctypes.c_int()
+    assert L.gbs_perm_get_lane_log2(family, n, ctypes.byref(old)) == 0
+    base = ext.perm(A) if family == 0 else ext.perm_dd(A)
+    try:
+        assert L.gbs_perm_set_lane_log2(family, n, k) == 0
+        if family == 0:
+            v = ext.perm(A)
+            assert np.max(rel_err(v, ref)) <= 1e-11 and np.max(rel_err(base, ref)) <= 1e-11
+            kv, _ = ext.perm_kappa(A)
+            cv, cb = ext.perm_certified(A)
+            assert np.array_equal(kv, v) and np.array_equal(cv, v)
+            assert np.all(np.abs(cv - ref) <= cb)
+        else:
+            v = ext.perm_dd(A)
+            assert np.max(rel_err(v, ref)) <= 4 * EPS
+            cv, cb = ext.perm_dd_certified(A)
+            assert np.array_equal(cv, v)
+            assert np.all(np.abs(cv - oracle.perm_q(A)) <= cb + 0.0)
+    finally:
+        assert L.gbs_perm_set_lane_log2(family, n, old.value) == 0
+    # back on the built-in entry: the built-in values again, bit for bit
+    assert np.array_equal(ext.perm(A) if family == 0 else ext.perm_dd(A), base)
