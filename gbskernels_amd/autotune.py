@@ -14,12 +14,16 @@ on the current GPU at two batches, because the split may depend on n only
 * one matrix alone, the served case (a request stream rarely holds two large
   permanents of one size in a batch), where large chunks leave the device idle.
 
-A candidate is admissible when its one-matrix time stays within
-max(``--latency-ms`` (default 0.25 ms), 1.25 x the current entry's one-matrix
-time): a retune may not make a lone request noticeably slower. Among the
-admissible candidates the fastest batch time wins, and it must beat the current
-entry by ``--margin`` (default 2%) to replace it, so that timing noise does not
-churn the table (and the last bits of the values, which depend on the split).
+Lone-matrix times are compared above a floor ``--latency-ms`` (default
+0.25 ms; shorter ones count as equal: a served batch takes milliseconds). A
+candidate is admissible when its lone-matrix time stays within 1.25x the
+built-in default's (2^9 terms per lane for fp64, 2^6 for dd): tuning may not
+make a lone request noticeably slower. Among the admissible candidates whose
+batch time is within ``--margin`` (default 2%) of the best, the one with the
+fastest lone matrix wins (then the fastest batch); the current entry stays
+unless it is outside that band or its lone-matrix time is 1.25x the winner's,
+so that timing noise does not churn the table (and the last bits of the values,
+which depend on the split).
 
     python -m gbskernels_amd.autotune              # measure and print
     python -m gbskernels_amd.autotune --write      # ... and rewrite perm_tune.hpp
@@ -41,6 +45,7 @@ import numpy as np
 TUNE_HEADER = Path(__file__).resolve().parent / "csrc" / "perm_tune.hpp"
 FAMILIES = {0: ("fp64", 0), 1: ("dd", 1)}  # family -> (name, tier code for gbs_eval_device)
 LOGMIN = {0: 4, 1: 2}
+DEFAULT = {0: 9, 1: 6}  # the untuned entries (log2 Gray terms per lane)
 
 
 def batch_for(n: int, family: int) -> int:
@@ -72,7 +77,7 @@ def _set(L, family: int, n: int, k: int) -> None:
 
 def candidates(L, family: int, n: int, max_split_log2: int = 12) -> tuple[int, list[int]]:
     """(one-workgroup limit, candidate k values) for side n: the limit down to
-    2^max_split_log2 workgroups per matrix, and the current entry."""
+    2^max_split_log2 workgroups per matrix, the current entry and the default."""
     old = _entry(L, family, n)
     _set(L, family, n, 27)
     lim = _split(L, family, n)[0]
@@ -80,7 +85,7 @@ def candidates(L, family: int, n: int, max_split_log2: int = 12) -> tuple[int, l
     if lim < LOGMIN[family]:
         return lim, []
     lo = max(LOGMIN[family], lim - max_split_log2)
-    return lim, sorted(set(range(lo, lim + 1)) | {min(old, lim)})
+    return lim, sorted(set(range(lo, lim + 1)) | {min(old, lim), min(DEFAULT[family], lim)})
 
 
 def measure(L, torch, family: int, n: int, ks: list[int], reps: int, B: int) -> dict[int, float]:
@@ -124,12 +129,19 @@ def measure(L, torch, family: int, n: int, ks: list[int], reps: int, B: int) -> 
     return times
 
 
-def choose(tb: dict[int, float], t1: dict[int, float], cur: int, latency_ms: float, margin: float) -> int:
-    """the admissible candidate with the fastest batch time (see the module text)"""
-    cap = max(latency_ms, 1.25 * t1[cur])
-    ok = [k for k in tb if t1[k] <= cap]
-    best = min(ok, key=tb.get)
-    return best if tb[best] < (1.0 - margin) * tb[cur] else cur
+def choose(tb: dict[int, float], t1: dict[int, float], cur: int, ref: int, latency_ms: float,
+           margin: float) -> int:
+    """batch-time band of the admissible candidates, fastest lone matrix in it
+    (see the module text); ref = the default entry, cur = the current one"""
+    lat = {k: max(t, latency_ms) for k, t in t1.items()}  # below the floor: a tie
+    cap = 1.25 * lat[ref]
+    ok = [k for k in tb if lat[k] <= cap]
+    bmin = min(tb[k] for k in ok)
+    near = [k for k in ok if tb[k] <= (1.0 + margin) * bmin]
+    pick = min(near, key=lambda k: (lat[k], tb[k]))
+    if cur in near and lat[cur] <= 1.25 * lat[pick]:
+        return cur
+    return pick
 
 
 def write_table(table: dict[int, dict[int, int]], path: Path = TUNE_HEADER, out: Path | None = None) -> None:
@@ -177,14 +189,14 @@ def main(argv=None) -> int:
             if len(ks) < 2:
                 continue
             entry = _entry(L, family, n)
-            cur = min(entry, lim)
+            cur, ref = min(entry, lim), min(DEFAULT[family], lim)
             # at n >= 10 a wave holds one matrix: skip splits that leave the
             # device with less than one wave per SIMD at the batch (they cannot win)
             B = batch_for(n, family)
-            ks = [k for k in ks if k == cur or B * 2 ** (lim - k) >= 1024]
+            ks = [k for k in ks if k in (cur, ref) or B * 2 ** (lim - k) >= 1024]
             tb = measure(L, torch, family, n, ks, args.reps, B)
             t1 = measure(L, torch, family, n, ks, args.reps, 1)
-            pick = choose(tb, t1, cur, args.latency_ms, args.margin)
+            pick = choose(tb, t1, cur, ref, args.latency_ms, args.margin)
             chosen.setdefault(family, {})[n] = entry if pick == cur else pick
             cells = ", ".join(f"{k}: {tb[k]:.3f} / {t1[k]:.3f} ({2 ** (lim - k)})" for k in ks)
             print(f"| {FAMILIES[family][0]} | {n} | {B} | {cur} | {lim} | {cells} | {pick} | "

@@ -136,12 +136,13 @@ __device__ inline void glynn_range_fp64(const cplx* __restrict__ AT,  // LDS, AT
       if (CERT) E = cert_round(acc, E + ep);
     }
   } else {
-    const uint32_t nblocks = nsteps >> U;
-    for (uint32_t jb = 0; jb < nblocks; ++jb) {
-#pragma unroll
-      for (int t = 1; t <= (1 << U); ++t) {
+    // blocks of 2^U steps: steps t < 2^U flip hot column ctz(t) + 1, step 2^U a
+    // cold column. The last block stops before its cold step (j = 2^M lies
+    // outside the range); it is peeled so that the loop body has no exit test
+    // (a data-dependent break there cost ~60 registers and half the occupancy)
+    const uint32_t nblocks = nsteps >> U;  // >= 1: M >= U
+    auto body = [&](uint32_t jb, int t) __attribute__((always_inline)) {
         const uint32_t j = (jb << U) + (uint32_t)t;
-        if (t == (1 << U) && jb + 1 == nblocks) break;  // j == 2^M: outside the range
         const uint32_t i = start + j;
         int k;
         if (t < (1 << U)) {
@@ -166,8 +167,13 @@ __device__ inline void glynn_range_fp64(const cplx* __restrict__ AT,  // LDS, AT
         acc = (t & 1) ? c_sub(acc, p) : c_add(acc, p);
         if (KAPPA) aacc += c_abs(p);
         if (CERT) E = cert_round(acc, E + ep);
-      }
+    };
+    for (uint32_t jb = 0; jb + 1 < nblocks; ++jb) {
+#pragma unroll
+      for (int t = 1; t <= (1 << U); ++t) body(jb, t);
     }
+#pragma unroll
+    for (int t = 1; t < (1 << U); ++t) body(nblocks - 1, t);
   }
   acc_out = acc;
   abs_out = CERT ? E : aacc;
@@ -321,8 +327,11 @@ __device__ inline double perm_ddcert_depth(int N, int M, int W) {
 //   out      : final values (W == 1) or chunk partials [B][W] (W > 1)
 //   aux      : kappa sum|term| (final or partial), may be null in plain mode
 //   M, W     : the run-time split (log2 terms per lane, workgroups per matrix)
+// At least two waves per SIMD: without the hint the allocator takes 256 VGPRs
+// plus a few AGPRs at n = 15..25 (one wave per SIMD); with it n = 14..18 fit in
+// 202..250 VGPRs and n = 19..25 spill 1..6 VGPRs.
 template <int N, int MODE>
-__global__ __launch_bounds__(64) void perm_kernel(const cplx* __restrict__ mats, int B, cplx* __restrict__ out,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void perm_kernel(const cplx* __restrict__ mats, int B, cplx* __restrict__ out,
                                                   double* __restrict__ aux, int g8, int M, int W) {
   using C = PermCfg<N, perm_is_dd(MODE) ? 1 : 0>;
   constexpr int LPM = C::LPM, MPW = 64 / LPM;

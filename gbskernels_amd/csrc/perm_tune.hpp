@@ -5,16 +5,16 @@
 // a matrix spreads over 2^(limit - value) workgroups. gbskernels_amd/autotune.py
 // measures the candidates on the device and rewrites this table; these entries
 // were measured on one MI355X (gfx950) with `python -m gbskernels_amd.autotune
-// --write` (fp64 n = 28 and dd n = 24..28 moved off the defaults 9 / 6).
+// --write` (fp64 n = 28 and dd n = 14, 24..28 moved off the defaults 9 / 6).
 #pragma once
 
 namespace gbsk {
 
 constexpr int kPermLaneLog2[2][29] = {
     // fp64 family
-    {9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 11},
+    {9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 9, 10},
     // dd family
-    {6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 7, 8, 9, 11, 12},
+    {6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 6, 7, 6, 6, 6, 6, 6, 6, 6, 6, 6, 7, 8, 9, 9, 11},
 };
 
 }  // namespace gbsk
