@@ -79,6 +79,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--haf", type=int, default=1, help="also time haf/lhaf buckets at N=16,24 (0 skips)")
+    ap.add_argument("--perm-tiers", type=int, default=1,
+                    help="also time the permanent's kappa, certified and certified-dd tiers at n = 16, 20, 24 (0 skips)")
     ap.add_argument("--tor", type=int, default=1,
                     help="also time the torontonian (fp64, certified, certified-dd) at nm = 8, 12, 16, 20 (0 skips)")
     ap.add_argument("--repeated", type=int, default=1,
@@ -206,6 +208,8 @@ def main() -> int:
             print(f"bench: {name} failed: {exc!r}", file=sys.stderr, flush=True)
             return {"error": repr(exc)}
 
+    if args.perm_tiers:
+        per_kernel.update(side("perm tiers n=16,20,24", lambda: perm_tiers(L, torch, dev, stream)))
     if args.haf:
         per_kernel.update(side("haf/lhaf n=16,24", lambda: haf_kernels(L, torch, dev, stream)))
     if args.tor:
@@ -460,6 +464,45 @@ def accuracy_bench():
             row.update(tiers(func, mats, exact))
             cancelling.append(row)
     return {"dps": 80, "oracle": "gbskernels_amd ref tier (mpmath)", "physical": physical, "cancelling": cancelling}
+
+
+def perm_tiers(L, torch, dev, stream):
+    """The permanent's other tiers (fp64 and dd are the headline's): kappa,
+    certified and certified-dd kernels at n = 16 (B = 4096), 20 (B = 512) and
+    24 (B = 64), config-2 style rows of a Haar unitary, kernel time by HIP
+    events on the launch stream (mean of 2 after a warm-up)."""
+    from gbskernels_amd.workloads import haar_unitary, perm_rows
+    out = {}
+    for n, B in ((16, 4096), (20, 512), (24, 64)):
+        rng = np.random.default_rng(40 + n)
+        U = haar_unitary(64, rng)
+        m = np.stack([perm_rows(U, n, rng) for _ in range(B)])
+        d_in = torch.from_numpy(m.view(np.float64)).to(dev)
+        d_out = torch.empty(2 * B, dtype=torch.float64, device=dev)
+        d_aux = torch.empty(B, dtype=torch.float64, device=dev)
+        for tname, tier in (("kappa", 2), ("certified", 3), ("dd_certified", 4)):
+            wsb = int(L.gbs_workspace_bytes(0, tier, n, B))
+            ws = torch.empty(max(wsb, 256), dtype=torch.uint8, device=dev)
+
+            def go():
+                rc = L.gbs_eval_device(0, tier, d_in.data_ptr(), n, B, d_out.data_ptr(), d_aux.data_ptr(),
+                                       ws.data_ptr(), ws.numel(), stream.cuda_stream)
+                if rc:
+                    raise RuntimeError(f"gbs_eval_device rc={rc}")
+            go()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            go()
+            go()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / 2
+            row = {"batch": B, "avg_ms": ms, "evals_per_s": B / (ms * 1e-3)}
+            if tier >= 3:
+                row["certified_finite"] = float(np.mean(np.isfinite(d_aux.cpu().numpy())))
+            out[f"perm_{tname}_n{n}"] = row
+    return out
 
 
 def haf_kernels(L, torch, dev, stream):

@@ -220,6 +220,13 @@ __device__ inline void glynn_range_dd(const cplx* __restrict__ AT, uint32_t star
   acc_out = acc;
 }
 
+// Compiler barrier on a bound accumulator (emits no instruction). Without it
+// LLVM sinks each operation's error computation to where the bound is next
+// read, keeping three intermediates per row sum alive across the whole step:
+// the certified-dd walk then spilled 900-1800 VGPRs to scratch (40-120x the dd
+// walk's time); with it the bound is folded in as the values are formed.
+__device__ __forceinline__ void pin(double& v) { asm volatile("" : "+v"(v)); }
+
 // glynn_range_dd with a running bound (certified double-double). The values
 // are glynn_range_dd's bit for bit. E bounds the error of every row sum (each
 // row receives the same updates, so one bound serves all rows: each step adds
@@ -242,6 +249,7 @@ __device__ inline void glynn_range_ddcert(const cplx* __restrict__ AT, uint32_t 
       double e;
       rs[r] = cddc_add_scaled(rs[r], s, AT[c * N + r], &e);
       m = fmax(m, e);
+      pin(m);
     }
     E += m;
   }
@@ -253,6 +261,7 @@ __device__ inline void glynn_range_ddcert(const cplx* __restrict__ AT, uint32_t 
       double eo;
       const ddc t = cddc_mul(t0, rs[r], &eo);
       e0 = cddc_mul_bound(t0, e0, rs[r], E, eo);
+      pin(e0);
       t0 = t;
     }
     if constexpr (N == 1) {
@@ -266,6 +275,7 @@ __device__ inline void glynn_range_ddcert(const cplx* __restrict__ AT, uint32_t 
       double eo;
       const ddc t = cddc_mul(t1, rs[r], &eo);
       e1 = cddc_mul_bound(t1, e1, rs[r], E, eo);
+      pin(e1);
       t1 = t;
     }
     double eo;
@@ -287,6 +297,7 @@ __device__ inline void glynn_range_ddcert(const cplx* __restrict__ AT, uint32_t 
       double e;
       rs[r] = cddc_add_scaled(rs[r], step, col[r], &e);
       m = fmax(m, e);
+      pin(m);
     }
     E += m;
     double ep, eo;
@@ -327,11 +338,16 @@ __device__ inline double perm_ddcert_depth(int N, int M, int W) {
 //   out      : final values (W == 1) or chunk partials [B][W] (W > 1)
 //   aux      : kappa sum|term| (final or partial), may be null in plain mode
 //   M, W     : the run-time split (log2 terms per lane, workgroups per matrix)
-// At least two waves per SIMD: without the hint the allocator takes 256 VGPRs
-// plus a few AGPRs at n = 15..25 (one wave per SIMD); with it n = 14..18 fit in
-// 202..250 VGPRs and n = 19..25 spill 1..6 VGPRs.
+// fp64, kappa and dd ask for at least two waves per SIMD: without the hint the
+// fp64 kernel takes 256 VGPRs plus a few AGPRs at n = 15..25 (one wave per
+// SIMD); with it n = 14..18 fit in 202..250 VGPRs and n = 19..25 spill 1..6
+// VGPRs. The certified modes carry a bound per row sum and would spill dozens
+// of VGPRs to scratch: they keep one wave per SIMD and AGPR spill space.
+template <int MODE>
+constexpr int perm_min_waves() { return (MODE == PERM_CERT || MODE == PERM_DDCERT) ? 1 : 2; }
+
 template <int N, int MODE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void perm_kernel(const cplx* __restrict__ mats, int B, cplx* __restrict__ out,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(perm_min_waves<MODE>()))) void perm_kernel(const cplx* __restrict__ mats, int B, cplx* __restrict__ out,
                                                   double* __restrict__ aux, int g8, int M, int W) {
   using C = PermCfg<N, perm_is_dd(MODE) ? 1 : 0>;
   constexpr int LPM = C::LPM, MPW = 64 / LPM;
