@@ -271,3 +271,24 @@ def test_perm_split_entry_changes_only_rounding(ext, oracle, family, n, k):
         assert L.gbs_perm_set_lane_log2(family, n, old.value) == 0
     # back on the built-in entry: the built-in values again, bit for bit
     assert np.array_equal(ext.perm(A) if family == 0 else ext.perm_dd(A), base)
+
+
+@pytest.mark.parametrize("n", [14, 24, 26, 28])
+def test_perm_largest_sizes_closed_forms(ext, n):
+    """The sizes whose chunk shape the tuned table moved (fp64 n = 28, dd n = 14
+    and 24..28) and their neighbours, against closed forms: a generalised
+    permutation matrix P D has perm = prod(d) (every Glynn term has that
+    magnitude), and perm(J_n) = n! in double-double (terms up to n^n, exact
+    value n! to dd resolution)."""
+    print(f"perm largest sizes n={n}", flush=True)
+    g = np.random.default_rng(3000 + n)
+    d = np.exp(1j * g.uniform(0, 2 * np.pi, n)) * g.uniform(0.8, 1.25, n)
+    PD = np.zeros((n, n), complex)
+    PD[np.arange(n), g.permutation(n)] = d
+    want = np.prod(d)
+    for v in (ext.perm(PD[None])[0], ext.perm_dd(PD[None])[0]):
+        assert abs(v - want) <= 1e-12 * abs(want)
+    v, b = ext.perm_certified(PD[None])
+    assert abs(v[0] - want) <= b[0] + 1e-14 * abs(want)  # want: numpy rounding
+    f = float(math.factorial(n))
+    assert abs(ext.perm_dd(np.ones((1, n, n)))[0] - f) <= 4 * EPS * f
