@@ -177,7 +177,7 @@ struct RCERT {
   static constexpr bool CERT = true;
   static constexpr bool ISD = true;
   static constexpr bool SYM = true;
-  static constexpr bool TASK_REGS = false;
+  static constexpr bool TASK_REGS = true;
   __device__ static T zero() { return cv{0.0, 0.0}; }
   __device__ static T one() { return cv{1.0, 0.0}; }
   __device__ static T one_minus(cplx o) {
