@@ -70,7 +70,7 @@ struct RDD {
   static constexpr bool CERT = false;
   static constexpr bool ISD = true;
   static constexpr bool SYM = true;
-  static constexpr bool TASK_REGS = false;
+  static constexpr bool TASK_REGS = true;
   __device__ static T zero() { return dd{0.0, 0.0}; }
   __device__ static T one() { return dd{1.0, 0.0}; }
   __device__ static T from(double x) { return dd{x, 0.0}; }
@@ -259,7 +259,7 @@ struct RCDD {
   static constexpr bool CERT = true;
   static constexpr bool ISD = true;
   static constexpr bool SYM = true;
-  static constexpr bool TASK_REGS = false;
+  static constexpr bool TASK_REGS = true;
   __device__ static T zero() { return cdd{dd{0.0, 0.0}, 0.0}; }
   __device__ static T one() { return cdd{dd{1.0, 0.0}, 0.0}; }
   __device__ static T one_minus(cplx o) { return cdd{two_sum(1.0, -o.re), 0.0}; }  // exact
