@@ -22,6 +22,8 @@
 //    fully unrolled prefix-Cholesky (reference algorithm) whose factor lives in
 //    VGPRs.
 // Partials: lanes -> butterfly -> one partial per workgroup -> fixed-order sum.
+#include <type_traits>
+
 #include "cert.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
@@ -550,8 +552,15 @@ __device__ inline void drop_first_mode(const T* X, int Dx, T* Y) {
 // workgroup = (instance b, leading pattern p1); out partials [B][2^T1] (T)
 // RERUN: only workgroups whose partial is NaN (a real-path breakdown or a
 // complex O) run, overwriting it (the complex fallback of the fp64 tier).
+// The fp64 policy asks for three waves per SIMD: at R = 5 that is 168 VGPRs and
+// 16 spilled (the default allocation took 186-190, two waves), measured 1.2x
+// faster at 12-20 modes; four waves helped at 12-16 modes but lost at 20, and
+// the dd policies gained nothing from a hint.
+template <class Ar>
+constexpr int tor_min_waves() { return std::is_same<Ar, RF64>::value ? 3 : 1; }
+
 template <class Ar, int R, int T2, bool KAPPA, bool RERUN>
-__global__ __launch_bounds__(64) void tor_real_kernel(const cplx* __restrict__ mats, int nm, int T1, int B,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(tor_min_waves<Ar>()))) void tor_real_kernel(const cplx* __restrict__ mats, int nm, int T1, int B,
                                                       typename Ar::P* __restrict__ part,
                                                       double* __restrict__ part_abs) {
   using T = typename Ar::T;
