@@ -108,6 +108,15 @@ def main() -> int:
               "(instances are independent; see DESIGN.md for how the path shards)", file=sys.stderr)
         return 2
 
+    # one hardware queue per server stream (4 slots x 4 streams): HIP's default
+    # of 4 queues makes 16 streams share them, and a light bucket then waits
+    # behind a heavy kernel of another slot (sampler stream p50 18 ms -> 6 ms)
+    try:
+        hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+    except ValueError:
+        hwq = 4
+    if hwq < 16:
+        os.environ["GPU_MAX_HW_QUEUES"] = "16"
     import torch
     from gbskernels_amd import _lib
 
