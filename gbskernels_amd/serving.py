@@ -18,6 +18,8 @@ bit-identical to evaluating the same item alone through the uniform API.
 from __future__ import annotations
 
 import ctypes
+import os
+import warnings
 from dataclasses import dataclass
 from typing import Sequence
 
@@ -152,10 +154,26 @@ class Server:
     batches overlap on the device; each bucket completes on its own event.
     ``auto`` requests run with kappa and are re-evaluated in double-double when
     ``sum|term| / |value| >= kappa_max``.
+
+    The process should have at least ``slots * streams_per_slot`` HIP hardware
+    queues (``GPU_MAX_HW_QUEUES``, read when HIP starts; its default is 4):
+    streams sharing a queue run in order, so a light bucket waits behind a heavy
+    kernel of another slot. A warning says so once per process.
     """
+
+    _queue_warned = False
 
     def __init__(self, budget_bytes: int = 0, slots: int = 4, streams_per_slot: int = 4, max_batch: int = 0,
                  kappa_max: float = 1e8):
+        try:
+            hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+        except ValueError:
+            hwq = 4
+        if hwq < slots * streams_per_slot and not Server._queue_warned:
+            Server._queue_warned = True
+            warnings.warn(f"GPU_MAX_HW_QUEUES={hwq} < {slots * streams_per_slot} server streams: streams share "
+                          "hardware queues and light requests can wait behind heavy kernels; set it before the "
+                          "first HIP call", RuntimeWarning, stacklevel=2)
         cfg = _lib.ServerConfig(budget_bytes, slots, streams_per_slot, max_batch, kappa_max)
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().gbs_server_create(ctypes.byref(h), ctypes.byref(cfg)), "gbs_server_create")
