@@ -95,7 +95,7 @@ def main() -> int:
     ap.add_argument("--served-requests", type=int, default=40000,
                     help="requests of the served mixed-size workload (0 skips)")
     ap.add_argument("--served-batch", type=int, default=2048, help="requests per batch / window")
-    ap.add_argument("--config3-requests", type=int, default=100000,
+    ap.add_argument("--config3-requests", type=int, default=200000,
                     help="ragged haf/lhaf requests through the server (0 skips)")
     ap.add_argument("--config5-requests", type=int, default=40000,
                     help="sampler-stream requests (Poisson 50k req/s) through the server (0 skips)")
