@@ -7,16 +7,18 @@
 //
 // MI355X mapping (this file's own design, not the reference's one-thread-per-
 // matrix): the 2^(n-1) term space of one matrix is cut into contiguous,
-// 2^M-aligned Gray ranges, one per lane; for n >= 20 the ranges of one matrix
-// also span several workgroups ("chunks"). Because every range starts at a
+// 2^M-aligned Gray ranges, one per lane; for large n the ranges of one matrix
+// also span several workgroups ("chunks"; 2^M comes from a per-size table,
+// perm_tune.hpp). Because every range starts at a
 // multiple of 2^M, the flipped column ctz(i) of a lane's j-th step is ctz(j) for
 // ALL lanes: the whole wave walks the same column sequence in lockstep. The row
 // sums live in VGPRs (fully unrolled over a compile-time n); columns 1..HOT
 // (flipped every 2nd, 4th, ... step) are cached in VGPRs, the rarer columns are
 // read from an LDS copy of A^T with a wave-uniform address (a broadcast read).
 // Lane partials are combined by a butterfly over the group's lanes, chunk
-// partials by a fixed-order second pass. The chunking depends only on n, so a
-// matrix's value never depends on what else is in the batch.
+// partials by a fixed-order second pass. The chunking depends only on n and
+// the tier family, so a matrix's value never depends on what else is in the
+// batch.
 #include "cert.hpp"
 #include "common.hpp"
 #include "perm.hpp"
