@@ -560,15 +560,14 @@ template <class Ar>
 constexpr int tor_min_waves() { return std::is_same<Ar, RF64>::value ? 3 : 1; }
 
 template <class Ar, int R, int T2, bool KAPPA, bool RERUN>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(tor_min_waves<Ar>()))) void tor_real_kernel(const cplx* __restrict__ mats, int nm, int T1, int B,
-                                                      typename Ar::P* __restrict__ part,
-                                                      double* __restrict__ part_abs) {
+__device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int nm, int T1, int B,
+                                              typename Ar::P* __restrict__ part, double* __restrict__ part_abs,
+                                              int64_t wg) {
   using T = typename Ar::T;
   extern __shared__ double lds_raw[];
   T* lds = reinterpret_cast<T*>(lds_raw);  // split: T1 (argument) + T2 + R = nm
   const int dim = 2 * nm;
   const int lane = threadIdx.x;
-  const int64_t wg = blockIdx.x;
   const int64_t b = wg >> T1;
   const unsigned p1 = (unsigned)(wg & ((1 << T1) - 1));
   if constexpr (RERUN) {
@@ -801,6 +800,39 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(tor_min_wave
   }
 }
 
+// RERUN scans the partial slots with a fixed grid (at most TOR_RERUN_GRID
+// workgroups): the 64 lanes test 64 slots at once and the workgroup then
+// recomputes the NaN ones in order. When no instance fell off the real path
+// the pass is a quick scan (at 20 modes, one workgroup per slot -- 8.4M --
+// took 5 ms to find nothing).
+constexpr int64_t TOR_RERUN_GRID = 8192;
+
+template <class Ar, int R, int T2, bool KAPPA, bool RERUN>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(tor_min_waves<Ar>()))) void tor_real_kernel(
+    const cplx* __restrict__ mats, int nm, int T1, int B, typename Ar::P* __restrict__ part,
+    double* __restrict__ part_abs) {
+  if constexpr (RERUN) {
+    const int64_t total = (int64_t)B << T1;
+    for (int64_t base = (int64_t)blockIdx.x * 64; base < total; base += (int64_t)gridDim.x * 64) {
+      const int64_t mine = base + threadIdx.x;
+      bool nan = false;
+      if (mine < total) {
+        const typename Ar::P prev = part[mine];
+        nan = prev.re != prev.re;
+      }
+      unsigned long long todo = __ballot(nan);  // one wave per workgroup: uniform
+      while (todo) {
+        const int k = __builtin_ctzll(todo);
+        todo &= todo - 1;
+        __syncthreads();  // the previous slot's LDS use is over
+        tor_workgroup<Ar, R, T2, KAPPA, RERUN>(mats, nm, T1, B, part, part_abs, base + k);
+      }
+    }
+  } else {
+    tor_workgroup<Ar, R, T2, KAPPA, RERUN>(mats, nm, T1, B, part, part_abs, blockIdx.x);
+  }
+}
+
 // Reductions: one wave per instance, 4 instances per 256-thread block. Lane l
 // sums partials l, l + 64, l + 128, ... in order, then a fixed butterfly over
 // the 64 lanes; lane 0 writes. The association depends on np only, so plain,
@@ -982,7 +1014,8 @@ static int launch_tor(int nm, const TorSplit& sp, const cplx* d_in, int B, typen
                       hipStream_t st) {
   const int np = 1 << sp.T1;
   const size_t lds = tor_lds_bytes<Ar>(nm, sp);
-  const dim3 grid((unsigned)((size_t)B * np));
+  const int64_t slots = (int64_t)B * np, scan = (slots + 63) / 64;
+  const dim3 grid((unsigned)(RERUN ? (scan < TOR_RERUN_GRID ? scan : TOR_RERUN_GRID) : slots));
   if (sp.T2 == TOR_T2) {
     switch (sp.R) {
 #define TOR_CASE(RR)                                                                                             \
