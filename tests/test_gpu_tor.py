@@ -177,3 +177,26 @@ def test_tor_dd_closed_form_cancellation(ext):
         want = a / (1 - a)
         assert ext.tor_dd(O)[0].real == pytest.approx(want, rel=1e-15)
     assert np.isnan(ext.tor_dd(np.array([1.5 * np.eye(4)], complex))[0].real)
+
+
+def test_tor_fallback_scan_large_sparse(ext):
+    """A batch with more partial slots than one pass of the fallback scan covers
+    (20 modes: 512 slots per instance, 1100 instances) and three complex
+    instances spread through it: the fallback recomputes exactly those, bit for
+    bit as when each is evaluated alone, and leaves the real ones untouched."""
+    print("tor fallback scan nm=20 B=1100", flush=True)
+    rng = np.random.default_rng(77)
+    base = np.stack([threshold_O(20, rng) for _ in range(1100)]).astype(np.complex128)
+    stack = base.copy()
+    idx = [0, 600, 1099]
+    for i in idx:
+        stack[i, 0, 1] += 1e-3j
+        stack[i, 1, 0] += 1e-3j
+    got = ext.tor(stack)
+    real_only = ext.tor(base)
+    keep = np.ones(len(base), bool)
+    keep[idx] = False
+    assert np.array_equal(got[keep], real_only[keep])
+    for i in idx:
+        assert np.isfinite(got[i]) and got[i].imag != 0.0
+        assert np.array_equal(got[i:i + 1], ext.tor(stack[i:i + 1]))
