@@ -105,7 +105,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
-              "(instances are independent; see DESIGN.md for how the path shards)", file=sys.stderr)
+              "(instances are independent: several GPUs would each take a share of the batch)", file=sys.stderr)
         return 2
 
     # one hardware queue per server stream (4 slots x 4 streams): HIP's default

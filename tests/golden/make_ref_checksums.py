@@ -6,7 +6,7 @@ are regenerable from the recipe in core/bench_kernels.cu:58-72,128-147
 (oracle/ref_inputs.cpp restates it). This script reads that DATA (no reference
 code is imported or run) and writes tests/golden/ref_checksums.json.
 
-Run in the development container only (the reference is not on the GPU box):
+Run where the reference checkout is present (its files are not shipped with the tests):
     python tests/golden/make_ref_checksums.py /root/reference
 """
 import json

@@ -9,7 +9,7 @@ the pairwise sum is exactly 2048 v). The result artifacts hold the same checksum
 on an RTX 4090 and an A100. This script reads that DATA (no reference code is
 imported or run) and writes tests/golden/repeated_ab_checksums.json.
 
-Run in the development container only (the reference is not on the GPU box):
+Run where the reference checkout is present (its files are not shipped with the tests):
     python tests/golden/make_repeated_checksums.py /root/reference
 """
 import glob

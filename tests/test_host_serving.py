@@ -61,7 +61,8 @@ def test_served_mix_stream():
 
 
 def test_canonical_flop_counts():
-    """Appendix-A counts the roofline divides by (SURVEY values)."""
+    """The canonical work counts the roofline divides by (perm 2^(n-1)(8n-4), the
+    power-trace hafnian sum, the torontonian append count)."""
     import bench
     assert bench.perm_flops(16) == pytest.approx(4.06e6, rel=2e-3)
     assert bench.perm_flops(24) == pytest.approx(1.58e9, rel=2e-3)
