@@ -99,6 +99,12 @@ def main() -> int:
                     help="ragged haf/lhaf requests through the server (0 skips)")
     ap.add_argument("--config5-requests", type=int, default=40000,
                     help="sampler-stream requests (Poisson 50k req/s) through the server (0 skips)")
+    ap.add_argument("--reps", type=int, default=7, help="timed repeats per side-kernel cell (median, IQR)")
+    ap.add_argument("--cells", type=lambda t: [c for c in t.split(",") if c], default=None,
+                    help="comma-separated side-kernel cells (e.g. haf_fp64_n24,tor_fp64_nm20): time only "
+                         "these, print their record and exit (for profiler runs)")
+    ap.add_argument("--detail", default="gpurun_out/bench_detail.json",
+                    help="file receiving the full measurement record (JSON)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (a known-size kernel that "
                          "calibrates hardware byte counters) and exit")
@@ -131,6 +137,14 @@ def main() -> int:
         return 0
     stream = torch.cuda.current_stream(dev)
     sp = stream.cuda_stream
+    if args.cells:
+        out = {}
+        for gi, make in enumerate((perm_tier_cells, haf_cells, tor_cells)):
+            print(f"bench: cells {make.__name__}", file=sys.stderr, flush=True)
+            cells = [c for c in make(L, torch, dev, stream) if c.name in args.cells]
+            out.update(run_cells(cells, torch, stream, args.reps, seed=100 + gi))
+        print(json.dumps({"cells": out}))
+        return 0
 
     m16, m24 = workload()
     cases = [(16, m16), (24, m24)]
@@ -167,11 +181,15 @@ def main() -> int:
                 evs[n].append((e0, e1))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        kern_ms = {n: float(np.mean([a.elapsed_time(b) for a, b in evs[n]])) for n in evs}
-        return wall, kern_ms
+        per = {n: np.array([a.elapsed_time(b) for a, b in evs[n]]) for n in evs}
+        kern_ms = {n: float(np.mean(v)) for n, v in per.items()}
+        return wall, kern_ms, per
 
     print(f"bench: fp64 warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
-    wall, kern_ms = run_steps(0, args.steps, args.warmup)
+    wall, kern_ms, per_ms = run_steps(0, args.steps, args.warmup)
+    q = lambda v: (float(np.median(v)), [float(np.percentile(v, 25)), float(np.percentile(v, 75))])
+    kern_med = {n: q(v) for n, v in per_ms.items()}
+    step_med, step_iqr = q(sum(per_ms.values()))
     evals_per_step = sum(m.shape[0] for _, m in cases)
     value = evals_per_step * args.steps / wall
     out16 = dev_out[16].cpu().numpy().view(np.complex128).copy()
@@ -180,7 +198,7 @@ def main() -> int:
     dd_wall, dd_kern_ms, dd16, dd24 = None, None, None, None
     if args.dd_steps > 0:
         print(f"bench: dd steps={args.dd_steps}", file=sys.stderr, flush=True)
-        dd_wall, dd_kern_ms = run_steps(1, args.dd_steps, 1)
+        dd_wall, dd_kern_ms, _ = run_steps(1, args.dd_steps, 1)
         dd16 = dev_out[16].cpu().numpy().view(np.complex128).copy()
         dd24 = dev_out[24].cpu().numpy().view(np.complex128).copy()
 
@@ -201,14 +219,15 @@ def main() -> int:
     for n, m in cases:
         f = perm_flops(n) * m.shape[0]
         per_kernel[f"perm_fp64_n{n}"] = {"batch": m.shape[0], "avg_ms": kern_ms[n],
-                                         "evals_per_s": m.shape[0] / (kern_ms[n] * 1e-3),
+                                         "median_ms": kern_med[n][0], "iqr_ms": kern_med[n][1],
+                                         "reps": args.steps, "evals_per_s": m.shape[0] / (kern_ms[n] * 1e-3),
                                          "tflops": f / (kern_ms[n] * 1e-3) / 1e12}
         if dd_kern_ms:
             per_kernel[f"perm_dd_n{n}"] = {"batch": m.shape[0], "avg_ms": dd_kern_ms[n],
                                            "evals_per_s": m.shape[0] / (dd_kern_ms[n] * 1e-3)}
 
-    # the side measurements report a Python-level failure in the line instead of
-    # losing the headline figure
+    # the side measurements report a Python-level failure in the detail record
+    # instead of losing the headline figure
     def side(name, fn):
         print(f"bench: {name}", file=sys.stderr, flush=True)
         try:
@@ -217,31 +236,27 @@ def main() -> int:
             print(f"bench: {name} failed: {exc!r}", file=sys.stderr, flush=True)
             return {"error": repr(exc)}
 
+    cell_groups = []
     if args.perm_tiers:
-        per_kernel.update(side("perm tiers n=16,20,24", lambda: perm_tiers(L, torch, dev, stream)))
+        cell_groups.append(("perm tiers n=16,20,24", perm_tier_cells))
     if args.haf:
-        per_kernel.update(side("haf/lhaf n=16,24", lambda: haf_kernels(L, torch, dev, stream)))
+        cell_groups.append(("haf/lhaf n=16,24", haf_cells))
     if args.tor:
-        per_kernel.update(side("tor nm=8,12,16,20 and haf config 1", lambda: tor_kernels(L, torch, dev, stream)))
-    repeated = None
-    if args.repeated:
-        repeated = side("repeated-row sieve", repeated_bench)
-    tor_single = None
-    if args.tor_single:
-        tor_single = side("single-large torontonian", tor_single_bench)
-    sampler = None
-    if args.sampler:
-        sampler = side("resident sampler", sampler_bench)
-    served = None
-    if args.served_requests > 0:
-        served = side("served mix", lambda: served_bench(args))
-    streams = None
-    if args.config3_requests > 0 or args.config5_requests > 0:
-        streams = side("config streams", lambda: config_streams(args))
-
-    accuracy = None
-    if args.accuracy:
-        accuracy = side("accuracy vs mpmath", accuracy_bench)
+        cell_groups.append(("tor nm=8,12,16,20 and haf config 1", tor_cells))
+    for gi, (gname, make) in enumerate(cell_groups):
+        def timed_group(make=make, gi=gi):
+            cells = make(L, torch, dev, stream)
+            if args.cells:
+                cells = [c for c in cells if c.name in args.cells]
+            return run_cells(cells, torch, stream, args.reps, seed=100 + gi)
+        per_kernel.update(side(gname, timed_group))
+    repeated = side("repeated-row sieve", repeated_bench) if args.repeated else None
+    tor_single = side("single-large torontonian", tor_single_bench) if args.tor_single else None
+    sampler = side("resident sampler", sampler_bench) if args.sampler else None
+    served = side("served mix", lambda: served_bench(args)) if args.served_requests > 0 else None
+    streams = (side("config streams", lambda: config_streams(args))
+               if args.config3_requests > 0 or args.config5_requests > 0 else None)
+    accuracy = side("accuracy vs mpmath", accuracy_bench) if args.accuracy else None
 
     cpu, parity = None, None
     if not args.no_cpu:
@@ -251,28 +266,76 @@ def main() -> int:
             print(f"bench: PARITY FAILURE {parity}", file=sys.stderr)
             return 1
 
+    detail = {
+        "roofline": roofline, "cpu_baseline": cpu, "parity": parity, "per_kernel": per_kernel,
+        "dd": ({"evals_per_s": evals_per_step * args.dd_steps / dd_wall,
+                "ms_per_step": dd_wall / args.dd_steps * 1e3, "steps": args.dd_steps} if dd_wall else None),
+        "accuracy": accuracy, "served": served, "streams": streams, "repeated_ab": repeated,
+        "tor_single": tor_single, "sampler": sampler,
+    }
+    detail_path = write_detail(detail, args.detail)
+    # the detailed legs as earlier lines (one per leg), the headline last
+    for k in ("per_kernel", "accuracy", "served", "streams", "repeated_ab", "tor_single", "sampler"):
+        if detail[k] is not None:
+            print(json.dumps({"bench_detail": k, k: detail[k]}))
+
     line = {
         "metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step_median_iqr": [step_med, step_iqr[0], step_iqr[1]], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "perm Glynn fp64: 65536 x 16x16 + 1024 x 24x24 complex128 rows of a "
                                "Haar 64x64 unitary (default_rng(1))",
                    "global_batch": evals_per_step, "tier": "fp64"},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "dd": ({"evals_per_s": evals_per_step * args.dd_steps / dd_wall,
-                "ms_per_step": dd_wall / args.dd_steps * 1e3, "steps": args.dd_steps} if dd_wall else None),
-        "per_kernel": per_kernel,
-        "parity": parity,
-        "accuracy": accuracy,
-        "served": served,
-        "streams": streams,
-        "repeated_ab": repeated,
-        "tor_single": tor_single,
-        "sampler": sampler,
+        "roofline": {k: roofline[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic", "kernel",
+                                              "avg_launch_ms", "pipe")},
+        "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "value_1core", "sample_short")}
+                         if cpu else None),
+        "dd": ({"evals_per_s": round(detail["dd"]["evals_per_s"], 1)} if dd_wall else None),
+        "parity": ({k: float(f"{v:.3g}") for k, v in parity.items() if k != "samples"} if parity else None),
+        "served": served_summary(served, streams),
+        "detail": detail_path,
     }
-    print(json.dumps(line))
+    text = json.dumps(line)
+    if len(text) > 2000:  # the driver keeps a short tail: the headline stays one short line
+        line.pop("served")
+        text = json.dumps(line)
+    print(text)
     return 0
+
+
+def write_detail(detail, path):
+    """Writes the full measurement record as JSON; returns the path written, or
+    None when it cannot be written."""
+    try:
+        p = Path(path)
+        p.parent.mkdir(parents=True, exist_ok=True)
+        p.write_text(json.dumps(detail, indent=1))
+        return str(path)
+    except OSError as exc:
+        print(f"bench: cannot write {path}: {exc!r}", file=sys.stderr)
+        return None
+
+
+def served_summary(served, streams):
+    """One-line summary of the served legs: speed-up over the plain per-size
+    path with p50 / p99 of the saturated and Poisson runs, and of the
+    sampler-stream configuration."""
+    out = {}
+    if isinstance(served, dict) and "speedup_saturated_vs_plain" in served:
+        out["speedup_vs_plain"] = round(served["speedup_saturated_vs_plain"], 3)
+        out["identical_values"] = served["identical_values"]
+        for k, short in (("served_saturated", "sat"), ("served_poisson_50k", "poisson50k")):
+            r = served[k]
+            out[short] = [float(f"{r['evals_per_s']:.4g}"), round(r["p50_ms"], 3), round(r["p99_ms"], 3)]
+    if isinstance(streams, dict):
+        for k, short in (("ragged_haf_lhaf_auto", "config3"), ("sampler_stream_poisson_50k", "config5")):
+            if k in streams:
+                r = streams[k]
+                out[short] = [float(f"{r['evals_per_s']:.4g}"), round(r["p50_ms"], 3), round(r["p99_ms"], 3)]
+    if out:
+        out["fields"] = "[evals/s, p50 ms, p99 ms]"
+    return out or None
 
 
 def sieve_flops(reps) -> float:
@@ -475,83 +538,104 @@ def accuracy_bench():
     return {"dps": 80, "oracle": "gbskernels_amd ref tier (mpmath)", "physical": physical, "cancelling": cancelling}
 
 
-def perm_tiers(L, torch, dev, stream):
+class Cell:
+    """One timed kernel cell: ``go`` launches it once on ``stream``; ``meta``
+    is reported beside the timing; ``post(median_s)`` adds derived fields."""
+
+    def __init__(self, name, go, meta, post=None):
+        self.name, self.go, self.meta, self.post = name, go, meta, post
+
+
+def run_cells(cells, torch, stream, reps, seed):
+    """The reference's benchmark hygiene (docs/benchmark_protocol.md:28-36): each
+    cell is warmed up once, then timed ``reps`` times with HIP events on the
+    launch stream; the cells run in a seeded random order; median and IQR are
+    reported."""
+    order = np.random.default_rng(seed).permutation(len(cells))
+    out = {}
+    for i in order:
+        c = cells[i]
+        c.go()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            c.go()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        med = float(np.median(ms))
+        rec = dict(c.meta)
+        rec.update({"median_ms": med, "iqr_ms": [float(np.percentile(ms, 25)), float(np.percentile(ms, 75))],
+                    "reps": reps, "evals_per_s": c.meta["batch"] / (med * 1e-3)})
+        if c.post is not None:
+            rec.update(c.post(med * 1e-3))
+        out[c.name] = rec
+    return {k: out[k] for k in sorted(out)}
+
+
+def device_cell(L, torch, dev, stream, name, func, tier, m, meta_extra=None, flops=None):
+    """A cell that evaluates the stack ``m`` (B, d, d) with ``gbs_eval_device``
+    from HBM-resident buffers; ``flops`` (per evaluation) gives canonical
+    TFLOP/s; certified tiers report the finite fraction of their bounds."""
+    B, dim = m.shape[0], m.shape[1]
+    d_in = torch.from_numpy(np.ascontiguousarray(m).astype(np.complex128).view(np.float64)).to(dev)
+    d_out = torch.empty(2 * B, dtype=torch.float64, device=dev)
+    d_aux = torch.empty(B, dtype=torch.float64, device=dev)
+    wsb = int(L.gbs_workspace_bytes(func, tier, dim, B))
+    ws = torch.empty(max(wsb, 256), dtype=torch.uint8, device=dev)
+
+    def go():
+        rc = L.gbs_eval_device(func, tier, d_in.data_ptr(), dim, B, d_out.data_ptr(), d_aux.data_ptr(),
+                               ws.data_ptr(), ws.numel(), stream.cuda_stream)
+        if rc:
+            raise RuntimeError(f"gbs_eval_device rc={rc} ({name})")
+
+    def post(t):
+        r = {}
+        if flops is not None:
+            r["tflops_canonical"] = flops * B / t / 1e12
+        if tier in (3, 4):
+            r["certified_finite"] = float(np.mean(np.isfinite(d_aux.cpu().numpy())))
+        return r
+    meta = {"batch": B}
+    meta.update(meta_extra or {})
+    return Cell(name, go, meta, post)
+
+
+def perm_tier_cells(L, torch, dev, stream):
     """The permanent's other tiers (fp64 and dd are the headline's): kappa,
     certified and certified-dd kernels at n = 16 (B = 4096), 20 (B = 512) and
-    24 (B = 64), config-2 style rows of a Haar unitary, kernel time by HIP
-    events on the launch stream (mean of 2 after a warm-up)."""
+    24 (B = 64), config-2 style rows of a Haar unitary."""
     from gbskernels_amd.workloads import haar_unitary, perm_rows
-    out = {}
+    cells = []
     for n, B in ((16, 4096), (20, 512), (24, 64)):
         rng = np.random.default_rng(40 + n)
         U = haar_unitary(64, rng)
         m = np.stack([perm_rows(U, n, rng) for _ in range(B)])
-        d_in = torch.from_numpy(m.view(np.float64)).to(dev)
-        d_out = torch.empty(2 * B, dtype=torch.float64, device=dev)
-        d_aux = torch.empty(B, dtype=torch.float64, device=dev)
         for tname, tier in (("kappa", 2), ("certified", 3), ("dd_certified", 4)):
-            wsb = int(L.gbs_workspace_bytes(0, tier, n, B))
-            ws = torch.empty(max(wsb, 256), dtype=torch.uint8, device=dev)
-
-            def go():
-                rc = L.gbs_eval_device(0, tier, d_in.data_ptr(), n, B, d_out.data_ptr(), d_aux.data_ptr(),
-                                       ws.data_ptr(), ws.numel(), stream.cuda_stream)
-                if rc:
-                    raise RuntimeError(f"gbs_eval_device rc={rc}")
-            go()
-            torch.cuda.synchronize()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            go()
-            go()
-            e1.record(stream)
-            torch.cuda.synchronize()
-            ms = e0.elapsed_time(e1) / 2
-            row = {"batch": B, "avg_ms": ms, "evals_per_s": B / (ms * 1e-3)}
-            if tier >= 3:
-                row["certified_finite"] = float(np.mean(np.isfinite(d_aux.cpu().numpy())))
-            out[f"perm_{tname}_n{n}"] = row
-    return out
+            cells.append(device_cell(L, torch, dev, stream, f"perm_{tname}_n{n}", 0, tier, m,
+                                     flops=perm_flops(n) * (10 if tier == 4 else 1)))
+    return cells
 
 
-def haf_kernels(L, torch, dev, stream):
+def haf_cells(L, torch, dev, stream):
     """Uniform haf / lhaf buckets at N = 16 (B = 4096) and N = 24 (B = 256),
-    config-3 style complex symmetric submatrices, fp64 and dd, kernel time by
-    HIP events on the launch stream."""
+    config-3 style complex symmetric submatrices, fp64 and dd (canonical flops
+    x10 for dd)."""
     from gbskernels_amd.workloads import gbs_adjacency, haf_subset
     A32 = gbs_adjacency(np.random.default_rng(3))
     rng = np.random.default_rng(7)
-    out = {}
+    cells = []
     for func, code in (("haf", 1), ("lhaf", 2)):
         for N, B in ((16, 4096), (24, 256)):
             m = np.stack([haf_subset(A32, N, rng, func == "lhaf") for _ in range(B)])
-            d_in = torch.from_numpy(m.view(np.float64)).to(dev)
-            d_out = torch.empty(2 * B, dtype=torch.float64, device=dev)
-            for tier, tname, reps in ((0, "fp64", 3), (1, "dd", 1)):
-                wsb = int(L.gbs_workspace_bytes(code, tier, N, B))
-                ws = torch.empty(max(wsb, 256), dtype=torch.uint8, device=dev)
-
-                def go():
-                    rc = L.gbs_eval_device(code, tier, d_in.data_ptr(), N, B, d_out.data_ptr(), None,
-                                           ws.data_ptr(), ws.numel(), stream.cuda_stream)
-                    if rc:
-                        raise RuntimeError(f"gbs_eval_device rc={rc}")
-                go()
-                torch.cuda.synchronize()
-                ms = []
-                for _ in range(reps):
-                    e0 = torch.cuda.Event(enable_timing=True)
-                    e1 = torch.cuda.Event(enable_timing=True)
-                    e0.record(stream)
-                    go()
-                    e1.record(stream)
-                    torch.cuda.synchronize()
-                    ms.append(e0.elapsed_time(e1))
-                t = float(np.mean(ms)) * 1e-3
-                f = haf_flops(N, func == "lhaf") * B
-                out[f"{func}_{tname}_n{N}"] = {"batch": B, "avg_ms": t * 1e3, "evals_per_s": B / t,
-                                               "tflops_canonical": f / t / 1e12 * (10 if tier else 1)}
-    return out
+            for tier, tname in ((0, "fp64"), (1, "dd")):
+                cells.append(device_cell(L, torch, dev, stream, f"{func}_{tname}_n{N}", code, tier, m,
+                                         flops=haf_flops(N, func == "lhaf") * (10 if tier else 1)))
+    return cells
 
 
 def tor_flops(nm: int) -> float:
@@ -562,59 +646,26 @@ def tor_flops(nm: int) -> float:
     return float(sum(comb(nm, k) * (8 * k * k + 4) for k in range(1, nm + 1)) + 3 * 2 ** nm)
 
 
-def tor_kernels(L, torch, dev, stream):
+def tor_cells(L, torch, dev, stream):
     """Torontonian of threshold-detector O matrices (two-mode squeezed pairs,
     r ~ U[0.3, 1.0], Haar interferometer, loss 0.7), 4096 per mode count
-    (512 distinct, tiled), fp64 / certified / certified-dd; plus the haf of
-    4096 real symmetric 8x8 matrices G + G^T, default_rng(0). Kernel time by
-    HIP events on the launch stream."""
+    (512 distinct, tiled; certified-dd at 20 modes: 512), fp64 / certified /
+    certified-dd; plus the hafnian of config 1: 4096 real symmetric 8x8
+    matrices G + G^T, default_rng(0)."""
     from gbskernels_amd.workloads import threshold_O
-    out = {}
-
-    def timed(func, tier, m, reps):
-        B, dim = m.shape[0], m.shape[1]
-        d_in = torch.from_numpy(np.ascontiguousarray(m).view(np.float64)).to(dev)
-        d_out = torch.empty(2 * B, dtype=torch.float64, device=dev)
-        d_aux = torch.empty(B, dtype=torch.float64, device=dev)
-        wsb = int(L.gbs_workspace_bytes(func, tier, dim, B))
-        ws = torch.empty(max(wsb, 256), dtype=torch.uint8, device=dev)
-
-        def go():
-            rc = L.gbs_eval_device(func, tier, d_in.data_ptr(), dim, B, d_out.data_ptr(), d_aux.data_ptr(),
-                                   ws.data_ptr(), ws.numel(), stream.cuda_stream)
-            if rc:
-                raise RuntimeError(f"gbs_eval_device rc={rc}")
-        go()
-        torch.cuda.synchronize()
-        ms = []
-        for _ in range(reps):
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            go()
-            e1.record(stream)
-            torch.cuda.synchronize()
-            ms.append(e0.elapsed_time(e1))
-        aux = d_aux.cpu().numpy()
-        return float(np.mean(ms)) * 1e-3, aux
-
+    cells = []
     for nm in (8, 12, 16, 20):
         rng = np.random.default_rng(4000 + nm)
-        uniq = np.stack([threshold_O(nm, rng) for _ in range(512)]).astype(np.complex128)
+        uniq = np.stack([threshold_O(nm, rng) for _ in range(512)])
         m = np.ascontiguousarray(np.tile(uniq, (8, 1, 1)))
         for tier, tname in ((0, "fp64"), (3, "certified"), (4, "dd_certified")):
             B = m.shape[0] if (tier != 4 or nm <= 16) else 512
-            t, aux = timed(3, tier, m[:B], 1 if nm >= 16 else 3)
-            rec = {"batch": B, "avg_ms": t * 1e3, "evals_per_s": B / t,
-                   "tflops_canonical": tor_flops(nm) * B / t / 1e12 * (10 if tier == 4 else 1)}
-            if tier in (3, 4):
-                rec["certified_finite"] = float(np.mean(np.isfinite(aux)))
-            out[f"tor_{tname}_nm{nm}"] = rec
+            cells.append(device_cell(L, torch, dev, stream, f"tor_{tname}_nm{nm}", 3, tier, m[:B],
+                                     flops=tor_flops(nm) * (10 if tier == 4 else 1)))
     G = np.random.default_rng(0).standard_normal((4096, 8, 8))
-    A = (G + G.transpose(0, 2, 1)).astype(np.complex128)
-    t, _ = timed(1, 0, A, 5)
-    out["haf_fp64_config1_n8"] = {"batch": 4096, "avg_ms": t * 1e3, "evals_per_s": 4096 / t}
-    return out
+    cells.append(device_cell(L, torch, dev, stream, "haf_fp64_config1_n8", 1, 0, G + G.transpose(0, 2, 1),
+                             flops=haf_flops(8)))
+    return cells
 
 
 def served_bench(args):
@@ -790,24 +841,34 @@ def cpu_baseline(m16, m24, budget_s, out16, out24, dd16, dd24):
         parity["n16_dd_vs_oracle_dd"] = rel(dd16[s16], ref16)
         parity["n24_dd_vs_oracle_dd"] = rel(dd24[:2], ref24)
     cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+
+    def per_eval(threads, k16, budget24):
+        """per-evaluation seconds at n = 16 and n = 24 on ``threads`` threads"""
+        O.set_threads(threads)
+        O.perm(m16[:threads])  # warm
+        t = time.perf_counter()
+        O.perm(m16[:k16])
+        t16 = (time.perf_counter() - t) / k16
+        # n = 24: size the sample to the budget, at least one eval per thread
+        est24 = t16 * 2 ** 8 * 1.5
+        k24 = int(max(threads, min(256, budget24 / max(est24, 1e-9))))
+        k24 = max(threads, (k24 // threads) * threads)
+        t = time.perf_counter()
+        O.perm(m24[:k24])
+        t24 = (time.perf_counter() - t) / k24
+        return t16, t24, k24
+
+    t16, t24, k24 = per_eval(cores, 512, budget_s * 0.6)
+    s16, s24, j24 = per_eval(1, 64, budget_s * 0.2)
     O.set_threads(cores)
-    O.perm(m16[:cores])  # warm
-    t = time.perf_counter()
-    k16 = 512
-    O.perm(m16[:k16])
-    t16 = (time.perf_counter() - t) / k16
-    # n = 24: size the sample to the remaining budget, at least one eval per core
-    est24 = t16 * 2 ** 8 * 1.5
-    k24 = int(max(cores, min(256, (budget_s * 0.8) / max(est24, 1e-9))))
-    k24 = max(cores, (k24 // cores) * cores)
-    t = time.perf_counter()
-    O.perm(m24[:k24])
-    t24 = (time.perf_counter() - t) / k24
-    step_s = 65536 * t16 + 1024 * t24
-    return {"value": (65536 + 1024) / step_s, "unit": "evals/s", "cores": cores, "kind": "port",
-            "sample": f"oracle Glynn fp64 (cpu_ref restated in C++), {k16} n=16 + {k24} n=24 evals "
-                      f"of the same workload, OpenMP {cores} threads; per-eval times "
-                      f"{t16 * 1e3:.3f} ms (n=16), {t24:.3f} s (n=24) extrapolated to the step mix"}, parity
+    mix = lambda a, b: (65536 + 1024) / (65536 * a + 1024 * b)
+    return {"value": mix(t16, t24), "unit": "evals/s", "cores": cores, "kind": "port",
+            "value_1core": mix(s16, s24),
+            "sample_short": f"oracle Glynn fp64, {cores} thr: {512} n16 + {k24} n24; 1 thr: 64 n16 + {j24} n24",
+            "sample": f"oracle Glynn fp64 (cpu_ref restated in C++), {512} n=16 + {k24} n=24 evals "
+                      f"of the same workload, OpenMP {cores} threads, per-eval {t16 * 1e3:.3f} ms (n=16), "
+                      f"{t24:.3f} s (n=24); 1 thread: 64 n=16 + {j24} n=24, per-eval {s16 * 1e3:.3f} ms, "
+                      f"{s24:.3f} s; both extrapolated to the step mix"}, parity
 
 
 if __name__ == "__main__":
