@@ -44,14 +44,8 @@ TRAFFIC_N24 = 13742848
 
 
 def workload():
-    from gbskernels_amd.workloads import haar_unitary
-    rng = np.random.default_rng(1)
-    U = haar_unitary(64, rng)
-    rows16 = rng.integers(0, 64, (65536, 16))
-    m16 = np.ascontiguousarray(U[rows16][:, :, :16])
-    rows24 = rng.integers(0, 64, (1024, 24))
-    m24 = np.ascontiguousarray(U[rows24][:, :, :24])
-    return m16, m24
+    from gbskernels_amd.workloads import perm_haar_rows_workload
+    return perm_haar_rows_workload()
 
 
 def perm_flops(n: int) -> float:

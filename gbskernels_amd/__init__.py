@@ -138,69 +138,84 @@ def _auto_batched(func: str, matrices, return_diagnostics: bool):
     return values, diags
 
 
-# --- certified (reference gbskernels/__init__.py:350-478) ----------------------
+# --- certified (contract of reference gbskernels/__init__.py:350-478) ----------
+#
+# The whole batch is judged at once: one certified-fp64 launch, a vectorised
+# test of every certificate against rtol, ONE certified-dd launch for all the
+# elements the fp64 bounds cannot prove, and the mpmath ref tier only for what
+# that still leaves. Each step is triggered by a rigorous bound, never by a
+# heuristic. Diagnostic keys and messages are the reference's.
 
-def _check_certified_input(A) -> None:
+def _finite_input_or_raise(A) -> None:
     try:
-        finite = bool(np.all(np.isfinite(np.asarray(A))))
+        ok = bool(np.isfinite(np.asarray(A)).all())
     except (TypeError, ValueError):
-        finite = False
-    if not finite:
+        ok = False
+    if not ok:
         raise ValueError("certified evaluation requires finite input entries")
 
 
-def _finite_certificate(v: complex, e: float) -> bool:
-    return bool(np.isfinite(v.real) and np.isfinite(v.imag) and np.isfinite(e) and e >= 0.0)
+class _Certificates:
+    """Values and bounds of one certified launch with, per element, whether the
+    certificate is usable (finite value, finite non-negative bound) and its
+    relative bound (+inf when unusable; 0 / 0 counts as 0)."""
+
+    def __init__(self, values, bounds) -> None:
+        self.values = np.array(values, dtype=np.complex128)
+        self.bounds = np.array(bounds, dtype=np.float64)
+        v, b = self.values, self.bounds
+        self.usable = np.isfinite(v.real) & np.isfinite(v.imag) & np.isfinite(b) & (b >= 0.0)
+        mag = np.abs(v)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            rel = np.where(mag > 0.0, b / np.where(mag > 0.0, mag, 1.0), np.where(b == 0.0, 0.0, np.inf))
+        self.rel = np.where(self.usable, rel, np.inf)
+
+    def proves(self, rtol: float) -> np.ndarray:
+        return self.usable & (self.rel <= rtol)
+
+    def diag(self, i: int, tier: str) -> dict:
+        return {"tier": tier, "abs_error_bound": float(self.bounds[i]), "rel_error_bound": float(self.rel[i])}
 
 
-def _certificate_rel(v: complex, e: float) -> float:
-    if not _finite_certificate(v, e):
-        return float("inf")
-    mag = abs(v)
-    if mag == 0.0:
-        return 0.0 if e == 0.0 else float("inf")
-    return float(e / mag)
-
-
-def _escalate(func: str, A, bound: float, valid: bool, rel: float, rtol: float):
-    """The proven ladder past certified-fp64: certified-dd, then ref."""
+def _dd_certificates(func: str, stack: np.ndarray):
+    """Certified-dd certificates of a uniform stack, or None where that tier
+    cannot take it (size beyond its kernels, complex torontonian input)."""
     try:
-        stack = _prepare(func, np.asarray(A)[None], "dd_certified")
-        v2, b2 = _run(func, "dd_certified", stack)
-        v2c, b2f = complex(v2[0]), float(b2[0])
-        rel2 = _certificate_rel(v2c, b2f)
-        if _finite_certificate(v2c, b2f) and np.isfinite(rel2) and rel2 <= rtol:
-            return v2c, {"tier": "certified-dd", "escalated": True,
-                         "abs_error_bound": b2f, "rel_error_bound": float(rel2)}
+        sub = _prepare(func, stack, "dd_certified")
     except (ValueError, NotImplementedError):
-        pass
-    ref = _ref.evaluate(func, A)
-    if not (np.isfinite(ref.real) and np.isfinite(ref.imag)):
-        raise FloatingPointError(f"reference {func} evaluator returned a non-finite value")
-    return ref, {"tier": "ref", "escalated": True,
-                 "abs_error_bound": bound if valid else float("inf"), "rel_error_bound": float(rel)}
+        return None
+    return _Certificates(*_run(func, "dd_certified", sub))
 
 
 def _certified_batched(func: str, matrices, rtol):
     stack = _prepare(func, matrices, "certified")
-    _check_certified_input(stack)
-    values, bounds = _run(func, "certified", stack)
-    out = values.astype(np.complex128, copy=True)
-    diags = []
-    for i in range(len(out)):
-        v, e = complex(out[i]), float(bounds[i])
-        valid = _finite_certificate(v, e)
-        rel = _certificate_rel(v, e)
-        if rtol is not None and (not valid or not np.isfinite(rel) or rel > rtol):
-            out[i], d = _escalate(func, stack[i], e, valid, rel, rtol)
-        elif not valid:
+    _finite_input_or_raise(stack)
+    fp = _Certificates(*_run(func, "certified", stack))
+    if rtol is None:
+        refused = np.flatnonzero(~fp.usable)
+        if refused.size:
             raise FloatingPointError(
-                f"certified {func} batch element {i} refused with a non-finite value or bound")
-        else:
-            d = {"tier": "certified-fp64", "abs_error_bound": e, "rel_error_bound": rel}
-            if rtol is not None:
-                d["escalated"] = False
-        diags.append(d)
+                f"certified {func} batch element {int(refused[0])} refused with a non-finite value or bound")
+        return fp.values, [fp.diag(i, "certified-fp64") for i in range(len(fp.values))]
+    out = fp.values.copy()
+    diags = [dict(fp.diag(i, "certified-fp64"), escalated=False) for i in range(len(out))]
+    todo = np.flatnonzero(~fp.proves(rtol))
+    if todo.size:
+        dd = _dd_certificates(func, stack[todo])
+        settled = dd.proves(rtol) if dd is not None else np.zeros(todo.size, bool)
+        for k, i in enumerate(todo):
+            if settled[k]:
+                out[i] = dd.values[k]
+                diags[i] = dict(dd.diag(k, "certified-dd"), escalated=True)
+                continue
+            ref = _ref.evaluate(func, stack[i])
+            if not (np.isfinite(ref.real) and np.isfinite(ref.imag)):
+                raise FloatingPointError(f"reference {func} evaluator returned a non-finite value")
+            out[i] = ref
+            # the diagnostics keep the fp64 certificate that forced the escalation
+            diags[i] = {"tier": "ref", "escalated": True,
+                        "abs_error_bound": float(fp.bounds[i]) if fp.usable[i] else float("inf"),
+                        "rel_error_bound": float(fp.rel[i])}
     return out, diags
 
 
@@ -245,7 +260,7 @@ def _dispatch_single(func, A, precision, backend, return_diagnostics=False, rtol
     if precision == "ref":
         return _ref.evaluate(func, A)
     if precision == "certified":
-        _check_certified_input(A)
+        _finite_input_or_raise(A)
     res = _dispatch_batched(func, np.asarray(A)[None, ...], precision, backend, return_diagnostics, rtol)
     if return_diagnostics:
         vals, diags = res
@@ -323,8 +338,8 @@ def lhaf_repeated(A: Any, gamma: Any, reps: Any, backend: str = "gpu", certified
     if g.shape != (M,):
         raise ValueError(f"lhaf_repeated: gamma must be a length-{M} vector, got {g.shape}")
     if certified:
-        _check_certified_input(A)
-        _check_certified_input(g)
+        _finite_input_or_raise(A)
+        _finite_input_or_raise(g)
     r = np.ascontiguousarray(reps, dtype=np.int32)
     single = r.ndim == 1
     if single:
@@ -351,69 +366,121 @@ def lhaf_repeated(A: Any, gamma: Any, reps: Any, backend: str = "gpu", certified
     return complex(v[0]) if single else v
 
 
-# --- single-large torontonian (reference gbskernels/__init__.py:635-691) --------
+# --- single-large torontonian (contract of reference gbskernels/__init__.py:635-691)
+
+# (certified, dd) -> extension entry point and the tier label of its certificate
+_TOR_SINGLE_ENTRY = {
+    (False, False): ("tor_single", None),
+    (True, False): ("tor_single_certified", "certified-fp64"),
+    (True, True): ("tor_single_ddcertified", "certified-dd"),
+    (False, True): ("tor_single_ddcertified", "certified-dd"),
+}
+
+
+def _single_tor_matrix(O: Any) -> np.ndarray:
+    """The binary64 matrix the single-large kernels walk: real (a complex array
+    must have zero imaginary parts), square, even, dim <= 64, finite and exactly
+    symmetric (the recursive Cholesky reads one triangle)."""
+    raw = np.asarray(O)
+    if np.iscomplexobj(raw) and np.any(raw.imag != 0):
+        raise ValueError("tor_single is real-domain only; use tor() for complex O")
+    M = np.ascontiguousarray(raw.real if np.iscomplexobj(raw) else raw, dtype=np.float64)
+    d = M.shape[0] if M.ndim == 2 else -1
+    problems = (
+        (M.ndim != 2 or M.shape[0] != M.shape[1] or d % 2 != 0,
+         f"tor_single: O must be square with even dimension, got {M.shape}"),
+        (d > 64, f"tor_single: dimension {d} exceeds the cap 64 (n <= 32 modes)"),
+        (not bool(np.isfinite(M).all()), "tor_single: O must contain only finite binary64 values"),
+        (not np.array_equal(M, M.T), "tor_single: O must be exactly symmetric after conversion to binary64"),
+    )
+    for failed, message in problems:
+        if failed:
+            raise ValueError(message)
+    return M
+
 
 def tor_single(O: Any, groups: int | None = None, certified: bool = False, dd: bool = False):
     """Torontonian of ONE real ``(2n, 2n)`` matrix, ``2n <= 64``, split across
-    the whole GPU. The matrix must be finite and exactly symmetric after
-    conversion to binary64; physical-domain inputs (``I - O_S`` SPD) only,
-    off-domain raises ``ValueError``. ``groups`` (default ``min(n, 14)``) is
-    checked for compatibility; the split is this backend's own. ``certified``
-    also returns ``{"tier", "abs_error_bound", "rel_error_bound"}``; ``dd=True``
-    (implies certified) carries the value in double-double."""
-    A = np.asarray(O)
-    if np.iscomplexobj(A):
-        if np.any(np.imag(A) != 0):
-            raise ValueError("tor_single is real-domain only; use tor() for complex O")
-        A = np.real(A)
-    M = np.ascontiguousarray(A, dtype=np.float64)
-    if M.ndim != 2 or M.shape[0] != M.shape[1] or M.shape[0] % 2 != 0:
-        raise ValueError(f"tor_single: O must be square with even dimension, got {M.shape}")
-    if M.shape[0] > 64:
-        raise ValueError(f"tor_single: dimension {M.shape[0]} exceeds the cap 64 (n <= 32 modes)")
-    if not np.all(np.isfinite(M)):
-        raise ValueError("tor_single: O must contain only finite binary64 values")
-    if not np.array_equal(M, M.T):
-        raise ValueError("tor_single: O must be exactly symmetric after conversion to binary64")
+    the whole GPU. Physical-domain inputs (``I - O_S`` SPD) only: off-domain
+    raises ``ValueError``. ``groups`` (default ``min(n, 14)``) is validated for
+    compatibility; the work split is this backend's own and the value does not
+    depend on it. ``certified`` also returns ``{"tier", "abs_error_bound",
+    "rel_error_bound"}``; ``dd=True`` (implies certified) carries the value in
+    double-double."""
+    M = _single_tor_matrix(O)
     n = M.shape[0] // 2
     g = min(n, 14) if groups is None else int(groups)
-    if not 0 <= g <= n:
+    if g < 0 or g > n:
         raise ValueError(f"groups must be in [0, {n}]")
-    ext = _ext()
-    if certified or dd:
-        tier = "certified-dd" if dd else "certified-fp64"
-        v, e = (ext.tor_single_ddcertified if dd else ext.tor_single_certified)(M, g)
-        if not (np.isfinite(v) and np.isfinite(e) and e >= 0.0):
-            raise ValueError("tor_single: off the physical domain or uncertifiable "
-                             "(the bound refuses rather than overclaims)")
-        rel = e / abs(v) if v != 0 else float("inf")
-        return v, {"tier": tier, "abs_error_bound": e, "rel_error_bound": rel}
-    v = float(ext.tor_single(M, g))
-    if not np.isfinite(v):
-        raise ValueError("tor_single: input is off the physical domain (I - O_S not SPD)")
-    return v
+    entry, tier = _TOR_SINGLE_ENTRY[(bool(certified), bool(dd))]
+    res = getattr(_ext(), entry)(M, g)
+    if tier is None:
+        value = float(res)
+        if not np.isfinite(value):
+            raise ValueError("tor_single: input is off the physical domain (I - O_S not SPD)")
+        return value
+    value, bound = float(res[0]), float(res[1])
+    if not (np.isfinite(value) and np.isfinite(bound) and bound >= 0.0):
+        raise ValueError("tor_single: off the physical domain or uncertifiable "
+                         "(the bound refuses rather than overclaims)")
+    return value, {"tier": tier, "abs_error_bound": bound,
+                   "rel_error_bound": bound / abs(value) if value != 0 else float("inf")}
 
 
-# --- Workspace (reference gbskernels/__init__.py:716-908) ----------------------
+# --- Workspace (API of reference gbskernels/__init__.py:716-908) ----------------
+
+def _square_members(matrices) -> list[np.ndarray]:
+    mats = []
+    for i, m in enumerate(matrices):
+        a = np.ascontiguousarray(np.asarray(m, dtype=np.complex128))
+        if a.ndim != 2 or a.shape[0] != a.shape[1]:
+            raise ValueError(f"Workspace: matrix #{i} must be square 2-D; got shape {a.shape}")
+        mats.append(a)
+    return mats
+
+
+def _dd_takes(func: str, A: np.ndarray) -> bool:
+    """Whether the double-double kernels cover this matrix (size; real O for tor)."""
+    try:
+        _prepare(func, A[None], "dd")
+    except ValueError:
+        return False
+    return True
+
 
 class Workspace:
-    """Bucketing, device-resident handle for ragged batches.
+    """Ragged, device-resident evaluation handle.
 
-    Groups a ragged list into one uniform launch per size (first-seen order),
-    reuses one session's grow-only device buffers across calls, and scatters
-    results back to input order; element ``i``'s value is identical to
-    evaluating it alone."""
+    A ragged call is ONE heterogeneous batch of the mixed-batch engine
+    (``gbs_batch_run``): the matrices are packed into one host buffer, copied to
+    the device in one transfer, and their per-size buckets run concurrently on
+    a pool of HIP streams; the engine's grow-only device buffers persist across
+    calls, so a sampler loop pays allocation once (:meth:`reallocs` counts the
+    growths). Device-resident outputs (``*_resident``) come from a device
+    session. Element ``i``'s value is identical to evaluating its matrix alone;
+    ``auto`` escalates each element on its own cancellation indicator.
+    ``backend="cpu"`` serves ``precision="ref"`` only."""
 
     def __init__(self, backend: str = "gpu") -> None:
         if backend not in ("cpu", "gpu"):
             raise ValueError(f"unknown backend {backend!r}; expected 'cpu' or 'gpu'")
         self._backend = backend
-        self._session = None
+        self._engine = None    # serving.Batch, opened on first use
+        self._session = None   # device session for resident outputs
         self._closed = False
 
-    def __enter__(self) -> "Workspace":
-        if self._backend == "gpu":
+    # lifecycle: handles open lazily (entering the context opens them early);
+    # close() releases both and is idempotent
+    def _open(self) -> None:
+        if self._closed:
+            raise RuntimeError("Workspace is closed")
+        if self._backend == "gpu" and self._engine is None:
+            from .serving import Batch
+            self._engine = Batch(nstreams=4, use_graphs=False)
             self._session = _ext().Session()
+
+    def __enter__(self) -> "Workspace":
+        self._open()
         return self
 
     def __exit__(self, *exc) -> bool:
@@ -421,12 +488,15 @@ class Workspace:
         return False
 
     def close(self) -> None:
-        if self._closed:
-            return
-        if self._session is not None:
-            self._session.close()
-        self._session = None
+        for h in (self._engine, self._session):
+            if h is not None:
+                h.close()
+        self._engine = self._session = None
         self._closed = True
+
+    def reallocs(self) -> int:
+        """Times the engine's device buffers grew (0 before the first call)."""
+        return 0 if self._engine is None else self._engine.stats()["device_reallocs"]
 
     def perm_batched(self, matrices, precision: str = "fp64") -> np.ndarray:
         return self._ragged("perm", matrices, precision)
@@ -443,37 +513,36 @@ class Workspace:
     def _ragged(self, func: str, matrices, precision: str) -> np.ndarray:
         if self._closed:
             raise RuntimeError("Workspace is closed")
-        mats = []
-        for i, m in enumerate(matrices):
-            a = np.ascontiguousarray(np.asarray(m, dtype=np.complex128))
-            if a.ndim != 2 or a.shape[0] != a.shape[1]:
-                raise ValueError(f"Workspace: matrix #{i} must be square 2-D; got shape {a.shape}")
-            mats.append(a)
-        out = np.empty(len(mats), dtype=np.complex128)
-        if not mats:
-            return out
-        buckets: dict[int, list[int]] = {}
-        for i, a in enumerate(mats):
-            buckets.setdefault(a.shape[0], []).append(i)
-        for d, idxs in buckets.items():
-            stack = np.ascontiguousarray(np.stack([mats[i] for i in idxs]))
-            try:
-                res = self._bucket(func, stack, precision)
-            except ValueError as e:
-                raise ValueError(f"matrix #{idxs[0]} (dim {d}): {e}") from e
-            out[idxs] = res
-        return out
-
-    def _bucket(self, func: str, stack: np.ndarray, precision: str) -> np.ndarray:
+        mats = _square_members(matrices)
         if self._backend == "cpu":
-            return _dispatch_batched(func, stack, precision, "cpu")
-        if precision == "auto":
-            return _auto_batched(func, stack, False)
-        if precision not in ("fp64", "dd"):
+            _check_args(precision, "cpu", False, None)
+            return np.array([_ref.evaluate(func, a) for a in mats], dtype=np.complex128)
+        if precision not in ("fp64", "dd", "auto"):
             raise NotImplementedError(f"Workspace precision {precision!r}: use 'fp64', 'dd' or 'auto'")
-        stack = _prepare(func, stack, precision)
-        name = func if precision == "fp64" else f"{func}_dd"
-        return np.asarray(getattr(self._session, name)(stack))
+        if not mats:
+            return np.empty(0, dtype=np.complex128)
+        run = "kappa" if precision == "auto" else precision
+        # each size is checked once, against its first member (named on failure)
+        first: dict[int, int] = {}
+        for i, a in enumerate(mats):
+            first.setdefault(a.shape[0], i)
+        for d, i in first.items():
+            try:
+                _prepare(func, mats[i][None], run)
+            except ValueError as e:
+                raise ValueError(f"matrix #{i} (dim {d}): {e}") from e
+        self._open()
+        values, aux = self._engine.run([(func, run, a) for a in mats])
+        values = values.copy()
+        if precision == "auto":
+            kappa = aux / np.maximum(np.abs(values), 1e-300)
+            risky = np.flatnonzero(kappa >= _AUTO_KAPPA_MAX)
+            dd = [i for i in risky if _dd_takes(func, mats[i])]
+            if dd:
+                values[dd] = self._engine.run([(func, "dd", mats[i]) for i in dd])[0]
+            for i in sorted(set(risky.tolist()) - set(dd)):
+                values[i] = _ref.evaluate(func, mats[i])
+        return values
 
     def perm_resident(self, matrices, precision: str = "fp64"):
         return self._resident("perm", matrices, precision)
@@ -498,11 +567,10 @@ class Workspace:
             raise NotImplementedError("device-resident output supports 'fp64' and 'dd'")
         try:
             stack = np.ascontiguousarray(np.asarray(matrices, dtype=np.complex128))
-            uniform = stack.ndim == 3
         except (ValueError, TypeError):
-            uniform = False
-        if not uniform:
+            stack = None
+        if stack is None or stack.ndim != 3:
             raise ValueError("device-resident output needs a uniform (B, d, d) batch")
         stack = _prepare(func, stack, precision)
-        name = func if precision == "fp64" else f"{func}_dd"
-        return self._session._resident(name, stack)
+        self._open()
+        return self._session._resident(func if precision == "fp64" else f"{func}_dd", stack)

@@ -29,7 +29,8 @@ GBSK_AUTO = 5  # server-only tier: kappa, re-evaluated in dd when ill-conditione
 
 
 class BatchStats(ctypes.Structure):
-    _fields_ = [(k, ctypes.c_ulonglong) for k in ("batches", "items", "buckets", "graphs_captured", "graph_replays")]
+    _fields_ = [(k, ctypes.c_ulonglong)
+                for k in ("batches", "items", "buckets", "graphs_captured", "graph_replays", "device_reallocs")]
 
 
 class ServerConfig(ctypes.Structure):
@@ -44,7 +45,8 @@ class RequestInfo(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_ulonglong)
-                for k in ("submitted", "completed", "batches", "buckets", "escalated", "max_in_flight")]
+                for k in ("submitted", "completed", "batches", "buckets", "escalated", "max_in_flight",
+                          "device_bytes_max")]
 
 _cp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -67,6 +69,7 @@ SIGNATURES = {
     "gbs_max_dim": ([_i, _i], _i),
     "gbs_error_string": ([_i], ctypes.c_char_p),
     "gbs_is_device_build": ([], _i),
+    "gbs_check_div_sqrt": ([_cp, _cp, ctypes.c_longlong, _cp, _cp, _cp], _i),
     "gbs_batch_create": ([ctypes.POINTER(_cp), ctypes.c_size_t, _i, _i], _i),
     "gbs_batch_run": ([_cp, _i, _cp, _cp, _cp, _cp, _cp, _cp, _cp], _i),
     "gbs_batch_get_stats": ([_cp, ctypes.POINTER(BatchStats)], _i),

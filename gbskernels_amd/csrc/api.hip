@@ -378,6 +378,29 @@ const char* gbs_error_string(int rc) {
 
 int gbs_is_device_build(void) { return 1; }
 
+int gbs_check_div_sqrt(const double* a, const double* b, long long n, double* q, double* s, double* h) {
+  if (n <= 0) return 0;
+  if (!a || !b || !q || !s || !h) return GBSK_ERR_ARG;
+  Ctx& c = global_ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  int rc = c.init();
+  const size_t bytes = (size_t)n * sizeof(double);
+  double* d = nullptr;  // a, b, q, s, h
+  if (!rc) rc = (int)hipMalloc(&d, 5 * bytes);
+  if (rc) return rc;
+  double *da = d, *db = d + n, *dq = d + 2 * n, *ds = d + 3 * n, *dh = d + 4 * n;
+  rc = (int)hipMemcpyAsync(da, a, bytes, hipMemcpyHostToDevice, c.stream);
+  if (!rc) rc = (int)hipMemcpyAsync(db, b, bytes, hipMemcpyHostToDevice, c.stream);
+  if (!rc) rc = tor_div_sqrt_probe(da, db, n, dq, ds, c.stream);
+  if (!rc) rc = haf_div_probe(da, n, dh, c.stream);
+  if (!rc) rc = (int)hipMemcpyAsync(q, dq, bytes, hipMemcpyDeviceToHost, c.stream);
+  if (!rc) rc = (int)hipMemcpyAsync(s, ds, bytes, hipMemcpyDeviceToHost, c.stream);
+  if (!rc) rc = (int)hipMemcpyAsync(h, dh, bytes, hipMemcpyDeviceToHost, c.stream);
+  const int rs = (int)hipStreamSynchronize(c.stream);
+  (void)hipFree(d);
+  return rc ? rc : rs;
+}
+
 int gbs_tor_recursive_single_host(const double* O, int n, int g, double* out) {
   return tor_single_host(GBSK_FP64, O, n, g, out, nullptr);
 }

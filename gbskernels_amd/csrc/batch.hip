@@ -158,6 +158,28 @@ static void drop_graphs(gbs_batch* b) {
   b->graphs.clear();
 }
 
+// device bytes a batch object holds (its grow-only buffers)
+static size_t held_bytes(const gbs_batch* b) { return b->cap_in + b->cap_res + b->cap_ws; }
+
+// frees a batch object's device buffers (its streams must be idle); cached
+// graphs refer to them and are dropped
+static void release_device(gbs_batch* b) {
+  for (void** p : {&b->d_in, &b->d_res, &b->d_ws}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  b->cap_in = b->cap_res = b->cap_ws = 0;
+  drop_graphs(b);
+}
+
+// the device bytes a batch object holds after growing its buffers to a batch
+// of in_b input, res_b result and ws_b workspace bytes (batch_submit's grow_dev
+// requests, no slack)
+static size_t footprint(const gbs_batch* b, size_t in_b, size_t res_b, size_t ws_b) {
+  return std::max(b->cap_in, in_b + 16) + std::max(b->cap_res, res_b + 16) + std::max(b->cap_ws, ws_b + 256);
+}
+
+
 // plan: buckets (longest first) with packed offsets; returns bytes needed
 // Buckets are ordered longest first (lpt: the makespan of one batch) or shortest
 // first (!lpt: mean completion time of a served batch) and each is given to the
@@ -251,7 +273,7 @@ static bool tier_has_aux(int tier) { return tier == GBSK_KAPPA || tier == GBSK_C
 
 // device bytes a batch occupies (inputs, outputs, auxiliaries, workspace)
 static size_t batch_bytes(size_t in_elems, size_t res_total, size_t ws_total) {
-  return in_elems * sizeof(cplx) + res_total + ws_total;
+  return in_elems * sizeof(cplx) + 16 + res_total + 16 + ws_total + 256;  // the buffers' requests
 }
 
 static int enqueue_device(gbs_batch* b, const std::vector<Bucket>& bks, size_t in_elems) {
@@ -325,6 +347,12 @@ static int batch_submit(gbs_batch* b, int n, const int* func, const int* tier, c
   b->res_total = res_total;
   const size_t need = batch_bytes(in_elems, res_total, ws_total);
   if (b->budget && need > b->budget) return GBSK_ERR_WORKSPACE;
+  // under a budget the grow-only buffers must not together exceed it: buffers
+  // that grew for different earlier batches are given back first
+  if (b->budget && footprint(b, in_elems * sizeof(cplx), res_total, ws_total) > b->budget) {
+    for (hipStream_t st : b->streams) (void)hipStreamSynchronize(st);
+    release_device(b);
+  }
   // grow-only buffers; any reallocation changes addresses a captured graph
   // refers to, so cached graphs are dropped
   const unsigned long long g0 = b->grows;
@@ -332,6 +360,7 @@ static int batch_submit(gbs_batch* b, int n, const int* func, const int* tier, c
   rc = grow_dev(&b->d_in, &b->cap_in, in_elems * sizeof(cplx) + 16, slack, &b->grows);
   if (!rc) rc = grow_dev(&b->d_res, &b->cap_res, res_total + 16, slack, &b->grows);
   if (!rc) rc = grow_dev(&b->d_ws, &b->cap_ws, ws_total + 256, slack, &b->grows);
+  b->stats.device_reallocs += b->grows - g0;
   if (!rc) rc = grow_pinned(&b->h_in, &b->hcap_in, in_elems + 1, slack, &b->grows);
   if (!rc) rc = grow_pinned(&b->h_res, &b->hcap_res, res_total + 16, slack, &b->grows);
   if (rc) return rc;
@@ -473,11 +502,6 @@ struct Slot {
   std::thread worker;
 };
 
-// device bytes one more item of (func, tier, dim) adds to a bucket of c items
-static size_t item_bytes(int func, int tier, int dim, int c) {
-  const size_t w1 = gbs_workspace_bytes(func, tier, dim, c + 1), w0 = c ? gbs_workspace_bytes(func, tier, dim, c) : 0;
-  return (size_t)dim * dim * sizeof(cplx) + sizeof(cplx) + sizeof(double) + (w1 > w0 ? w1 - w0 : 0) + (c ? 0 : 256 + 16);
-}
 
 }  // namespace gbsk
 
@@ -490,7 +514,6 @@ struct gbs_server {
   std::map<long long, Req*> table;
   long long next_ticket = 1;
   std::vector<Slot> slots;
-  size_t in_use = 0;
   bool stop = false;
   bool stop_workers = false;
   std::thread th;
@@ -554,10 +577,20 @@ static void complete_bucket(gbs_server* s, Slot& sl, Bucket& bk, int err, std::v
 
 // release a slot whose batch is complete or failed (s->mu held)
 static void free_slot(gbs_server* s, Slot& sl) {
+  (void)s;
   sl.state = SLOT_FREE;
-  s->in_use -= sl.reserved;
   sl.reserved = 0;
   sl.reqs.clear();
+}
+
+// Device bytes the slots other than `self` hold or will hold: a busy slot its
+// admitted footprint (its buffers grow to at most that), a free slot what its
+// buffers hold now. Called with s->mu held.
+static size_t others_bytes(const gbs_server* s, const Slot* self) {
+  size_t t = 0;
+  for (const Slot& o : s->slots)
+    if (&o != self) t += o.state == SLOT_FREE ? held_bytes(o.b) : o.reserved;
+  return t;
 }
 
 // The slot's worker: packs and queues each batch the dispatcher assigns.
@@ -632,20 +665,41 @@ static void dispatcher(gbs_server* s) {
       else
         any_busy = true;
     }
-    // 2. admission: everything pending that fits, into each free slot
+    // 2. admission: everything pending that fits, into each free slot. Under a
+    // budget, what every slot's buffers hold counts (they are grow-only), so
+    // the slots together never hold more than budget_bytes of device memory.
     for (Slot& sl : s->slots) {
       if (sl.state != SLOT_FREE || s->pending.empty()) continue;
       std::vector<Req*> take;
       std::map<long long, int> counts;
-      size_t bytes = 0;
-      const size_t avail = s->cfg.budget_bytes ? s->cfg.budget_bytes - s->in_use : (size_t)-1;
+      const size_t budget = s->cfg.budget_bytes;
+      size_t others = budget ? others_bytes(s, &sl) : 0;
+      size_t in_b = 0, res_b = 0, ws_b = 0, fp = 0;
+      bool released = false;
       while (!s->pending.empty() && (s->cfg.max_batch <= 0 || (int)take.size() < s->cfg.max_batch)) {
         Req* r = s->pending.front();
         const long long key = ((long long)r->func << 40) | ((long long)r->run_tier << 32) | (long long)r->dim;
         int& c = counts[key];
-        const size_t add = item_bytes(r->func, r->run_tier, r->dim, c);
-        if (bytes + add > avail) {
-          if (take.empty() && s->in_use == 0) {  // can never fit
+        // the plan's packing: inputs back to back, per bucket 16-byte aligned
+        // results and a 256-byte aligned workspace slice
+        const size_t w1 = gbs_workspace_bytes(r->func, r->run_tier, r->dim, c + 1);
+        const size_t w0 = c ? gbs_workspace_bytes(r->func, r->run_tier, r->dim, c) : 0;
+        const size_t a_in = in_b + (size_t)r->dim * r->dim * sizeof(cplx);
+        const size_t a_res = res_b + sizeof(cplx) + sizeof(double) + (c ? 0 : 16);
+        const size_t a_ws = ws_b + (w1 > w0 ? w1 - w0 : 0) + (c ? 0 : 256);
+        const size_t a_fp = budget ? footprint(sl.b, a_in, a_res, a_ws) : 0;
+        if (budget && others + a_fp > budget) {
+          if (take.empty() && !released) {
+            // idle slots (this one included) give their buffers back, then retry
+            for (Slot& o : s->slots)
+              if (o.state == SLOT_FREE && held_bytes(o.b)) release_device(o.b);
+            released = true;
+            others = others_bytes(s, &sl);
+            continue;
+          }
+          bool busy = false;
+          for (const Slot& o : s->slots) busy |= (&o != &sl && o.state != SLOT_FREE);
+          if (take.empty() && !busy) {  // can never fit
             s->pending.pop_front();
             r->status = GBSK_ERR_WORKSPACE;
             finish_req(r, now_ns());
@@ -655,7 +709,10 @@ static void dispatcher(gbs_server* s) {
           }
           break;
         }
-        bytes += add;
+        in_b = a_in;
+        res_b = a_res;
+        ws_b = a_ws;
+        fp = a_fp;
         ++c;
         take.push_back(r);
         s->pending.pop_front();
@@ -665,8 +722,11 @@ static void dispatcher(gbs_server* s) {
       for (Req* r : take)
         if (!r->admitted) r->admitted = t;
       sl.reqs = std::move(take);
-      sl.reserved = bytes;
-      s->in_use += bytes;
+      sl.reserved = fp;
+      if (budget) {
+        const size_t total = others + fp;
+        if (total > s->stats.device_bytes_max) s->stats.device_bytes_max = total;
+      }
       sl.state = SLOT_ASSIGNED;
       ++s->stats.batches;
       progress = true;

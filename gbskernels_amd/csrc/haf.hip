@@ -103,6 +103,23 @@ int haf_launch_multi(bool loop, int tier, const HafJob* jobs, int njobs, hipStre
   return (int)hipGetLastError();
 }
 
+// Arithmetic-model self-check: the certified hafnian charges one rounding of
+// u|result| to the Newton step's division by the index j (ArF64::div_int), which
+// holds only if that division is correctly rounded. out[i] = div_int(a[i], j_i)
+// with j_i = 1 + i mod 32, the expression the value path evaluates.
+__global__ void haf_div_probe_kernel(const double* __restrict__ a, int64_t n, double* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = ArF64::div_int(cplx{a[i], 0.0}, (int)(1 + i % 32)).re;
+}
+
+int haf_div_probe(const double* d_a, int64_t n, double* d_out, hipStream_t st) {
+  if (n <= 0) return 0;
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(haf_div_probe_kernel, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0, st, d_a,
+                     n, d_out);
+  return (int)hipGetLastError();
+}
+
 int haf_launch(bool loop, int tier, int N, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws,
                hipStream_t st) {
   if (B <= 0) return 0;

@@ -59,4 +59,10 @@ int sieve_launch_device(bool cert, const cplx* d_A, const cplx* d_g, int M, cons
                         RepPat* d_plan, int64_t* d_total, void* d_part, cplx* d_out, double* d_bound,
                         hipStream_t st);
 
+// arithmetic-model self-checks (device pointers): q = a / b and s = sqrt|a| as
+// the certified torontonian evaluates them; h = a / (1 + i mod 32) as the
+// certified hafnian's Newton step does
+int tor_div_sqrt_probe(const double* d_a, const double* d_b, int64_t n, double* d_q, double* d_s, hipStream_t st);
+int haf_div_probe(const double* d_a, int64_t n, double* d_out, hipStream_t st);
+
 }  // namespace gbsk

@@ -78,7 +78,9 @@ struct RDD {
   __device__ static T from(double x) { return dd{x, 0.0}; }
   __device__ static T one_minus(cplx o) { return two_sum(1.0, -o.re); }  // exact
   __device__ static T neg(cplx o) { return dd{-o.re, 0.0}; }
-  __device__ static bool domain(cplx o) { return o.im == 0.0; }
+  // the dd tier reads the real part only, as the reference's dd kernel does
+  // (core/torontonian_dd.cu:72); callers gate the imaginary part beforehand
+  __device__ static bool domain(cplx) { return true; }
   __device__ static double mag(T a) { return dd_to_d(a); }
   __device__ static T add(T a, T b) { return dd_add(a, b); }
   __device__ static T sub(T a, T b) { return dd_sub(a, b); }
@@ -1189,6 +1191,27 @@ size_t tor_single_workspace_bytes(int tier, int nm) {
   const TorSplit sp = tor_single_split(nm, tor_single_rmax(tier));
   const size_t el = (tier == GBSK_DD || tier == GBSK_DD_CERTIFIED) ? sizeof(dd) : sizeof(cplx);
   return ((size_t)1 << sp.T1) * (el + sizeof(double));
+}
+
+// Arithmetic-model self-check: RCERT::div and RCERT::sqrt_ charge one rounding
+// of u|result| to the division and square root they evaluate, which holds only
+// if those operations are correctly rounded (IEEE-754 round to nearest). The
+// kernel evaluates exactly those value expressions on operand pairs so that a
+// host can compare them bit for bit with correctly rounded results.
+__global__ void tor_div_sqrt_probe_kernel(const double* __restrict__ a, const double* __restrict__ b, int64_t n,
+                                          double* __restrict__ q, double* __restrict__ s) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    q[i] = RCERT::div(cv{a[i], 0.0}, cv{b[i], 0.0}).v;
+    s[i] = RCERT::sqrt_(cv{fabs(a[i]), 0.0}).v;
+  }
+}
+
+int tor_div_sqrt_probe(const double* d_a, const double* d_b, int64_t n, double* d_q, double* d_s, hipStream_t st) {
+  if (n <= 0) return 0;
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(tor_div_sqrt_probe_kernel, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0, st,
+                     d_a, d_b, n, d_q, d_s);
+  return (int)hipGetLastError();
 }
 
 int tor_single_launch(int tier, int nm, const cplx* d_in, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {

@@ -150,7 +150,9 @@ class Server:
     """Continuous-batching server (``gbs_server_*``).
 
     A dispatcher thread admits everything pending into a free batch slot when
-    its device bytes fit the budget left by batches in flight; up to ``slots``
+    its device bytes fit the budget left by the other slots (each slot's
+    grow-only buffers count, busy or idle; idle slots give theirs back when a
+    request would not fit otherwise); up to ``slots``
     batches overlap on the device; each bucket completes on its own event.
     ``auto`` requests run with kappa and are re-evaluated in double-double when
     ``sum|term| / |value| >= kappa_max``.

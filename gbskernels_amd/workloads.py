@@ -7,7 +7,8 @@ matrix definition (``sampling/gbs.py:241-275``: O = I - Sigma_x^-1 with
 Sigma_x = (cov + I) / 2, hbar = 2); the generators themselves are written here.
 
 * :func:`perm_rows` -- permanent inputs: n random rows (with repetition) of the
-  first n columns of a Haar-random 64 x 64 unitary;
+  first n columns of a Haar-random 64 x 64 unitary (:func:`perm_haar_rows_workload`:
+  the permanent benchmark's 65536 x 16x16 + 1024 x 24x24);
 * :func:`gbs_adjacency` / :func:`haf_subset` -- complex symmetric
   A = U diag(tanh r) U^T (U Haar 32 x 32, r ~ U[0.5, 1.2]) and its principal
   submatrices; loop hafnians replace the diagonal by gamma ~ CN(0, 0.3^2);
@@ -155,6 +156,20 @@ def served_mix(n_req: int, seed: int, funcs=("perm", "haf", "lhaf"), nmax: int =
         items.append((f, t, M))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
     return items, arrivals
+
+
+def perm_haar_rows_workload(take16: int | None = None, take24: int | None = None):
+    """Permanent benchmark inputs: 65536 16x16 and 1024 24x24 complex128
+    matrices, each n random rows (with repetition) of the first n columns of one
+    Haar 64x64 unitary, all drawn from default_rng(1) in that order. take16 /
+    take24 keep only the first matrices of each set (same draws)."""
+    rng = np.random.default_rng(1)
+    U = haar_unitary(64, rng)
+    rows16 = rng.integers(0, 64, (65536, 16))
+    rows24 = rng.integers(0, 64, (1024, 24))
+    m16 = np.ascontiguousarray(U[:, :16][rows16[:take16]])
+    m24 = np.ascontiguousarray(U[:, :24][rows24[:take24]])
+    return m16, m24
 
 
 def repeated_ab_base(modes: int = 3, seed: int = 0):

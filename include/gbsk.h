@@ -116,6 +116,16 @@ int gbs_max_dim(int func, int tier);
 const char* gbs_error_string(int rc);
 /* 1 when the library was compiled for a GPU (always: there is no host build) */
 int gbs_is_device_build(void);
+/* Self-check of the arithmetic model the certified fp64 tier relies on (no
+ * reference counterpart; the reference states the same assumption for its
+ * directed-rounding layer, core/certified_rounding.cuh:11-52). The certified
+ * bounds charge one rounding of u|result| to each division and square root,
+ * i.e. they hold if those are correctly rounded. For host arrays of n doubles:
+ * q[i] = a[i] / b[i] and s[i] = sqrt(|a[i]|) evaluated by the certified
+ * torontonian's own device expressions, h[i] = a[i] / (1 + i mod 32) by the
+ * certified hafnian's Newton-step division. A caller compares them bit for bit
+ * with IEEE-754 round-to-nearest results. Synchronous; allocates 5 n doubles. */
+int gbs_check_div_sqrt(const double* a, const double* b, long long n, double* q, double* s, double* h);
 
 /* ---- 5. heterogeneous batches (no reference counterpart) ------------------
  * Replaces the per-bucket loop of Workspace._ragged / _dispatch_bucket
@@ -128,12 +138,13 @@ int gbs_is_device_build(void);
  * with use_graphs the device part is captured into a hipGraph, cached by the
  * bucket signature (the multiset of (func, tier, dim, count)). Every bucket runs
  * the kernels of a uniform call, so values are bit-identical to evaluating each
- * item alone. budget_bytes (0 = no limit) caps the device bytes of one batch:
- * a larger batch returns GBSK_ERR_WORKSPACE. A handle is reentrant (calls are
+ * item alone. budget_bytes (0 = no limit) caps the device bytes the handle's
+ * buffers hold: a batch that needs more returns GBSK_ERR_WORKSPACE. A handle is reentrant (calls are
  * serialised on it); distinct handles run concurrently. */
 typedef struct gbs_batch gbs_batch;
 typedef struct {
   unsigned long long batches, items, buckets, graphs_captured, graph_replays;
+  unsigned long long device_reallocs; /* times a grow-only device buffer grew */
 } gbs_batch_stats;
 int gbs_batch_create(gbs_batch** out, size_t budget_bytes, int nstreams, int use_graphs);
 int gbs_batch_run(gbs_batch* b, int n, const int* func, const int* tier, const int* dim, const gbs_complex* in,
@@ -145,8 +156,10 @@ int gbs_batch_destroy(gbs_batch* b);
  * Requests are submitted one at a time, from any thread, while earlier ones are
  * on the device. A dispatcher thread forms a batch from everything pending
  * whenever one of `slots` batch slots is free and the batch's device bytes fit
- * in what the in-flight batches leave of budget_bytes; batches in different
- * slots overlap on the device. Each bucket's results are copied back and
+ * in what the other slots leave of budget_bytes (each slot keeps grow-only
+ * buffers, and what they hold counts; idle slots give theirs back when a
+ * request would not fit otherwise); batches in different slots overlap on the
+ * device. Each bucket's results are copied back and
  * completed as soon as its own kernels finish. Tier GBSK_AUTO evaluates with
  * kappa and re-evaluates in double-double when sum|term| / |value| >=
  * kappa_max (the reference's auto tier, gbskernels/__init__.py:240-297). The
@@ -154,7 +167,7 @@ int gbs_batch_destroy(gbs_batch* b);
 #define GBSK_AUTO 5
 typedef struct gbs_server gbs_server;
 typedef struct {
-  size_t budget_bytes;  /* device bytes admitted batches may occupy together; 0 = no limit */
+  size_t budget_bytes;  /* device bytes the slots' buffers may hold together; 0 = no limit */
   int slots;            /* batches in flight at once, 1..8 */
   int streams_per_slot; /* 1..16 */
   int max_batch;        /* requests per batch; 0 = no limit */
@@ -167,6 +180,9 @@ typedef struct {
 } gbs_request_info;
 typedef struct {
   unsigned long long submitted, completed, batches, buckets, escalated, max_in_flight;
+  /* largest device bytes the slots' buffers held together, admitted batches
+   * included (counted only under a budget; never above budget_bytes) */
+  unsigned long long device_bytes_max;
 } gbs_server_stats;
 long long gbs_now_ns(void);
 int gbs_server_create(gbs_server** out, const gbs_server_config* cfg);

@@ -6,6 +6,7 @@ import pytest
 
 import gbskernels_amd as gk
 from conftest import rel_err, rng_complex, rng_symmetric
+from physics import threshold_O
 
 pytestmark = pytest.mark.gpu
 
@@ -84,6 +85,20 @@ def test_certified_api_and_ladder(oracle, func):
     assert v == out[0] and d["abs_error_bound"] == diags[0]["abs_error_bound"]
 
 
+@pytest.mark.parametrize("func", ["perm", "haf", "lhaf", "tor"])
+def test_certified_ladder_batch_equals_per_element(func):
+    """The ladder escalates a whole batch at once (one certified-dd launch for
+    every element the fp64 bounds cannot prove); each element's value and
+    diagnostics equal those of the same element certified alone."""
+    mats = np.stack([MK[func](d) for d in (1e-1, 1e-9, 1e-3, 1e-12, 0.5, 1e-6)])
+    out, diags = getattr(gk, f"{func}_batched")(mats, precision="certified", backend="gpu",
+                                                 return_diagnostics=True, rtol=1e-10)
+    assert any(d["escalated"] for d in diags) and not all(d["escalated"] for d in diags)
+    for i in range(len(mats)):
+        v, d = getattr(gk, func)(mats[i], precision="certified", backend="gpu", return_diagnostics=True, rtol=1e-10)
+        assert v == out[i] and d == diags[i], (i, d, diags[i])
+
+
 def test_certified_refuses_nonfinite_input():
     A = np.eye(4)
     A[0, 0] = np.nan
@@ -102,17 +117,39 @@ def test_odd_loop_hafnian_is_augmented(oracle):
 def test_workspace_bucketing_and_reuse():
     mats = [rng_complex((n, n), 10 + i) for i, n in enumerate([5, 9, 5, 12, 9, 3])]
     with gk.Workspace(backend="gpu") as ws:
-        r1 = ws._session.reallocs()
+        r1 = ws.reallocs()
         small = ws.perm_batched(mats[:3])
-        r2 = ws._session.reallocs()
+        r2 = ws.reallocs()
         res = ws.perm_batched(mats)
-        r3 = ws._session.reallocs()
+        r3 = ws.reallocs()
         again = ws.perm_batched(mats[:3])
-        r4 = ws._session.reallocs()
+        r4 = ws.reallocs()
     alone = np.array([gk.perm(m, backend="gpu") for m in mats])
     assert np.array_equal(res, alone)                 # bucketing never changes values
     assert np.array_equal(small, again) and np.array_equal(small, alone[:3])
     assert r1 == 0 and r2 >= 1 and r3 >= r2 and r4 == r3   # a smaller call does not grow
+
+
+def test_workspace_ragged_mixed_sizes_every_function():
+    """One ragged call per function (sizes interleaved, odd loop hafnians, the
+    empty matrix): every value bit-identical to the uniform call on that matrix
+    alone; an oversize member is rejected naming its index."""
+    g = np.random.default_rng(41)
+    cases = {
+        "perm": [rng_complex((n, n), 60 + i) for i, n in enumerate([3, 14, 1, 9, 14, 0, 20])],
+        "haf": list(rng_symmetric(1, 4, 61)) + list(rng_symmetric(2, 10, 62)) + list(rng_symmetric(1, 16, 63)),
+        "lhaf": list(rng_symmetric(1, 5, 64)) + list(rng_symmetric(2, 8, 65)) + list(rng_symmetric(1, 7, 66)),
+        "tor": [threshold_O(nm, g).astype(complex) for nm in (2, 5, 2, 7, 3)],
+    }
+    with gk.Workspace(backend="gpu") as ws:
+        for func, mats in cases.items():
+            got = getattr(ws, f"{func}_batched")(mats)
+            alone = np.array([getattr(gk, func)(m, backend="gpu") for m in mats])
+            assert np.array_equal(got, alone), func
+        with pytest.raises(ValueError, match="matrix #1 "):
+            ws.perm_batched([rng_complex((4, 4), 1), rng_complex((30, 30), 2)])
+    with pytest.raises(RuntimeError):
+        ws.perm_batched(cases["perm"])
 
 
 def test_workspace_auto_equals_alone():

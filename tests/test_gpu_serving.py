@@ -220,3 +220,45 @@ def test_empty_batches_and_streams():
         v, a, infos = s.replay([], [])
         assert v.shape == (0,) and infos == []
     assert st["items"] == 0
+
+
+def test_server_budget_bounds_what_slots_hold():
+    """Under a budget the slots' grow-only buffers together never hold more
+    than budget_bytes, even when successive batches grow different buffers of
+    different slots (inputs for one, workspace for another); values stay
+    bit-identical to evaluating each request alone."""
+    from gbskernels_amd.serving import Server
+    budget = 3 << 20
+    g = np.random.default_rng(31)
+    waves = [
+        [("perm", "fp64", rng_complex((14, 14), 100 + i)) for i in range(400)],     # inputs
+        [("haf", "kappa", rng_symmetric(1, 18, 200 + i)[0]) for i in range(60)],    # workspace
+        [("perm", "fp64", rng_complex((22, 22), 300 + i)) for i in range(40)],      # workspace
+        [("lhaf", "fp64", rng_symmetric(1, 6, 400 + i)[0]) for i in range(900)],    # inputs
+    ]
+    with Server(budget_bytes=budget, slots=3, streams_per_slot=2) as s:
+        for wave in waves:
+            order = g.permutation(len(wave))
+            tickets = [(i, s.submit(wave[i][0], wave[i][2], wave[i][1])) for i in order]
+            for i, tk in tickets:
+                v, a, info = s.wait(tk)
+                f, t, A = wave[i]
+                w, c = alone(f, t, A)
+                assert info.status == 0 and same(v, a, w, c), (f, t, A.shape)
+        st = s.stats()
+    assert 0 < st["device_bytes_max"] <= budget, st
+
+
+def test_batch_budget_bounds_buffers_across_batches():
+    """A budgeted Batch whose successive batches need large inputs, then a large
+    workspace, gives its buffers back instead of holding both maxima."""
+    from gbskernels_amd import _lib
+    from gbskernels_amd.serving import Batch
+    big_in = [("perm", "fp64", rng_complex((16, 16), 7 + i)) for i in range(300)]    # 1.2 MB of input
+    big_ws = [("haf", "kappa", rng_symmetric(1, 20, 9 + i)[0]) for i in range(20)]
+    with Batch(budget_bytes=int(1.6 * (1 << 20))) as b:
+        for items in (big_in, big_ws, big_in):
+            vals, aux = b.run(items)
+            for (f, t, A), v, a in zip(items, vals, aux):
+                w, c = alone(f, t, A)
+                assert same(v, a, w, c)
