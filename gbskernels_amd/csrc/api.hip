@@ -346,6 +346,14 @@ int gbs_perm_get_lane_log2(int family, int n, int* log2_terms) {
 
 int gbs_perm_set_lane_log2(int family, int n, int log2_terms) { return perm_lane_log2_set(family, n, log2_terms); }
 
+int gbs_tor_get_schedule(int family, int* grouped_max_modes, int* bfs_max_modes) {
+  return tor_schedule_get(family, grouped_max_modes, bfs_max_modes);
+}
+
+int gbs_tor_set_schedule(int family, int grouped_max_modes, int bfs_max_modes) {
+  return tor_schedule_set(family, grouped_max_modes, bfs_max_modes);
+}
+
 int gbs_perm_split(int family, int n, int* lane_log2, int* workgroups) {
   if ((family != 0 && family != 1) || n < 1 || n > PERM_MAX_N || !lane_log2 || !workgroups) return GBSK_ERR_ARG;
   const PermSplit sp = perm_split(n, family);

@@ -211,6 +211,14 @@ __host__ __device__ inline int haf_part_offset(int n, int S) {
   return t;
 }
 
+// first term slot of subset size S when an instance's terms are stored size
+// by size (all C(n, 1) subsets of size 2, then the C(n, 2) of size 4, ...)
+__host__ __device__ inline int haf_small_term_offset(int n, int S) {
+  int t = 0;
+  for (int m = 1; m < S / 2; ++m) t += haf_binom(n, m);
+  return t;
+}
+
 // rounding depth of a phase's bound arithmetic (generous; see cert_finish)
 __device__ inline double haf_cert_depth(int S, int n, int count) {
   return 64.0 * (n + 1) * (S + 8) + 8.0 * count + 16.0 * n * n + 256.0;
@@ -229,6 +237,360 @@ struct HafPhaseJobs {
   int n;
 };
 
+// LDS scratch of one wave's subset evaluations at subset size S (every lane
+// group g of S lanes owns row g of the per-group arrays)
+// (PMAX: the largest pair count n it serves, plus one for the odd-n powers)
+template <class Ar, bool LOOP, int S, int MODE, int PMAX = HAF_MAX_PAIRS>
+struct HafScratch {
+  using T = typename Ar::T;
+  static constexpr bool ANYC = (MODE == HM_CERT || MODE == HM_DDCERT);
+  static constexpr int G = 64 / S;
+  static constexpr int P = PMAX;
+  T red[64];
+  T kgs[G][PMAX + 1];  // k g_k of each group's current subset
+  T es[G][PMAX + 1];   // Newton coefficients e_0..e_n
+  double rede[ANYC ? 64 : 1];
+  T kvs[LOOP ? G : 1][PMAX + 2];  // k v_k (loop hafnian)
+  double kve[LOOP && ANYC ? G : 1][PMAX + 2];
+  double kge[ANYC ? G : 1][PMAX + 1];
+  double ese[ANYC ? G : 1][PMAX + 1];
+};
+
+// ------------------------------------------------------------- one subset term
+// Every lane of the wave calls it (the group reductions hold barriers). Lane
+// group g = lane / S (g < 64 / S) evaluates subset q (lexicographic m-subset of
+// the n = N / 2 index pairs, m = S / 2) of the N x N matrix at As, which may
+// differ from group to group; lane r of the group owns row r of the powers
+// C^j. On return lane r == 0 of each group holds the subset's term
+// (-1)^(n - m) [x^n] exp(sum_k g_k x^k) in coeff and, in the certified modes,
+// its bound in ecoeff. The arithmetic depends on (As contents, N, q) only.
+template <class Ar, bool LOOP, int S, int MODE, class Scratch>
+__device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, int ab, int N, int q, Scratch& sh,
+                                                typename Ar::T& coeff_out, double& ecoeff_out) {
+  // As0 is the workgroup's LDS array and ab this group's matrix offset in it
+  // (an integer offset keeps every access a plain LDS address)
+  const cplx* __restrict__ As = As0 + ab;
+  using T = typename Ar::T;
+  constexpr bool KAPPA = (MODE == HM_KAPPA);
+  constexpr bool CERT = (MODE == HM_CERT);
+  constexpr bool DDC = (MODE == HM_DDCERT);  // certified double-double (Ar = ArDD)
+  constexpr bool ANYC = CERT || DDC;
+  constexpr int M = S / 2;
+  constexpr int G = 64 / S;
+  (void)KAPPA;
+  T* red = sh.red;
+  auto& kgs = sh.kgs;
+  auto& es = sh.es;
+  double* rede = sh.rede;
+  auto& kvs = sh.kvs;
+  auto& kve = sh.kve;
+  auto& kge = sh.kge;
+  auto& ese = sh.ese;
+  (void)kve;
+  (void)kge;
+  (void)ese;
+  (void)rede;
+  const int lane = threadIdx.x;
+  const int n = N / 2;
+  const int g = lane / S;
+  const int r = lane % S;
+  const bool lane_ok = g < G;
+  const int h = (n + 1) / 2;
+  const double gam = CERT ? cert_gamma(2 * S) : 0.0;
+  (void)gam;
+  int p[M];
+  unrank<M>(q, n, p);
+  // lane r owns row r of C = B_S X: C[r][2u] = A[R][2p_u + 1], C[r][2u+1] = A[R][2p_u]
+  const int R = 2 * p[r >> 1] + (r & 1);
+  T P[S];
+  double eP[ANYC ? S : 1];
+#pragma unroll
+  for (int u = 0; u < M; ++u) {
+    P[2 * u] = Ar::from(As[R * N + 2 * p[u] + 1]);
+    P[2 * u + 1] = Ar::from(As[R * N + 2 * p[u]]);
+  }
+  if (ANYC) {
+#pragma unroll
+    for (int c = 0; c < S; ++c) eP[c] = 0.0;  // C is exact
+  }
+  // Loop hafnian: k v_k = k/2 d^T X C^(k-1) d with d = diag B_S. With
+  // u_j = C^j d (lane r: row r of C^j times d, no lane exchange) and
+  // (C^a)^T = X C^a X, d^T X C^(a+b) d = sum_r u_a[r^1] u_b[r], so every v_k
+  // up to k - 1 = 2h comes from u_0 .. u_h with a = floor((k-1)/2),
+  // b = ceil((k-1)/2): v_{2j}, v_{2j+1} at power j. Stored apart (kvs) and
+  // added to the trace part p_k / 2 before the Newton recurrence.
+  auto dot_d = [&](const T (&row)[S], const double (&erow)[ANYC ? S : 1], T& u, double& eu) {
+    u = Ar::zero();
+    eu = CERT ? (double)S * CERT_UFL : 0.0;
+#pragma unroll
+    for (int c = 0; c < S; ++c) {
+      const int Rc = 2 * p[c >> 1] + (c & 1);
+      const cplx dc = As[Rc * N + Rc];
+      if constexpr (DDC) {
+        double o;
+        u = cddc_fma_c(row[c], dc, u, &o);
+        eu = fma(erow[c], cert_norm1(dc), eu + o);
+      } else {
+        if constexpr (CERT) eu = fma(fma(gam, cert_norm1(Ar::to_c(row[c])), erow[c]), cert_norm1(dc), eu);
+        u = Ar::fma_c(row[c], dc, u);
+      }
+    }
+  };
+  auto add_v = [&](int k, const T& ua, double eua, const T& ub, double eub) {
+    if (k > n) return;  // uniform over the wave (reductions hold barriers)
+    const T x = Ar::shfl_xor(ua, 1);
+    const double ex = ANYC ? shfl_xor_d(eua, 1) : 0.0;
+    const double hk = 0.5 * (double)k;
+    if constexpr (DDC) {
+      double o;
+      T v = cddc_fma(x, ub, Ar::zero(), &o);
+      double ev = cddc_mul_bound(x, ex, ub, eub, o);
+      ddcert_group_reduce<S>(v, ev, red, rede);
+      if (r == 0 && lane_ok) {
+        double os;
+        kvs[g][k] = cddc_scale(v, hk, &os);
+        kve[g][k] = fma(hk, ev, os);
+      }
+    } else if constexpr (CERT) {
+      T v = Ar::fma(x, ub, Ar::zero());
+      double ev = cert_fma(Ar::to_c(x), ex, Ar::to_c(ub), eub, cplx{0.0, 0.0}, 0.0, Ar::to_c(v));
+      cert_group_reduce<S>(v, ev, red, rede);
+      if (r == 0 && lane_ok) {
+        const T kv = Ar::scale(v, hk);
+        kvs[g][k] = kv;
+        kve[g][k] = cert_round(Ar::to_c(kv), hk * ev);
+      }
+    } else {
+      const T v = group_reduce<Ar, S>(Ar::fma(x, ub, Ar::zero()), red);
+      if (r == 0 && lane_ok) kvs[g][k] = Ar::scale(v, hk);
+    }
+  };
+  T uprev = Ar::zero();  // u_{j-1}
+  double euprev = 0.0;
+  if constexpr (LOOP) {
+    const T u0 = Ar::from(As[R * N + R]);
+    T u1;
+    double eu1;
+    dot_d(P, eP, u1, eu1);
+    add_v(1, u0, 0.0, u0, 0.0);
+    add_v(2, u0, 0.0, u1, eu1);
+    add_v(3, u1, eu1, u1, eu1);
+    uprev = u1;
+    euprev = eu1;
+  }
+  // p_1 = tr C (C[r][r] = A[R][R^1]) and p_2 = tr(C C) through the partner row
+  {
+    T d1 = Ar::from(As[R * N + (R ^ 1)]);
+    T t2 = Ar::zero();
+    double e2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < S; ++c) {
+      const T y = Ar::shfl_xor(P[c ^ 1], 1);
+      if constexpr (DDC) {
+        double eo;
+        t2 = cddc_fma(P[c], y, t2, &eo);  // exact operands
+        e2 += eo;
+      } else {
+        const T t2n = Ar::fma(P[c], y, t2);
+        if constexpr (CERT) e2 = cert_fma(P[c], 0.0, y, 0.0, t2, e2, t2n);
+        t2 = t2n;
+      }
+    }
+    if constexpr (DDC) {
+      double e1 = 0.0;
+      ddcert_group_reduce<S>(d1, e1, red, rede);
+      ddcert_group_reduce<S>(t2, e2, red, rede);
+      if (r == 0 && lane_ok) {  // halving is exact (up to underflow)
+        kgs[g][1] = Ar::scale(d1, 0.5);
+        kgs[g][2] = Ar::scale(t2, 0.5);
+        kge[g][1] = fma(0.5, e1, 2.0 * CERT_UFL);
+        kge[g][2] = fma(0.5, e2, 2.0 * CERT_UFL);
+      }
+    } else if constexpr (CERT) {
+      double e1 = 0.0;
+      cert_group_reduce<S>(d1, e1, red, rede);
+      cert_group_reduce<S>(t2, e2, red, rede);
+      if (r == 0 && lane_ok) {
+        kgs[g][1] = Ar::scale(d1, 0.5);
+        kgs[g][2] = Ar::scale(t2, 0.5);
+        kge[g][1] = fma(0.5, e1, CERT_UFL);
+        kge[g][2] = fma(0.5, e2, CERT_UFL);
+      }
+    } else {
+      const T s1 = group_reduce<Ar, S>(d1, red);
+      const T s2 = group_reduce<Ar, S>(t2, red);
+      if (r == 0 && lane_ok) {
+        kgs[g][1] = Ar::scale(s1, 0.5);
+        kgs[g][2] = Ar::scale(s2, 0.5);
+      }
+    }
+  }
+  // C^j for j = 2..h: one matrix product each, then p_{2j-1}, p_{2j}
+  for (int j = 2; j <= h; ++j) {
+    T acc[S];
+    double eA[ANYC ? S : 1];
+#pragma unroll
+    for (int c = 0; c < S; ++c) acc[c] = Ar::zero();
+    if (ANYC) {
+#pragma unroll
+      for (int c = 0; c < S; ++c) eA[c] = CERT ? (double)S * CERT_UFL : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+      const cplx* At = As + (2 * p[t >> 1] + (t & 1)) * N;
+      // a-priori bound of the fma-chain dot product: sum_t (eP_t + gamma_2S |P_t|) |C_tc|
+      const double wt = CERT ? fma(gam, cert_norm1(Ar::to_c(P[t])), eP[t]) : 0.0;
+#pragma unroll
+      for (int u = 0; u < M; ++u) {
+        const cplx a1 = At[2 * p[u] + 1], a0 = At[2 * p[u]];
+        if constexpr (DDC) {
+          double o1, o0;
+          acc[2 * u] = cddc_fma_c(P[t], a1, acc[2 * u], &o1);
+          acc[2 * u + 1] = cddc_fma_c(P[t], a0, acc[2 * u + 1], &o0);
+          eA[2 * u] = fma(eP[t], cert_norm1(a1), eA[2 * u] + o1);
+          eA[2 * u + 1] = fma(eP[t], cert_norm1(a0), eA[2 * u + 1] + o0);
+          continue;
+        }
+        acc[2 * u] = Ar::fma_c(P[t], a1, acc[2 * u]);
+        acc[2 * u + 1] = Ar::fma_c(P[t], a0, acc[2 * u + 1]);
+        if (CERT) {
+          eA[2 * u] = fma(wt, cert_norm1(a1), eA[2 * u]);
+          eA[2 * u + 1] = fma(wt, cert_norm1(a0), eA[2 * u + 1]);
+        }
+      }
+    }
+    if constexpr (LOOP) {
+      T uj;
+      double euj;
+      dot_d(acc, eA, uj, euj);
+      add_v(2 * j, uprev, euprev, uj, euj);
+      add_v(2 * j + 1, uj, euj, uj, euj);
+      uprev = uj;
+      euprev = euj;
+    }
+    T todd = Ar::zero(), teven = Ar::zero();
+    double eo = 0.0, ee = 0.0;
+#pragma unroll
+    for (int c = 0; c < S; ++c) {
+      const T yo = Ar::shfl_xor(P[c ^ 1], 1);
+      const T ye = Ar::shfl_xor(acc[c ^ 1], 1);
+      if constexpr (DDC) {
+        const double eyo = shfl_xor_d(eP[c ^ 1], 1), eye = shfl_xor_d(eA[c ^ 1], 1);
+        double oo, oe;
+        const T ton = cddc_fma(acc[c], yo, todd, &oo);
+        const T ten = cddc_fma(acc[c], ye, teven, &oe);
+        eo = eo + cddc_mul_bound(acc[c], eA[c], yo, eyo, oo);
+        ee = ee + cddc_mul_bound(acc[c], eA[c], ye, eye, oe);
+        todd = ton;
+        teven = ten;
+        continue;
+      }
+      const T ton = Ar::fma(acc[c], yo, todd);
+      const T ten = Ar::fma(acc[c], ye, teven);
+      if constexpr (CERT) {
+        const double eyo = shfl_xor_d(eP[c ^ 1], 1), eye = shfl_xor_d(eA[c ^ 1], 1);
+        eo = cert_fma(acc[c], eA[c], yo, eyo, todd, eo, ton);
+        ee = cert_fma(acc[c], eA[c], ye, eye, teven, ee, ten);
+      }
+      todd = ton;
+      teven = ten;
+    }
+    if constexpr (DDC) {
+      ddcert_group_reduce<S>(todd, eo, red, rede);
+      ddcert_group_reduce<S>(teven, ee, red, rede);
+      if (r == 0 && lane_ok) {
+        kgs[g][2 * j - 1] = Ar::scale(todd, 0.5);
+        kge[g][2 * j - 1] = fma(0.5, eo, 2.0 * CERT_UFL);
+        if (2 * j <= Scratch::P) {
+          kgs[g][2 * j] = Ar::scale(teven, 0.5);
+          kge[g][2 * j] = fma(0.5, ee, 2.0 * CERT_UFL);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < S; ++c) eP[c] = eA[c];
+    } else if constexpr (CERT) {
+      cert_group_reduce<S>(todd, eo, red, rede);
+      cert_group_reduce<S>(teven, ee, red, rede);
+      if (r == 0 && lane_ok) {
+        kgs[g][2 * j - 1] = Ar::scale(todd, 0.5);
+        kge[g][2 * j - 1] = fma(0.5, eo, CERT_UFL);
+        if (2 * j <= Scratch::P) {
+          kgs[g][2 * j] = Ar::scale(teven, 0.5);
+          kge[g][2 * j] = fma(0.5, ee, CERT_UFL);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < S; ++c) eP[c] = eA[c];
+    } else {
+      const T so = group_reduce<Ar, S>(todd, red);
+      const T se = group_reduce<Ar, S>(teven, red);
+      if (r == 0 && lane_ok) {
+        kgs[g][2 * j - 1] = Ar::scale(so, 0.5);
+        if (2 * j <= Scratch::P) kgs[g][2 * j] = Ar::scale(se, 0.5);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < S; ++c) P[c] = acc[c];
+  }
+  if constexpr (LOOP) {
+    // kg_k = p_k / 2 + k v_k
+    if (r == 0 && lane_ok) {
+      for (int k = 1; k <= n; ++k) {
+        if constexpr (DDC) {
+          double o;
+          kgs[g][k] = cddc_add(kgs[g][k], kvs[g][k], &o);
+          kge[g][k] = kge[g][k] + kve[g][k] + o;
+        } else if constexpr (CERT) {
+          const T sum = Ar::add(kgs[g][k], kvs[g][k]);
+          kge[g][k] = cert_round(Ar::to_c(sum), kge[g][k] + kve[g][k]);
+          kgs[g][k] = sum;
+        } else {
+          kgs[g][k] = Ar::add(kgs[g][k], kvs[g][k]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // Newton: e_j = (1/j) sum_{k=1..j} kg_k e_{j-k}, by the first lane of each group
+  T coeff = Ar::zero();
+  double ecoeff = 0.0;
+  if (r == 0 && lane_ok) {
+    T* e = es[g];
+    e[0] = Ar::one();
+    if (ANYC) ese[g][0] = 0.0;
+    for (int jj = 1; jj <= n; ++jj) {
+      T a = Ar::zero();
+      double ea = 0.0;
+      if constexpr (DDC) {
+        for (int k = 1; k <= jj; ++k) {
+          double o;
+          const T an = cddc_fma(kgs[g][k], e[jj - k], a, &o);
+          ea = ea + cddc_mul_bound(kgs[g][k], kge[g][k], e[jj - k], ese[g][jj - k], o);
+          a = an;
+        }
+        double od;
+        e[jj] = cddc_div_d(a, (double)jj, &od);
+        ese[g][jj] = ea / (double)jj + od;
+        continue;
+      }
+      for (int k = 1; k <= jj; ++k) {
+        const T an = Ar::fma(kgs[g][k], e[jj - k], a);
+        if (CERT) ea = cert_fma(Ar::to_c(kgs[g][k]), kge[g][k], Ar::to_c(e[jj - k]), ese[g][jj - k], Ar::to_c(a), ea,
+                                Ar::to_c(an));
+        a = an;
+      }
+      e[jj] = Ar::div_int(a, jj);
+      if (CERT) ese[g][jj] = cert_round(Ar::to_c(e[jj]), ea / (double)jj);
+    }
+    coeff = e[n];
+    if (ANYC) ecoeff = ese[g][n];
+  }
+  __syncthreads();
+  if ((n - M) & 1) coeff = Ar::neg(coeff);
+  coeff_out = coeff;
+  ecoeff_out = ecoeff;
+}
+
 // ---------------------------------------------------------------- phase kernel
 // One workgroup (one wave) = one instance; evaluates every subset of size
 // m = S/2 and writes the phase partial (and sum |term| in kappa mode, the
@@ -244,14 +606,9 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const HafPhaseJobs<typena
   constexpr int M = S / 2;
   constexpr int G = 64 / S;
   extern __shared__ cplx As[];  // N * N, dynamic
-  __shared__ T red[64];
-  __shared__ T kgs[G][HAF_MAX_PAIRS + 1];  // k g_k of each group's current subset
-  __shared__ T es[G][HAF_MAX_PAIRS + 1];   // Newton coefficients e_0..e_n
-  __shared__ double rede[ANYC ? 64 : 1];
-  __shared__ T kvs[LOOP ? G : 1][HAF_MAX_PAIRS + 2];  // k v_k (loop hafnian)
-  __shared__ double kve[LOOP && ANYC ? G : 1][HAF_MAX_PAIRS + 2];
-  __shared__ double kge[ANYC ? G : 1][HAF_MAX_PAIRS + 1];
-  __shared__ double ese[ANYC ? G : 1][HAF_MAX_PAIRS + 1];
+  __shared__ HafScratch<Ar, LOOP, S, MODE> sh;
+  T* red = sh.red;
+  double* rede = sh.rede;
   const int lane = threadIdx.x;
   // this workgroup's bucket (wave-uniform scan of at most HAF_MAX_JOBS entries)
   int jb = 0;
@@ -278,8 +635,6 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const HafPhaseJobs<typena
   const int r = lane % S;
   const bool lane_ok = g < G;
   const int count = binom(n, M);
-  const int h = (n + 1) / 2;
-  const double gam = CERT ? cert_gamma(2 * S) : 0.0;
 
   T group_acc = Ar::zero();
   double group_abs = 0.0;  // kappa: sum |term|; certified: bound of group_acc
@@ -289,295 +644,9 @@ __global__ __launch_bounds__(64) void haf_phase_kernel(const HafPhaseJobs<typena
   for (int q0 = qbeg; q0 < qend; q0 += G) {
     const int q = q0 + (lane_ok ? g : 0);  // groups past G mirror group 0 (discarded)
     const bool active = lane_ok && q < qend;
-    int p[M];
-    unrank<M>(active ? q : 0, n, p);
-    // lane r owns row r of C = B_S X: C[r][2u] = A[R][2p_u + 1], C[r][2u+1] = A[R][2p_u]
-    const int R = 2 * p[r >> 1] + (r & 1);
-    T P[S];
-    double eP[ANYC ? S : 1];
-#pragma unroll
-    for (int u = 0; u < M; ++u) {
-      P[2 * u] = Ar::from(As[R * N + 2 * p[u] + 1]);
-      P[2 * u + 1] = Ar::from(As[R * N + 2 * p[u]]);
-    }
-    if (ANYC) {
-#pragma unroll
-      for (int c = 0; c < S; ++c) eP[c] = 0.0;  // C is exact
-    }
-    // Loop hafnian: k v_k = k/2 d^T X C^(k-1) d with d = diag B_S. With
-    // u_j = C^j d (lane r: row r of C^j times d, no lane exchange) and
-    // (C^a)^T = X C^a X, d^T X C^(a+b) d = sum_r u_a[r^1] u_b[r], so every v_k
-    // up to k - 1 = 2h comes from u_0 .. u_h with a = floor((k-1)/2),
-    // b = ceil((k-1)/2): v_{2j}, v_{2j+1} at power j. Stored apart (kvs) and
-    // added to the trace part p_k / 2 before the Newton recurrence.
-    auto dot_d = [&](const T (&row)[S], const double (&erow)[ANYC ? S : 1], T& u, double& eu) {
-      u = Ar::zero();
-      eu = CERT ? (double)S * CERT_UFL : 0.0;
-#pragma unroll
-      for (int c = 0; c < S; ++c) {
-        const int Rc = 2 * p[c >> 1] + (c & 1);
-        const cplx dc = As[Rc * N + Rc];
-        if constexpr (DDC) {
-          double o;
-          u = cddc_fma_c(row[c], dc, u, &o);
-          eu = fma(erow[c], cert_norm1(dc), eu + o);
-        } else {
-          if constexpr (CERT) eu = fma(fma(gam, cert_norm1(Ar::to_c(row[c])), erow[c]), cert_norm1(dc), eu);
-          u = Ar::fma_c(row[c], dc, u);
-        }
-      }
-    };
-    auto add_v = [&](int k, const T& ua, double eua, const T& ub, double eub) {
-      if (k > n) return;  // uniform over the wave (reductions hold barriers)
-      const T x = Ar::shfl_xor(ua, 1);
-      const double ex = ANYC ? shfl_xor_d(eua, 1) : 0.0;
-      const double hk = 0.5 * (double)k;
-      if constexpr (DDC) {
-        double o;
-        T v = cddc_fma(x, ub, Ar::zero(), &o);
-        double ev = cddc_mul_bound(x, ex, ub, eub, o);
-        ddcert_group_reduce<S>(v, ev, red, rede);
-        if (r == 0 && lane_ok) {
-          double os;
-          kvs[g][k] = cddc_scale(v, hk, &os);
-          kve[g][k] = fma(hk, ev, os);
-        }
-      } else if constexpr (CERT) {
-        T v = Ar::fma(x, ub, Ar::zero());
-        double ev = cert_fma(Ar::to_c(x), ex, Ar::to_c(ub), eub, cplx{0.0, 0.0}, 0.0, Ar::to_c(v));
-        cert_group_reduce<S>(v, ev, red, rede);
-        if (r == 0 && lane_ok) {
-          const T kv = Ar::scale(v, hk);
-          kvs[g][k] = kv;
-          kve[g][k] = cert_round(Ar::to_c(kv), hk * ev);
-        }
-      } else {
-        const T v = group_reduce<Ar, S>(Ar::fma(x, ub, Ar::zero()), red);
-        if (r == 0 && lane_ok) kvs[g][k] = Ar::scale(v, hk);
-      }
-    };
-    T uprev = Ar::zero();  // u_{j-1}
-    double euprev = 0.0;
-    if constexpr (LOOP) {
-      const T u0 = Ar::from(As[R * N + R]);
-      T u1;
-      double eu1;
-      dot_d(P, eP, u1, eu1);
-      add_v(1, u0, 0.0, u0, 0.0);
-      add_v(2, u0, 0.0, u1, eu1);
-      add_v(3, u1, eu1, u1, eu1);
-      uprev = u1;
-      euprev = eu1;
-    }
-    // p_1 = tr C (C[r][r] = A[R][R^1]) and p_2 = tr(C C) through the partner row
-    {
-      T d1 = Ar::from(As[R * N + (R ^ 1)]);
-      T t2 = Ar::zero();
-      double e2 = 0.0;
-#pragma unroll
-      for (int c = 0; c < S; ++c) {
-        const T y = Ar::shfl_xor(P[c ^ 1], 1);
-        if constexpr (DDC) {
-          double eo;
-          t2 = cddc_fma(P[c], y, t2, &eo);  // exact operands
-          e2 += eo;
-        } else {
-          const T t2n = Ar::fma(P[c], y, t2);
-          if constexpr (CERT) e2 = cert_fma(P[c], 0.0, y, 0.0, t2, e2, t2n);
-          t2 = t2n;
-        }
-      }
-      if constexpr (DDC) {
-        double e1 = 0.0;
-        ddcert_group_reduce<S>(d1, e1, red, rede);
-        ddcert_group_reduce<S>(t2, e2, red, rede);
-        if (r == 0 && lane_ok) {  // halving is exact (up to underflow)
-          kgs[g][1] = Ar::scale(d1, 0.5);
-          kgs[g][2] = Ar::scale(t2, 0.5);
-          kge[g][1] = fma(0.5, e1, 2.0 * CERT_UFL);
-          kge[g][2] = fma(0.5, e2, 2.0 * CERT_UFL);
-        }
-      } else if constexpr (CERT) {
-        double e1 = 0.0;
-        cert_group_reduce<S>(d1, e1, red, rede);
-        cert_group_reduce<S>(t2, e2, red, rede);
-        if (r == 0 && lane_ok) {
-          kgs[g][1] = Ar::scale(d1, 0.5);
-          kgs[g][2] = Ar::scale(t2, 0.5);
-          kge[g][1] = fma(0.5, e1, CERT_UFL);
-          kge[g][2] = fma(0.5, e2, CERT_UFL);
-        }
-      } else {
-        const T s1 = group_reduce<Ar, S>(d1, red);
-        const T s2 = group_reduce<Ar, S>(t2, red);
-        if (r == 0 && lane_ok) {
-          kgs[g][1] = Ar::scale(s1, 0.5);
-          kgs[g][2] = Ar::scale(s2, 0.5);
-        }
-      }
-    }
-    // C^j for j = 2..h: one matrix product each, then p_{2j-1}, p_{2j}
-    for (int j = 2; j <= h; ++j) {
-      T acc[S];
-      double eA[ANYC ? S : 1];
-#pragma unroll
-      for (int c = 0; c < S; ++c) acc[c] = Ar::zero();
-      if (ANYC) {
-#pragma unroll
-        for (int c = 0; c < S; ++c) eA[c] = CERT ? (double)S * CERT_UFL : 0.0;
-      }
-#pragma unroll
-      for (int t = 0; t < S; ++t) {
-        const cplx* At = As + (2 * p[t >> 1] + (t & 1)) * N;
-        // a-priori bound of the fma-chain dot product: sum_t (eP_t + gamma_2S |P_t|) |C_tc|
-        const double wt = CERT ? fma(gam, cert_norm1(Ar::to_c(P[t])), eP[t]) : 0.0;
-#pragma unroll
-        for (int u = 0; u < M; ++u) {
-          const cplx a1 = At[2 * p[u] + 1], a0 = At[2 * p[u]];
-          if constexpr (DDC) {
-            double o1, o0;
-            acc[2 * u] = cddc_fma_c(P[t], a1, acc[2 * u], &o1);
-            acc[2 * u + 1] = cddc_fma_c(P[t], a0, acc[2 * u + 1], &o0);
-            eA[2 * u] = fma(eP[t], cert_norm1(a1), eA[2 * u] + o1);
-            eA[2 * u + 1] = fma(eP[t], cert_norm1(a0), eA[2 * u + 1] + o0);
-            continue;
-          }
-          acc[2 * u] = Ar::fma_c(P[t], a1, acc[2 * u]);
-          acc[2 * u + 1] = Ar::fma_c(P[t], a0, acc[2 * u + 1]);
-          if (CERT) {
-            eA[2 * u] = fma(wt, cert_norm1(a1), eA[2 * u]);
-            eA[2 * u + 1] = fma(wt, cert_norm1(a0), eA[2 * u + 1]);
-          }
-        }
-      }
-      if constexpr (LOOP) {
-        T uj;
-        double euj;
-        dot_d(acc, eA, uj, euj);
-        add_v(2 * j, uprev, euprev, uj, euj);
-        add_v(2 * j + 1, uj, euj, uj, euj);
-        uprev = uj;
-        euprev = euj;
-      }
-      T todd = Ar::zero(), teven = Ar::zero();
-      double eo = 0.0, ee = 0.0;
-#pragma unroll
-      for (int c = 0; c < S; ++c) {
-        const T yo = Ar::shfl_xor(P[c ^ 1], 1);
-        const T ye = Ar::shfl_xor(acc[c ^ 1], 1);
-        if constexpr (DDC) {
-          const double eyo = shfl_xor_d(eP[c ^ 1], 1), eye = shfl_xor_d(eA[c ^ 1], 1);
-          double oo, oe;
-          const T ton = cddc_fma(acc[c], yo, todd, &oo);
-          const T ten = cddc_fma(acc[c], ye, teven, &oe);
-          eo = eo + cddc_mul_bound(acc[c], eA[c], yo, eyo, oo);
-          ee = ee + cddc_mul_bound(acc[c], eA[c], ye, eye, oe);
-          todd = ton;
-          teven = ten;
-          continue;
-        }
-        const T ton = Ar::fma(acc[c], yo, todd);
-        const T ten = Ar::fma(acc[c], ye, teven);
-        if constexpr (CERT) {
-          const double eyo = shfl_xor_d(eP[c ^ 1], 1), eye = shfl_xor_d(eA[c ^ 1], 1);
-          eo = cert_fma(acc[c], eA[c], yo, eyo, todd, eo, ton);
-          ee = cert_fma(acc[c], eA[c], ye, eye, teven, ee, ten);
-        }
-        todd = ton;
-        teven = ten;
-      }
-      if constexpr (DDC) {
-        ddcert_group_reduce<S>(todd, eo, red, rede);
-        ddcert_group_reduce<S>(teven, ee, red, rede);
-        if (r == 0 && lane_ok) {
-          kgs[g][2 * j - 1] = Ar::scale(todd, 0.5);
-          kge[g][2 * j - 1] = fma(0.5, eo, 2.0 * CERT_UFL);
-          if (2 * j <= HAF_MAX_PAIRS) {
-            kgs[g][2 * j] = Ar::scale(teven, 0.5);
-            kge[g][2 * j] = fma(0.5, ee, 2.0 * CERT_UFL);
-          }
-        }
-#pragma unroll
-        for (int c = 0; c < S; ++c) eP[c] = eA[c];
-      } else if constexpr (CERT) {
-        cert_group_reduce<S>(todd, eo, red, rede);
-        cert_group_reduce<S>(teven, ee, red, rede);
-        if (r == 0 && lane_ok) {
-          kgs[g][2 * j - 1] = Ar::scale(todd, 0.5);
-          kge[g][2 * j - 1] = fma(0.5, eo, CERT_UFL);
-          if (2 * j <= HAF_MAX_PAIRS) {
-            kgs[g][2 * j] = Ar::scale(teven, 0.5);
-            kge[g][2 * j] = fma(0.5, ee, CERT_UFL);
-          }
-        }
-#pragma unroll
-        for (int c = 0; c < S; ++c) eP[c] = eA[c];
-      } else {
-        const T so = group_reduce<Ar, S>(todd, red);
-        const T se = group_reduce<Ar, S>(teven, red);
-        if (r == 0 && lane_ok) {
-          kgs[g][2 * j - 1] = Ar::scale(so, 0.5);
-          if (2 * j <= HAF_MAX_PAIRS) kgs[g][2 * j] = Ar::scale(se, 0.5);
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < S; ++c) P[c] = acc[c];
-    }
-    if constexpr (LOOP) {
-      // kg_k = p_k / 2 + k v_k
-      if (r == 0 && lane_ok) {
-        for (int k = 1; k <= n; ++k) {
-          if constexpr (DDC) {
-            double o;
-            kgs[g][k] = cddc_add(kgs[g][k], kvs[g][k], &o);
-            kge[g][k] = kge[g][k] + kve[g][k] + o;
-          } else if constexpr (CERT) {
-            const T sum = Ar::add(kgs[g][k], kvs[g][k]);
-            kge[g][k] = cert_round(Ar::to_c(sum), kge[g][k] + kve[g][k]);
-            kgs[g][k] = sum;
-          } else {
-            kgs[g][k] = Ar::add(kgs[g][k], kvs[g][k]);
-          }
-        }
-      }
-    }
-    __syncthreads();
-    // Newton: e_j = (1/j) sum_{k=1..j} kg_k e_{j-k}, by the first lane of each group
-    T coeff = Ar::zero();
-    double ecoeff = 0.0;
-    if (r == 0 && lane_ok) {
-      T* e = es[g];
-      e[0] = Ar::one();
-      if (ANYC) ese[g][0] = 0.0;
-      for (int jj = 1; jj <= n; ++jj) {
-        T a = Ar::zero();
-        double ea = 0.0;
-        if constexpr (DDC) {
-          for (int k = 1; k <= jj; ++k) {
-            double o;
-            const T an = cddc_fma(kgs[g][k], e[jj - k], a, &o);
-            ea = ea + cddc_mul_bound(kgs[g][k], kge[g][k], e[jj - k], ese[g][jj - k], o);
-            a = an;
-          }
-          double od;
-          e[jj] = cddc_div_d(a, (double)jj, &od);
-          ese[g][jj] = ea / (double)jj + od;
-          continue;
-        }
-        for (int k = 1; k <= jj; ++k) {
-          const T an = Ar::fma(kgs[g][k], e[jj - k], a);
-          if (CERT) ea = cert_fma(Ar::to_c(kgs[g][k]), kge[g][k], Ar::to_c(e[jj - k]), ese[g][jj - k], Ar::to_c(a), ea,
-                                  Ar::to_c(an));
-          a = an;
-        }
-        e[jj] = Ar::div_int(a, jj);
-        if (CERT) ese[g][jj] = cert_round(Ar::to_c(e[jj]), ea / (double)jj);
-      }
-      coeff = e[n];
-      if (ANYC) ecoeff = ese[g][n];
-    }
-    __syncthreads();
-    if ((n - M) & 1) coeff = Ar::neg(coeff);
+    T coeff;
+    double ecoeff;
+    haf_subset_term<Ar, LOOP, S, MODE>(As, 0, N, active ? q : 0, sh, coeff, ecoeff);
     if constexpr (DDC) {
       if (active && r == 0) {
         double o;
@@ -705,6 +774,173 @@ __global__ __launch_bounds__(64 * HAF_REDUCE_WAVES) void haf_reduce_kernel(const
   }
 }
 
+// ------------------------------------------- small sizes: one fused launch
+// For N <= HAF_SMALL_MAX_N (fp64 / kappa / dd) one launch replaces the n phase
+// launches and the reduce. A workgroup (one wave) takes `T` consecutive
+// instances of a bucket: it stages their matrices in LDS, evaluates every subset
+// of every size with haf_subset_term (lane groups of S lanes, 64 / S groups per
+// round, subsets of several instances in one round, since a term depends only
+// on its own matrix), keeps the terms in LDS, forms each chunk partial in the
+// phase kernel's order (the chunk's groups summed in group order, idle groups
+// contributing zero) and sums the partials with the reduce kernel's butterfly.
+// Values and kappa sums are therefore bit-identical to the phase path, and the
+// number of instances per workgroup only changes the schedule.
+constexpr int HAF_SMALL_MAX_N = 12;
+constexpr int HAF_SMALL_MAX_PAIRS = HAF_SMALL_MAX_N / 2;
+constexpr int HAF_SMALL_MAX_T = 4;  // instances per workgroup
+static_assert(HAF_CHUNK_ITERS == 1, "the fused small-size kernel replays one-round chunks");
+
+struct HafSmallJobs {
+  const cplx* mats[HAF_MAX_JOBS];
+  cplx* out[HAF_MAX_JOBS];
+  double* aux[HAF_MAX_JOBS];
+  int N[HAF_MAX_JOBS];
+  int B[HAF_MAX_JOBS];
+  int T[HAF_MAX_JOBS];
+  int wg_begin[HAF_MAX_JOBS + 1];
+  int n;
+};
+
+// one phase (subset size S) of the ninst staged instances: terms[i][off + q]
+// receives instance i's subset-q term as the phase kernel's group accumulator
+// holds it (zero + term), tabs the kappa magnitude (0 + |term|)
+template <class Ar, bool LOOP, int MODE, int S>
+__device__ __forceinline__ void haf_small_phase(const cplx* As, int N, int ninst, void* scratch,
+                                                typename Ar::T (*terms)[64], double (*tabs)[64]) {
+  using T = typename Ar::T;
+  using Sc = HafScratch<Ar, LOOP, S, MODE, HAF_SMALL_MAX_PAIRS + 1>;
+  constexpr int G = 64 / S;
+  Sc& sh = *reinterpret_cast<Sc*>(scratch);
+  const int lane = threadIdx.x;
+  const int g = lane / S, r = lane % S;
+  const bool lane_ok = g < G;
+  const int n = N / 2;
+  const int count = haf_binom(n, S / 2);
+  const int off = haf_small_term_offset(n, S);
+  const int total = ninst * count;
+  for (int base = 0; base < total; base += G) {  // wave-uniform
+    const int item = base + g;
+    const bool active = lane_ok && item < total;
+    const int inst = active ? item / count : 0;
+    const int q = active ? item % count : 0;
+    T coeff;
+    double ecoeff;
+    haf_subset_term<Ar, LOOP, S, MODE>(As, inst * N * N, N, q, sh, coeff, ecoeff);
+    (void)ecoeff;
+    if (active && r == 0) {
+      terms[inst][off + q] = Ar::add(Ar::zero(), coeff);
+      if (MODE == HM_KAPPA) tabs[inst][off + q] = 0.0 + c_abs(Ar::to_c(coeff));
+    }
+  }
+  __syncthreads();
+}
+
+template <class Ar, bool LOOP, int MODE, int... Ss>
+__device__ __forceinline__ void haf_small_phases(const cplx* As, int N, int ninst, void* scratch,
+                                                 typename Ar::T (*terms)[64], double (*tabs)[64],
+                                                 std::integer_sequence<int, Ss...>) {
+  ((2 * (Ss + 1) <= N ? (haf_small_phase<Ar, LOOP, MODE, 2 * (Ss + 1)>(As, N, ninst, scratch, terms, tabs), 0) : 0),
+   ...);
+}
+
+template <class Ar, bool LOOP, int MODE>
+__global__ __launch_bounds__(64) void haf_small_kernel(const HafSmallJobs jobs) {
+  using T = typename Ar::T;
+  constexpr bool KAPPA = (MODE == HM_KAPPA);
+  constexpr int NN = HAF_SMALL_MAX_N * HAF_SMALL_MAX_N;
+  // the largest phase scratch is the S = 2 one (32 lane groups)
+  constexpr size_t SCR = sizeof(HafScratch<Ar, LOOP, 2, MODE, HAF_SMALL_MAX_PAIRS + 1>);
+  __shared__ cplx As[HAF_SMALL_MAX_T * NN];
+  __shared__ T terms[HAF_SMALL_MAX_T][64];
+  __shared__ double tabs[KAPPA ? HAF_SMALL_MAX_T : 1][64];
+  __shared__ T parts[HAF_SMALL_MAX_T][64];
+  __shared__ double pabs[KAPPA ? HAF_SMALL_MAX_T : 1][64];
+  __shared__ __align__(16) unsigned char scratch[SCR];
+  const int lane = threadIdx.x;
+  int jb = 0;
+  while (jb + 1 < jobs.n && (int)blockIdx.x >= jobs.wg_begin[jb + 1]) ++jb;
+  const int N = jobs.N[jb], n = N / 2, Tj = jobs.T[jb];
+  const int64_t b0 = (int64_t)((int)blockIdx.x - jobs.wg_begin[jb]) * Tj;
+  const int ninst = (int)min((int64_t)Tj, (int64_t)jobs.B[jb] - b0);
+  const cplx* __restrict__ src = jobs.mats[jb] + b0 * N * N;
+  // stage (the ordinary hafnian zeroes the diagonal, as the phase kernels do)
+  for (int e = lane; e < ninst * N * N; e += 64) {
+    cplx v = src[e];
+    const int rc = e % (N * N);
+    if (!LOOP && (rc / N) == (rc % N)) v = cplx{0.0, 0.0};
+    As[e] = v;
+  }
+  __syncthreads();
+  haf_small_phases<Ar, LOOP, MODE>(As, N, ninst, scratch, terms, KAPPA ? tabs : nullptr,
+                                   std::make_integer_sequence<int, HAF_SMALL_MAX_PAIRS>{});
+  // chunk partials, in the phase kernels' slot order and association
+  const int np = haf_nparts(n);
+  for (int idx = lane; idx < ninst * np; idx += 64) {
+    const int i = idx / np;
+    int slot = idx % np, m = 1;
+    while (slot >= haf_nchunks(n, 2 * m)) slot -= haf_nchunks(n, 2 * m++);
+    const int S = 2 * m, G = 64 / S, count = haf_binom(n, m);
+    const int off = haf_small_term_offset(n, S), q0 = slot * G;
+    T acc = terms[i][off + q0];
+    double a = KAPPA ? tabs[i][off + q0] : 0.0;
+    for (int k = 1; k < G; ++k) {
+      const bool on = q0 + k < count;
+      acc = Ar::add(acc, on ? terms[i][off + q0 + k] : Ar::zero());
+      if (KAPPA) a += on ? tabs[i][off + q0 + k] : 0.0;
+    }
+    parts[i][idx % np] = acc;
+    if (KAPPA) pabs[i][idx % np] = a;
+  }
+  __syncthreads();
+  // the reduce kernel's butterfly over the partials, one instance at a time
+  for (int i = 0; i < ninst; ++i) {
+    T s = Ar::zero();
+    double a = 0.0;
+    if (lane < np) {
+      s = parts[i][lane];
+      if (KAPPA) a = pabs[i][lane];
+    }
+#pragma unroll
+    for (int k = 1; k < 64; k <<= 1) {
+      const T o = Ar::shfl_xor(s, k);
+      const double ao = KAPPA ? shfl_xor_d(a, k) : 0.0;
+      s = Ar::add(s, o);
+      if (KAPPA) a += ao;
+    }
+    if (lane == 0) {
+      jobs.out[jb][b0 + i] = Ar::to_c(s);
+      if (KAPPA) jobs.aux[jb][b0 + i] = a;
+    }
+  }
+}
+
+// instances per workgroup of the fused small-size kernel: enough workgroups to
+// give every SIMD two waves where the batch allows, at most HAF_SMALL_MAX_T
+inline int haf_small_T(int B) {
+  const int t = B / 2048;
+  return t < 1 ? 1 : (t > HAF_SMALL_MAX_T ? HAF_SMALL_MAX_T : t);
+}
+
+template <class Ar, bool LOOP, int MODE>
+static void launch_small(const HafJob* jobs, int njobs, hipStream_t st) {
+  HafSmallJobs sj{};
+  int wg = 0;
+  for (int j = 0; j < njobs; ++j) {
+    const int k = sj.n++;
+    sj.mats[k] = jobs[j].in;
+    sj.out[k] = jobs[j].out;
+    sj.aux[k] = jobs[j].aux;
+    sj.N[k] = jobs[j].N;
+    sj.B[k] = jobs[j].B;
+    sj.T[k] = haf_small_T(jobs[j].B);
+    sj.wg_begin[k] = wg;
+    wg += (jobs[j].B + sj.T[k] - 1) / sj.T[k];
+  }
+  if (sj.n == 0 || wg == 0) return;
+  sj.wg_begin[sj.n] = wg;
+  hipLaunchKernelGGL((haf_small_kernel<Ar, LOOP, MODE>), dim3((unsigned)wg), dim3(64), 0, st, sj);
+}
+
 template <class Ar, bool LOOP, int MODE, int S>
 static void launch_phase(const HafJob* jobs, int njobs, typename Ar::T* const* part, double* const* part_abs,
                          hipStream_t st) {
@@ -738,8 +974,20 @@ static void launch_phases(const HafJob* jobs, int njobs, typename Ar::T* const* 
 
 // jobs: even N >= 2 and B >= 1 each, at most HAF_MAX_JOBS
 template <class Ar, bool LOOP, int MODE>
-int run_haf_multi(const HafJob* jobs, int njobs, hipStream_t st) {
+int run_haf_multi(const HafJob* all, int nall, hipStream_t st) {
   using T = typename Ar::T;
+  // small sizes of the plain and kappa modes: one fused launch
+  HafJob jobs[HAF_MAX_JOBS];
+  int njobs = 0;
+  if constexpr (MODE == HM_PLAIN || MODE == HM_KAPPA) {
+    HafJob small[HAF_MAX_JOBS];
+    int ns = 0;
+    for (int j = 0; j < nall; ++j) (all[j].N <= HAF_SMALL_MAX_N ? small[ns++] : jobs[njobs++]) = all[j];
+    if (ns) launch_small<Ar, LOOP, MODE>(small, ns, st);
+    if (njobs == 0) return (int)hipGetLastError();
+  } else {
+    for (int j = 0; j < nall; ++j) jobs[njobs++] = all[j];
+  }
   T* part[HAF_MAX_JOBS];
   double* part_abs[HAF_MAX_JOBS];
   HafReduceJobs<T> rj{};

@@ -43,6 +43,12 @@ int tor_single_max_modes();
 size_t tor_single_workspace_bytes(int tier, int nm);
 int tor_single_launch(int tier, int nm, const cplx* d_in, cplx* d_out, double* d_aux, void* ws, hipStream_t st);
 
+// torontonian schedule knobs (family 0: fp64 / kappa / certified, 1: dd /
+// certified-dd): grouped small-count kernel up to grouped_max modes (changes
+// last bits), breadth-first middle walk up to bfs_max modes (same values)
+int tor_schedule_get(int family, int* grouped_max, int* bfs_max);
+int tor_schedule_set(int family, int grouped_max, int bfs_max);
+
 // repeated-row loop hafnian (sieve): the host plan from the (B x M) host reps
 // table (one RepPat per pattern, copied to d_plan by the caller), then the chunk
 // and reduction launches; d_part holds sieve_partial_bytes(cert, parts) bytes

@@ -33,6 +33,15 @@ namespace gbsk {
 constexpr int TOR_MAX_MODES = 32;        // the kernels: dim 64 (one instance at a time at the top)
 constexpr int TOR_BATCH_MAX_MODES = 20;  // batched entry points: dim 40
 constexpr int TOR_T2 = 6;          // cooperative middle modes (64 tasks)
+// built-in schedule (tor_tune): the grouped small-count kernel up to these mode
+// counts per family, the breadth-first middle walk up to these (measured on
+// MI355X at 4096 instances: fp64 nm = 6 / 8 / 12 3.5x / 2.8x / 1.13x faster,
+// even at 16 and 4 % slower at 20, where its larger LDS costs occupancy; dd
+// 1.15x at 16)
+constexpr int TOR_GROUPED_MAX_F64 = 0;
+constexpr int TOR_GROUPED_MAX_DD = 0;
+constexpr int TOR_BFS_MAX_F64 = 16;
+constexpr int TOR_BFS_MAX_DD = 20;
 
 // real arithmetic policies: binary64 and double-double (common.hpp)
 struct RF64 {
@@ -61,7 +70,11 @@ struct RF64 {
   __device__ static T fms(T a, T b, T c) { return fma(-a, b, c); }  // c - a b
   __device__ static bool pos(T a) { return a > 0.0; }
   __device__ static double to_d(T a) { return a; }
-  __device__ static T sum64(T v) { return group_sum<64>(v); }
+  template <int W>
+  __device__ static T sumw(T v) {
+    return group_sum<W>(v);
+  }
+  __device__ static T sum64(T v) { return sumw<64>(v); }
   using P = cplx;  // partials: complex, so the complex fallback can overwrite them
   __device__ static P part(T a) { return cplx{a, 0.0}; }
   __device__ static P part_nan() { return cplx{NAN, NAN}; }
@@ -94,14 +107,17 @@ struct RDD {
   using P = dd;
   __device__ static P part(T a) { return a; }
   __device__ static P part_nan() { return dd{NAN, NAN}; }
-  __device__ static T sum64(T v) {
+  // butterfly over W lanes, operands in lane order so both partners agree
+  template <int W>
+  __device__ static T sumw(T v) {
 #pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
+    for (int m = 1; m < W; m <<= 1) {
       dd o{shfl_xor_d(v.hi, m), shfl_xor_d(v.lo, m)};
       v = ((threadIdx.x & m) == 0) ? dd_add(v, o) : dd_add(o, v);
     }
     return v;
   }
+  __device__ static T sum64(T v) { return sumw<64>(v); }
 };
 
 // any complex O (complex symmetric, Hermitian as in the complex-basis
@@ -151,7 +167,11 @@ struct RC64 {
   __device__ static T fms(T a, T b, T c) { return c_sub(c, c_mul(a, b)); }
   __device__ static bool pos(T a) { return a.re != 0.0 || a.im != 0.0; }
   __device__ static double to_d(T a) { return a.re; }
-  __device__ static T sum64(T v) { return group_sum<64>(v); }
+  template <int W>
+  __device__ static T sumw(T v) {
+    return group_sum<W>(v);
+  }
+  __device__ static T sum64(T v) { return sumw<64>(v); }
   using P = cplx;
   __device__ static P part(T a) { return a; }
   __device__ static P part_nan() { return cplx{NAN, NAN}; }
@@ -228,15 +248,17 @@ struct RCERT {
     return cv{v, fma(CERT_U, v, fma(1.0 + 2.0 * CERT_U, a.e / v, CERT_UFL))};
   }
   __device__ static bool pos(T a) { return a.v > 0.0; }
-  __device__ static T sum64(T x) {
+  template <int W>
+  __device__ static T sumw(T x) {
 #pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
+    for (int m = 1; m < W; m <<= 1) {
       const double ov = shfl_xor_d(x.v, m), oe = shfl_xor_d(x.e, m);
-      x.v += ov;  // the same butterfly as group_sum<64> of the fp64 tier
+      x.v += ov;  // the same butterfly as group_sum<W> of the fp64 tier
       x.e = fma(CERT_U, fabs(x.v), x.e + oe);
     }
     return x;
   }
+  __device__ static T sum64(T x) { return sumw<64>(x); }
   using P = cplx;
   __device__ static P part(T a) { return cplx{a.v, 0.0}; }
   __device__ static P part_nan() { return cplx{NAN, NAN}; }
@@ -312,14 +334,16 @@ struct RCDD {
     return cdd{y, ey + 2.0 * a.e / ly};
   }
   __device__ static bool pos(T a) { return a.v.hi > 0.0; }
-  __device__ static T sum64(T x) {
+  template <int W>
+  __device__ static T sumw(T x) {
 #pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
+    for (int m = 1; m < W; m <<= 1) {
       const T o{dd{shfl_xor_d(x.v.hi, m), shfl_xor_d(x.v.lo, m)}, shfl_xor_d(x.e, m)};
-      x = ((threadIdx.x & m) == 0) ? add(x, o) : add(o, x);  // RDD::sum64's order
+      x = ((threadIdx.x & m) == 0) ? add(x, o) : add(o, x);  // RDD::sumw's order
     }
     return x;
   }
+  __device__ static T sum64(T x) { return sumw<64>(x); }
   using P = dd;
   __device__ static P part(T a) { return a.v; }
   __device__ static P part_nan() { return dd{NAN, NAN}; }
@@ -328,6 +352,7 @@ struct RCDD {
 
 struct TorSplit {
   int R, T2, T1;
+  int G = 1;  // instances per wave (lane groups of 2^T2 lanes; 1 = the whole-wave kernel)
 };
 // lanes first (T2 middle modes -> up to 64 tasks), then per-lane depth R, then
 // leading patterns as separate workgroups
@@ -531,7 +556,7 @@ __device__ inline typename Ar::T eliminate(typename Ar::T* X, int D, int p) {
 // policies (a Schur complement of a symmetric matrix is symmetric: half the
 // storage and half the update), full D x D otherwise.
 template <bool SYM>
-__device__ constexpr int lev_size(int D) { return SYM ? D * (D + 1) / 2 : D * D; }
+__host__ __device__ constexpr int lev_size(int D) { return SYM ? D * (D + 1) / 2 : D * D; }
 
 // Y = X (Dx x Dx) with its first mode (rows and columns 0, 1) removed.
 template <bool SYM, class T>
@@ -561,7 +586,7 @@ __device__ inline void drop_first_mode(const T* X, int Dx, T* Y) {
 template <class Ar>
 constexpr int tor_min_waves() { return std::is_same<Ar, RF64>::value ? 3 : 1; }
 
-template <class Ar, int R, int T2, bool KAPPA, bool RERUN>
+template <class Ar, int R, int T2, bool KAPPA, bool RERUN, bool BFS>
 __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int nm, int T1, int B,
                                               typename Ar::P* __restrict__ part, double* __restrict__ part_abs,
                                               int64_t wg) {
@@ -667,114 +692,210 @@ __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int
     __syncthreads();
   }
 
-  // cooperative walk over the T2 middle modes' include / exclude tree,
-  // exclude first, one Schur complement per level (level d's Dd x Dd matrix,
-  // the next level right after it). The walk's state is wave-uniform and lives
-  // in LDS (every lane writes the same value), not in per-lane scratch. Leaf k
-  // in walk order becomes task k.
-  __shared__ T wdet[TOR_T2 + 1];
-  __shared__ int wcnt[TOR_T2 + 1], wstate[TOR_T2 + 1], woff[TOR_T2 + 1];
-  wdet[0] = det;
-  wcnt[0] = count;
-  wstate[0] = 0;
-  woff[0] = 0;
-  int ntask = 0;
-  int d = 0;
   T Mreg[TREG && SLOT > 0 ? SLOT : 1];  // TREG: this lane's task (packed lower triangle)
   T my_det = Ar::one();
   int my_cnt = 0;
-  for (;;) {
-    const int Dd = 2 * (nm - T1 - d);
-    T* X = lev + woff[d];
-    if (d == T2 && TREG) {
-      // leaf: task ntask goes to lane ntask's registers
-      if (lane == ntask) {
-#pragma unroll
-        for (int e = 0; e < SLOT; ++e) Mreg[e] = X[e];  // the leaf level is the packed task
-        my_det = wdet[d];
-        my_cnt = wcnt[d];
-      }
-      ++ntask;
-      if (d == 0) break;
-      --d;
-      continue;
+  int ntask = 0;
+  if constexpr (BFS) {
+    // Breadth-first walk over the T2 middle modes (register-task policies):
+    // level d holds the Schur complements of all 2^d include / exclude
+    // patterns of middle modes 0..d-1 (node index = pattern bits, first mode
+    // most significant, include = 1), each level formed in one pass over the 64
+    // lanes; even and odd levels alternate between two LDS buffers. Pivots,
+    // complements and determinants are the depth-first walk's operations, so
+    // the values are identical; the leaf level is formed in the lanes'
+    // registers (lane k = leaf k = task k of the depth-first order).
+    static_assert(Ar::SYM, "breadth-first walk of the symmetric policies");
+    constexpr int NODES = 1 << (T2 - 1);
+    __shared__ T bpiv[NODES][3];      // ra, rc, b01 of the current level's nodes
+    __shared__ T bdet[2][1 << T2];    // determinants: current / next level
+    const int base = nm - T1;         // modes of the level-0 matrix
+    int even = 0;                     // size of the even-level buffer
+    for (int d = 0; d < T2; d += 2) {
+      const int z = (1 << d) * lev_size<true>(2 * (base - d));
+      even = z > even ? z : even;
     }
-    if (d == T2) {
-      // leaf: the task is this level's matrix (the trailing R modes)
-      // (the leaf level has the task's layout: packed lower or full 2R x 2R)
-      for (int e = lane; e < SLOT; e += 64) slots[ntask * SLOT + e] = X[e];
-      if (lane == 0) {
-        slot_det[ntask] = wdet[d];
-        slot_cnt[ntask] = wcnt[d];
-      }
-      ++ntask;
-      if (d == 0) break;
-      --d;
-      continue;
-    }
-    T* Y = X + lev_size<Ar::SYM>(Dd);
-    const int st = wstate[d];
-    if (st == 0) {  // exclude middle mode d: drop its rows
-      wstate[d] = 1;
-      drop_first_mode<Ar::SYM>(X, Dd, Y);
-      wdet[d + 1] = wdet[d];
-      wcnt[d + 1] = wcnt[d];
-    } else if (st == 1) {
-      // include: two pivots (x then p), Schur complement into Y. With the
-      // 2 x 2 pivot block [[a, b], [b, c']], c = c' - b^2 / a, l_i0 = X_i0 / a
-      // and l_i1 = (X_i1 - l_i0 b) / c, the complement is
-      // Y_ij = X_ij - l_i0 X_0j - l_i1 (X_1j - b X_0j / a): the reciprocals of
-      // the two pivots, no division per entry
-      // Symmetric policies (packed lower): X_0j = X_j0, X_1j = X_j1, and only
-      // the lower triangle of Y is formed.
-      wstate[d] = 2;
-      const T a = X[0];
-      const T b01 = X[1];                                   // packed: X_10
-      const T b10 = Ar::SYM ? b01 : X[Dd];                  // [[a, b01], [b10, c']]
-      const T ra = Ar::recip(a);
-      const T c = Ar::sub(Ar::SYM ? X[2] : X[Dd + 1], Ar::mul(Ar::mul(b10, b01), ra));
-      const T rc = Ar::recip(c);
-      const int Dy = Dd - 2;
-      for (int e = lane; e < lev_size<Ar::SYM>(Dy); e += 64) {
-        int i, j, xi0, xi1, x0j_, x1j_, xij;
-        if constexpr (Ar::SYM) {
-          tri_decode(e, i, j);
-          i += 2;
-          j += 2;
-          xi0 = tri(i, 0);
-          xi1 = tri(i, 1);
-          x0j_ = tri(j, 0);
-          x1j_ = tri(j, 1);
-          xij = tri(i, j);
-        } else {
-          i = e / Dy + 2;
-          j = e % Dy + 2;
-          xi0 = i * Dd;
-          xi1 = i * Dd + 1;
-          x0j_ = j;
-          x1j_ = Dd + j;
-          xij = i * Dd + j;
-        }
-        const T li0 = Ar::mul(X[xi0], ra);
-        const T li1 = Ar::mul(Ar::fms(li0, b01, X[xi1]), rc);
-        const T x0j = X[x0j_];
-        const T x1j = Ar::fms(Ar::mul(x0j, ra), b10, X[x1j_]);
-        T v = Ar::fms(li0, x0j, X[xij]);
-        v = Ar::fms(li1, x1j, v);
-        Y[e] = v;
+    T* const bufs[2] = {lev, lev + even};
+    if (lane == 0) bdet[0][0] = det;
+    __syncthreads();
+    for (int d = 0; d < T2; ++d) {
+      const int Dd = 2 * (base - d), Ld = lev_size<true>(Dd), Ly = lev_size<true>(Dd - 2);
+      const T* const Xl = bufs[d & 1];
+      // pass 1: each node's two pivots and its children's determinants
+      if (lane < (1 << d)) {
+        const T* X = Xl + lane * Ld;
+        const T a = X[0];
+        const T b01 = X[1];
+        const T ra = Ar::recip(a);
+        const T c = Ar::sub(X[2], Ar::mul(Ar::mul(b01, b01), ra));
+        const T rc = Ar::recip(c);
+        bpiv[lane][0] = ra;
+        bpiv[lane][1] = rc;
+        bpiv[lane][2] = b01;
+        bad |= !Ar::pos(a) || !Ar::pos(c);
+        const T dn = bdet[d & 1][lane];
+        bdet[(d + 1) & 1][2 * lane] = dn;
+        bdet[(d + 1) & 1][2 * lane + 1] = Ar::mul(dn, Ar::mul(a, c));
       }
       __syncthreads();
-      bad |= !Ar::pos(a) || !Ar::pos(c);
-      wdet[d + 1] = Ar::mul(wdet[d], Ar::mul(a, c));
-      wcnt[d + 1] = wcnt[d] + 1;
-    } else {
-      if (d == 0) break;
-      --d;
-      continue;
+      if (d + 1 == T2) break;
+      // pass 2: every child's complement (exclude: the parent without its first
+      // mode; include: the Schur complement of the two pivots)
+      T* const Yl = bufs[(d + 1) & 1];
+      for (int idx = lane; idx < (2 << d) * Ly; idx += 64) {
+        const int child = idx / Ly, e = idx - child * Ly, nu = child >> 1;
+        const T* X = Xl + nu * Ld;
+        int i, j;
+        tri_decode(e, i, j);
+        i += 2;
+        j += 2;
+        T v;
+        if ((child & 1) == 0) {
+          v = X[tri(i, j)];
+        } else {
+          const T ra = bpiv[nu][0], rc = bpiv[nu][1], b01 = bpiv[nu][2];
+          const T li0 = Ar::mul(X[tri(i, 0)], ra);
+          const T li1 = Ar::mul(Ar::fms(li0, b01, X[tri(i, 1)]), rc);
+          const T x0j = X[tri(j, 0)];
+          const T x1j = Ar::fms(Ar::mul(x0j, ra), b01, X[tri(j, 1)]);
+          v = Ar::fms(li0, x0j, X[tri(i, j)]);
+          v = Ar::fms(li1, x1j, v);
+        }
+        Yl[idx] = v;
+      }
+      __syncthreads();
     }
-    wstate[d + 1] = 0;
-    woff[d + 1] = woff[d] + lev_size<Ar::SYM>(Dd);
-    ++d;
+    // leaves: lane k forms task k from its parent (level T2 - 1) in registers
+    ntask = 1 << T2;
+    if (lane < ntask) {
+      const int nu = lane >> 1;
+      const T* X = bufs[(T2 - 1) & 1] + nu * lev_size<true>(2 * R + 2);
+      const bool inc = lane & 1;
+      const T ra = bpiv[nu][0], rc = bpiv[nu][1], b01 = bpiv[nu][2];
+#pragma unroll
+      for (int e = 0; e < SLOT; ++e) {
+        const int i = tri_row(e) + 2, j = tri_col(e) + 2;
+        if (!inc) {
+          Mreg[e] = X[tri(i, j)];
+        } else {
+          const T li0 = Ar::mul(X[tri(i, 0)], ra);
+          const T li1 = Ar::mul(Ar::fms(li0, b01, X[tri(i, 1)]), rc);
+          const T x0j = X[tri(j, 0)];
+          const T x1j = Ar::fms(Ar::mul(x0j, ra), b01, X[tri(j, 1)]);
+          T v = Ar::fms(li0, x0j, X[tri(i, j)]);
+          Mreg[e] = Ar::fms(li1, x1j, v);
+        }
+      }
+      my_det = bdet[T2 & 1][lane];
+      my_cnt = count + __popc((unsigned)lane);
+    }
+  } else {
+    // cooperative walk over the T2 middle modes' include / exclude tree,
+    // exclude first, one Schur complement per level (level d's Dd x Dd matrix,
+    // the next level right after it). The walk's state is wave-uniform and lives
+    // in LDS (every lane writes the same value), not in per-lane scratch. Leaf k
+    // in walk order becomes task k.
+    __shared__ T wdet[TOR_T2 + 1];
+    __shared__ int wcnt[TOR_T2 + 1], wstate[TOR_T2 + 1], woff[TOR_T2 + 1];
+    wdet[0] = det;
+    wcnt[0] = count;
+    wstate[0] = 0;
+    woff[0] = 0;
+    int d = 0;
+    for (;;) {
+      const int Dd = 2 * (nm - T1 - d);
+      T* X = lev + woff[d];
+      if (d == T2 && TREG) {
+        // leaf: task ntask goes to lane ntask's registers
+        if (lane == ntask) {
+  #pragma unroll
+          for (int e = 0; e < SLOT; ++e) Mreg[e] = X[e];  // the leaf level is the packed task
+          my_det = wdet[d];
+          my_cnt = wcnt[d];
+        }
+        ++ntask;
+        if (d == 0) break;
+        --d;
+        continue;
+      }
+      if (d == T2) {
+        // leaf: the task is this level's matrix (the trailing R modes)
+        // (the leaf level has the task's layout: packed lower or full 2R x 2R)
+        for (int e = lane; e < SLOT; e += 64) slots[ntask * SLOT + e] = X[e];
+        if (lane == 0) {
+          slot_det[ntask] = wdet[d];
+          slot_cnt[ntask] = wcnt[d];
+        }
+        ++ntask;
+        if (d == 0) break;
+        --d;
+        continue;
+      }
+      T* Y = X + lev_size<Ar::SYM>(Dd);
+      const int st = wstate[d];
+      if (st == 0) {  // exclude middle mode d: drop its rows
+        wstate[d] = 1;
+        drop_first_mode<Ar::SYM>(X, Dd, Y);
+        wdet[d + 1] = wdet[d];
+        wcnt[d + 1] = wcnt[d];
+      } else if (st == 1) {
+        // include: two pivots (x then p), Schur complement into Y. With the
+        // 2 x 2 pivot block [[a, b], [b, c']], c = c' - b^2 / a, l_i0 = X_i0 / a
+        // and l_i1 = (X_i1 - l_i0 b) / c, the complement is
+        // Y_ij = X_ij - l_i0 X_0j - l_i1 (X_1j - b X_0j / a): the reciprocals of
+        // the two pivots, no division per entry
+        // Symmetric policies (packed lower): X_0j = X_j0, X_1j = X_j1, and only
+        // the lower triangle of Y is formed.
+        wstate[d] = 2;
+        const T a = X[0];
+        const T b01 = X[1];                                   // packed: X_10
+        const T b10 = Ar::SYM ? b01 : X[Dd];                  // [[a, b01], [b10, c']]
+        const T ra = Ar::recip(a);
+        const T c = Ar::sub(Ar::SYM ? X[2] : X[Dd + 1], Ar::mul(Ar::mul(b10, b01), ra));
+        const T rc = Ar::recip(c);
+        const int Dy = Dd - 2;
+        for (int e = lane; e < lev_size<Ar::SYM>(Dy); e += 64) {
+          int i, j, xi0, xi1, x0j_, x1j_, xij;
+          if constexpr (Ar::SYM) {
+            tri_decode(e, i, j);
+            i += 2;
+            j += 2;
+            xi0 = tri(i, 0);
+            xi1 = tri(i, 1);
+            x0j_ = tri(j, 0);
+            x1j_ = tri(j, 1);
+            xij = tri(i, j);
+          } else {
+            i = e / Dy + 2;
+            j = e % Dy + 2;
+            xi0 = i * Dd;
+            xi1 = i * Dd + 1;
+            x0j_ = j;
+            x1j_ = Dd + j;
+            xij = i * Dd + j;
+          }
+          const T li0 = Ar::mul(X[xi0], ra);
+          const T li1 = Ar::mul(Ar::fms(li0, b01, X[xi1]), rc);
+          const T x0j = X[x0j_];
+          const T x1j = Ar::fms(Ar::mul(x0j, ra), b10, X[x1j_]);
+          T v = Ar::fms(li0, x0j, X[xij]);
+          v = Ar::fms(li1, x1j, v);
+          Y[e] = v;
+        }
+        __syncthreads();
+        bad |= !Ar::pos(a) || !Ar::pos(c);
+        wdet[d + 1] = Ar::mul(wdet[d], Ar::mul(a, c));
+        wcnt[d + 1] = wcnt[d] + 1;
+      } else {
+        if (d == 0) break;
+        --d;
+        continue;
+      }
+      wstate[d + 1] = 0;
+      woff[d + 1] = woff[d] + lev_size<Ar::SYM>(Dd);
+      ++d;
+    }
+    __syncthreads();
   }
   __syncthreads();
 
@@ -809,7 +930,7 @@ __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int
 // took 5 ms to find nothing).
 constexpr int64_t TOR_RERUN_GRID = 8192;
 
-template <class Ar, int R, int T2, bool KAPPA, bool RERUN>
+template <class Ar, int R, int T2, bool KAPPA, bool RERUN, bool BFS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(tor_min_waves<Ar>()))) void tor_real_kernel(
     const cplx* __restrict__ mats, int nm, int T1, int B, typename Ar::P* __restrict__ part,
     double* __restrict__ part_abs) {
@@ -827,12 +948,241 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(tor_min_wave
         const int k = __builtin_ctzll(todo);
         todo &= todo - 1;
         __syncthreads();  // the previous slot's LDS use is over
-        tor_workgroup<Ar, R, T2, KAPPA, RERUN>(mats, nm, T1, B, part, part_abs, base + k);
+        tor_workgroup<Ar, R, T2, KAPPA, RERUN, BFS>(mats, nm, T1, B, part, part_abs, base + k);
       }
     }
   } else {
-    tor_workgroup<Ar, R, T2, KAPPA, RERUN>(mats, nm, T1, B, part, part_abs, blockIdx.x);
+    tor_workgroup<Ar, R, T2, KAPPA, RERUN, BFS>(mats, nm, T1, B, part, part_abs, blockIdx.x);
   }
+}
+
+// ---------------------------------------------------- small mode counts
+// When R = min(nm, RMAX) trailing modes per lane leave T2 = nm - R < 6 middle
+// modes, an instance needs only 2^T2 lanes: a wave carries G = 2^(6 - T2)
+// instances, lane group gi walking instance gi's include / exclude tree over the
+// T2 middle modes with the whole-wave kernel's level operations (each level's
+// Schur complement spread over the group's lanes, packed lower triangles in
+// LDS), one leaf task per lane, the unrolled trailing walk in VGPRs, and a
+// butterfly over the group. Every group walks the same tree shape (no leading
+// modes here), so the walk's control state is shared; determinants are per group.
+template <class Ar, int R, int T2>
+__host__ __device__ constexpr int tor_small_levtot() {
+  int t = 0;
+  for (int d = 0; d <= T2; ++d) t += lev_size<true>(2 * (T2 + R - d));
+  return t;
+}
+
+template <class Ar, int R, int T2, bool KAPPA>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(tor_min_waves<Ar>()))) void tor_small_kernel(
+    const cplx* __restrict__ mats, int B, typename Ar::P* __restrict__ part, double* __restrict__ part_abs) {
+  static_assert(Ar::SYM && Ar::TASK_REGS && Ar::ISD, "the grouped walk is the symmetric register-task one");
+  using T = typename Ar::T;
+  constexpr int nm = T2 + R, dim = 2 * nm;
+  constexpr int LPI = 1 << T2, G = 64 / LPI;
+  constexpr int LEV = tor_small_levtot<Ar, R, T2>();
+  constexpr int SLOT = tor_slot<Ar, R>();
+  extern __shared__ double lds_raw[];  // G level stacks of LEV entries (T2 > 0)
+  __shared__ T wdet[G][T2 + 1];
+  __shared__ int wcnt[T2 + 1], wstate[T2 + 1], woff[T2 + 1];
+  __shared__ int gbad[G];
+  const int lane = threadIdx.x;
+  const int gi = lane / LPI, t = lane % LPI;
+  const int64_t b = (int64_t)blockIdx.x * G + gi;
+  const bool live = b < B;
+  T* const mylev = reinterpret_cast<T*>(lds_raw) + gi * LEV;
+  if (t == 0) gbad[gi] = 0;
+  __syncthreads();
+  bool bad = false;
+  T my_det = Ar::one();
+  int my_cnt = 0;
+  T Mreg[SLOT > 0 ? SLOT : 1];
+  // A = I - O mode-major (mode i -> rows 2i, 2i+1), packed lower triangle;
+  // every entry of O is checked for the real domain
+  const cplx* O = mats + (live ? b : 0) * dim * dim;
+  if constexpr (T2 == 0) {
+    // one lane, one instance: the task is the whole matrix, loaded straight
+    // into the lane's registers
+    if (live) {
+      for (int e = 0; e < dim * dim; ++e) bad |= !Ar::domain(O[e]);
+#pragma unroll
+      for (int e = 0; e < SLOT; ++e) {
+        const int r = tri_row(e), c = tri_col(e);
+        const cplx o = O[((r >> 1) + (r & 1) * nm) * dim + (c >> 1) + (c & 1) * nm];
+        Mreg[e] = (r == c) ? Ar::one_minus(o) : Ar::neg(o);
+      }
+    }
+  } else {
+    if (live) {
+      for (int e = t; e < dim * dim; e += LPI) {
+        const int r = e / dim, c = e % dim;
+        const int gr = (r >> 1) + (r & 1) * nm, gc = (c >> 1) + (c & 1) * nm;
+        const cplx o = O[gr * dim + gc];
+        if (!Ar::domain(o)) gbad[gi] = 1;
+        if (c <= r) mylev[tri(r, c)] = (r == c) ? Ar::one_minus(o) : Ar::neg(o);
+      }
+    }
+    __syncthreads();
+    wdet[gi][0] = Ar::one();
+    wcnt[0] = 0;
+    wstate[0] = 0;
+    woff[0] = 0;
+    int ntask = 0, d = 0;
+    for (;;) {
+      const int Dd = 2 * (nm - d);
+      T* X = mylev + woff[d];
+      if (d == T2) {
+        if (t == ntask) {
+#pragma unroll
+          for (int e = 0; e < SLOT; ++e) Mreg[e] = X[e];
+          my_det = wdet[gi][d];
+          my_cnt = wcnt[d];
+        }
+        ++ntask;
+        --d;
+        continue;
+      }
+      T* Y = X + lev_size<true>(Dd);
+      const int st = wstate[d];
+      const int Dy = Dd - 2;
+      if (st == 0) {  // exclude middle mode d: drop its rows and columns
+        wstate[d] = 1;
+        for (int e = t; e < lev_size<true>(Dy); e += LPI) {
+          int i, j;
+          tri_decode(e, i, j);
+          Y[e] = X[tri(i + 2, j + 2)];
+        }
+        __syncthreads();
+        wdet[gi][d + 1] = wdet[gi][d];
+        wcnt[d + 1] = wcnt[d];
+      } else if (st == 1) {
+        // include: the two pivots (x then p) and the Schur complement, as the
+        // whole-wave kernel forms it
+        wstate[d] = 2;
+        const T a = X[0];
+        const T b01 = X[1];
+        const T ra = Ar::recip(a);
+        const T c = Ar::sub(X[2], Ar::mul(Ar::mul(b01, b01), ra));
+        const T rc = Ar::recip(c);
+        for (int e = t; e < lev_size<true>(Dy); e += LPI) {
+          int i, j;
+          tri_decode(e, i, j);
+          i += 2;
+          j += 2;
+          const T li0 = Ar::mul(X[tri(i, 0)], ra);
+          const T li1 = Ar::mul(Ar::fms(li0, b01, X[tri(i, 1)]), rc);
+          const T x0j = X[tri(j, 0)];
+          const T x1j = Ar::fms(Ar::mul(x0j, ra), b01, X[tri(j, 1)]);
+          T v = Ar::fms(li0, x0j, X[tri(i, j)]);
+          v = Ar::fms(li1, x1j, v);
+          Y[e] = v;
+        }
+        __syncthreads();
+        bad |= !Ar::pos(a) || !Ar::pos(c);
+        wdet[gi][d + 1] = Ar::mul(wdet[gi][d], Ar::mul(a, c));
+        wcnt[d + 1] = wcnt[d] + 1;
+      } else {
+        if (d == 0) break;
+        --d;
+        continue;
+      }
+      wstate[d + 1] = 0;
+      woff[d + 1] = woff[d] + lev_size<true>(Dd);
+      ++d;
+    }
+  }
+  // the task's trailing walk; bottom_walk turns the upstream determinant into
+  // its reciprocal square root first
+  T acc = Ar::zero();
+  double aacc = 0.0;
+  if (live) bottom_walk<Ar, R>(Mreg, my_det, my_cnt, nm, bad, acc, aacc);
+  if (bad) gbad[gi] = 1;
+  acc = Ar::template sumw<LPI>(acc);
+  if (KAPPA) aacc = group_sum<LPI>(aacc);
+  __syncthreads();
+  if (live && t == 0) {
+    const bool fail = gbad[gi] != 0;
+    part[b] = fail ? Ar::part_nan() : Ar::part(acc);
+    if constexpr (Ar::CERT) {
+      part_abs[b] = fail ? INFINITY : Ar::bound(acc);
+    } else if (KAPPA) {
+      part_abs[b] = fail ? NAN : aacc;
+    }
+  }
+}
+
+// Schedule knobs of the batched torontonian (host state, per policy family:
+// 0 = fp64 / kappa / certified, 1 = dd / certified-dd). grouped_max: mode counts
+// up to which the grouped small-count kernel runs (its split differs from the
+// whole-wave kernel's, so it changes values' last bits; the built-in choice is
+// fixed per family). bfs_max: mode counts up to which the whole-wave kernel
+// walks its middle tree breadth-first (values identical to the depth-first walk).
+struct TorTune {
+  int grouped_max;
+  int bfs_max;
+};
+static TorTune tor_tune[2] = {{TOR_GROUPED_MAX_F64, TOR_BFS_MAX_F64}, {TOR_GROUPED_MAX_DD, TOR_BFS_MAX_DD}};
+template <class Ar>
+static TorTune& tor_tune_of() {
+  return tor_tune[(std::is_same<Ar, RDD>::value || std::is_same<Ar, RCDD>::value) ? 1 : 0];
+}
+
+// the batched split of a policy: the grouped small-count form up to the
+// family's grouped_max modes (T1 = 0, T2 = nm - min(nm, RMAX) < TOR_T2), else
+// the whole-wave form
+template <class Ar>
+inline TorSplit tor_batch_split(int nm) {
+  if (Ar::TASK_REGS && Ar::SYM && nm <= tor_tune_of<Ar>().grouped_max) {
+    const int R = nm < Ar::RMAX ? nm : Ar::RMAX;
+    if (nm - R < TOR_T2) {
+      TorSplit s;
+      s.R = R;
+      s.T2 = nm - R;
+      s.T1 = 0;
+      s.G = 64 >> s.T2;
+      return s;
+    }
+  }
+  return tor_split(nm, Ar::RMAX);
+}
+
+template <class Ar, bool KAPPA, int R, int T2>
+static void launch_tor_small_rt(const cplx* d_in, int B, typename Ar::P* part, double* part_abs, hipStream_t st) {
+  constexpr int G = 64 >> T2;
+  const size_t lds = T2 > 0 ? (size_t)G * tor_small_levtot<Ar, R, T2>() * sizeof(typename Ar::T) : 0;
+  if (lds > 65536)  // the certified policies' wider entries: up to gfx950's 160 KiB
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&tor_small_kernel<Ar, R, T2, KAPPA>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((tor_small_kernel<Ar, R, T2, KAPPA>), dim3((unsigned)(((int64_t)B + G - 1) / G)), dim3(64),
+                     lds, st, d_in, B, part, part_abs);
+}
+
+// R = min(nm, RMAX): nm <= RMAX gives (R = nm, T2 = 0), larger nm (R = RMAX, T2 = nm - RMAX)
+template <class Ar, bool KAPPA>
+static int launch_tor_small(int nm, const cplx* d_in, int B, typename Ar::P* part, double* part_abs,
+                            hipStream_t st) {
+  constexpr int RM = Ar::RMAX;
+  switch (nm) {
+#define TOR_SMALL_CASE(NM)                                                                                 \
+  case NM:                                                                                                 \
+    if constexpr (NM - (NM < RM ? NM : RM) < TOR_T2)                                                       \
+      launch_tor_small_rt<Ar, KAPPA, (NM < RM ? NM : RM), NM - (NM < RM ? NM : RM)>(d_in, B, part, part_abs, st); \
+    else                                                                                                   \
+      return GBSK_ERR_DIM;                                                                                 \
+    break;
+    TOR_SMALL_CASE(1)
+    TOR_SMALL_CASE(2)
+    TOR_SMALL_CASE(3)
+    TOR_SMALL_CASE(4)
+    TOR_SMALL_CASE(5)
+    TOR_SMALL_CASE(6)
+    TOR_SMALL_CASE(7)
+    TOR_SMALL_CASE(8)
+    TOR_SMALL_CASE(9)
+    TOR_SMALL_CASE(10)
+#undef TOR_SMALL_CASE
+    default: return GBSK_ERR_DIM;
+  }
+  return 0;
 }
 
 // Reductions: one wave per instance, 4 instances per 256-thread block. Lane l
@@ -982,14 +1332,27 @@ __global__ void tor_n0_kernel(int B, cplx* out, double* aux, double auxval) {
   if (aux) aux[b] = auxval;
 }
 
+// LDS of the breadth-first walk: the even and odd levels' buffers (level d:
+// 2^d matrices of the modes after T1 + d), in entries
+static size_t tor_bfs_entries(int nm, const TorSplit& sp) {
+  size_t even = 0, odd = 0;
+  for (int d = 0; d < sp.T2; ++d) {
+    const size_t D = 2 * (size_t)(nm - sp.T1 - d), z = ((size_t)1 << d) * (D * (D + 1) / 2);
+    size_t& m = (d & 1) ? odd : even;
+    m = z > m ? z : m;
+  }
+  return even + odd;
+}
+
 template <class Ar>
-static size_t tor_lds_bytes(int nm, const TorSplit& sp) {
+static size_t tor_lds_bytes(int nm, const TorSplit& sp, bool bfs) {
   const int dim = 2 * nm;
   size_t lev = 0;
   for (int d = 0; d <= sp.T2; ++d) {
     const size_t Dd = 2 * (nm - sp.T1 - d);
     lev += Ar::SYM ? Dd * (Dd + 1) / 2 : Dd * Dd;  // the kernel's lev_size
   }
+  if (Ar::TASK_REGS && bfs) lev = tor_bfs_entries(nm, sp);
   const size_t slot = Ar::SYM ? (size_t)(2 * sp.R) * (2 * sp.R + 1) / 2 : (size_t)(2 * sp.R) * (2 * sp.R);
   const size_t T = sizeof(typename Ar::T);
   // task matrices in VGPRs: the level stack overlays the work matrix
@@ -999,30 +1362,51 @@ static size_t tor_lds_bytes(int nm, const TorSplit& sp) {
   return lev * T + (work > task_area ? work : task_area);
 }
 
-template <class Ar, int R, int T2, bool KAPPA, bool RERUN>
-static void launch_tor_rt(int nm, int T1, const cplx* d_in, int B, typename Ar::P* part, double* part_abs, dim3 grid,
-                          size_t lds, hipStream_t st) {
+template <class Ar, int R, int T2, bool KAPPA, bool RERUN, bool BFS>
+static void launch_tor_rt1(int nm, int T1, const cplx* d_in, int B, typename Ar::P* part, double* part_abs,
+                           dim3 grid, size_t lds, hipStream_t st) {
   if (lds > 65536)  // dim 64 at the wider tiers: up to gfx950's 160 KiB per workgroup
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&tor_real_kernel<Ar, R, T2, KAPPA, RERUN>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&tor_real_kernel<Ar, R, T2, KAPPA, RERUN, BFS>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((tor_real_kernel<Ar, R, T2, KAPPA, RERUN>), grid, dim3(64), lds, st, d_in, nm, T1, B, part,
+  hipLaunchKernelGGL((tor_real_kernel<Ar, R, T2, KAPPA, RERUN, BFS>), grid, dim3(64), lds, st, d_in, nm, T1, B, part,
                      part_abs);
 }
+
+// the whole-wave kernel, with the breadth-first middle walk when `bfs` (the
+// register-task policies only; the depth-first walk otherwise)
+template <class Ar, int R, int T2, bool KAPPA, bool RERUN>
+static void launch_tor_rt(int nm, int T1, const cplx* d_in, int B, typename Ar::P* part, double* part_abs, dim3 grid,
+                          bool bfs, hipStream_t st) {
+  const TorSplit sp{R, T2, T1};
+  if constexpr (Ar::TASK_REGS && Ar::SYM && T2 >= 1) {
+    if (bfs) {
+      launch_tor_rt1<Ar, R, T2, KAPPA, RERUN, true>(nm, T1, d_in, B, part, part_abs, grid,
+                                                    tor_lds_bytes<Ar>(nm, sp, true), st);
+      return;
+    }
+  }
+  launch_tor_rt1<Ar, R, T2, KAPPA, RERUN, false>(nm, T1, d_in, B, part, part_abs, grid,
+                                                 tor_lds_bytes<Ar>(nm, sp, false), st);
+}
+
 
 // the split's R and T2 are template arguments: T2 = 6 with R = 0..RMAX
 // (nm >= 6), or T2 = nm < 6 with R = 0
 template <class Ar, bool KAPPA, bool RERUN>
 static int launch_tor(int nm, const TorSplit& sp, const cplx* d_in, int B, typename Ar::P* part, double* part_abs,
                       hipStream_t st) {
+  if constexpr (!RERUN && Ar::TASK_REGS && Ar::SYM) {
+    if (sp.G > 1) return launch_tor_small<Ar, KAPPA>(nm, d_in, B, part, part_abs, st);
+  }
   const int np = 1 << sp.T1;
-  const size_t lds = tor_lds_bytes<Ar>(nm, sp);
+  const bool bfs = Ar::TASK_REGS && nm <= tor_tune_of<Ar>().bfs_max;
   const int64_t slots = (int64_t)B * np, scan = (slots + 63) / 64;
   const dim3 grid((unsigned)(RERUN ? (scan < TOR_RERUN_GRID ? scan : TOR_RERUN_GRID) : slots));
   if (sp.T2 == TOR_T2) {
     switch (sp.R) {
 #define TOR_CASE(RR)                                                                                             \
   case RR:                                                                                                       \
-    if constexpr (RR <= Ar::RMAX) launch_tor_rt<Ar, RR, TOR_T2, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, lds, st); \
+    if constexpr (RR <= Ar::RMAX) launch_tor_rt<Ar, RR, TOR_T2, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st); \
     break;
       TOR_CASE(0)
       TOR_CASE(1)
@@ -1037,11 +1421,11 @@ static int launch_tor(int nm, const TorSplit& sp, const cplx* d_in, int B, typen
   }
   if (sp.R != 0) return GBSK_ERR_DIM;
   switch (sp.T2) {
-    case 1: launch_tor_rt<Ar, 0, 1, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, lds, st); break;
-    case 2: launch_tor_rt<Ar, 0, 2, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, lds, st); break;
-    case 3: launch_tor_rt<Ar, 0, 3, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, lds, st); break;
-    case 4: launch_tor_rt<Ar, 0, 4, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, lds, st); break;
-    case 5: launch_tor_rt<Ar, 0, 5, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, lds, st); break;
+    case 1: launch_tor_rt<Ar, 0, 1, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st); break;
+    case 2: launch_tor_rt<Ar, 0, 2, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st); break;
+    case 3: launch_tor_rt<Ar, 0, 3, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st); break;
+    case 4: launch_tor_rt<Ar, 0, 4, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st); break;
+    case 5: launch_tor_rt<Ar, 0, 5, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st); break;
     default: return GBSK_ERR_DIM;
   }
   return 0;
@@ -1058,7 +1442,7 @@ template <class ArR, bool KAPPA>
 static int run_tor_fp64(int nm, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
   static_assert(ArR::RMAX == RF64::RMAX, "the real split must be the fp64 tier's");
   constexpr bool CERT = ArR::CERT;
-  const TorSplit sr = tor_split(nm, RF64::RMAX), sc = tor_split(nm, RC64::RMAX);
+  const TorSplit sr = tor_batch_split<ArR>(nm), sc = tor_split(nm, RC64::RMAX);
   const int np = 1 << (sr.T1 > sc.T1 ? sr.T1 : sc.T1);
   (void)np;
   cplx* part = reinterpret_cast<cplx*>(ws);
@@ -1087,7 +1471,7 @@ static int run_tor_fp64(int nm, const cplx* d_in, int B, cplx* d_out, double* d_
 }
 
 static int run_tor_dd(int nm, const cplx* d_in, int B, cplx* d_out, void* ws, hipStream_t st) {
-  const TorSplit sp = tor_split(nm, RDD::RMAX);
+  const TorSplit sp = tor_batch_split<RDD>(nm);
   const int np = 1 << sp.T1;
   dd* part = reinterpret_cast<dd*>(ws);
   int rc = launch_tor<RDD, false, false>(nm, sp, d_in, B, part, nullptr, st);
@@ -1100,7 +1484,7 @@ static int run_tor_dd(int nm, const cplx* d_in, int B, cplx* d_out, void* ws, hi
 // certified dd: the dd tier's kernels with bounds (no complex fallback: the
 // dd tiers are real-domain; off-domain instances are NaN with bound +inf)
 static int run_tor_ddcert(int nm, const cplx* d_in, int B, cplx* d_out, double* d_aux, void* ws, hipStream_t st) {
-  const TorSplit sp = tor_split(nm, RCDD::RMAX);
+  const TorSplit sp = tor_batch_split<RCDD>(nm);
   const int np = 1 << sp.T1;
   dd* part = reinterpret_cast<dd*>(ws);
   double* part_e = reinterpret_cast<double*>(part + (size_t)B * np);
@@ -1118,11 +1502,11 @@ static int run_tor_ddcert(int nm, const cplx* d_in, int B, cplx* d_out, double* 
 size_t tor_workspace_bytes(int tier, int dim, int B) {
   if (dim < 2 || (dim & 1) || dim > 2 * TOR_BATCH_MAX_MODES || B <= 0) return 0;
   if (tier == GBSK_DD || tier == GBSK_DD_CERTIFIED) {
-    const TorSplit sp = tor_split(dim / 2, RDD::RMAX);
+    const TorSplit sp = tor_batch_split<RDD>(dim / 2);
     return (size_t)B * ((size_t)1 << sp.T1) * (sizeof(dd) + sizeof(double));
   }
   if (tier != GBSK_FP64 && tier != GBSK_KAPPA && tier != GBSK_CERTIFIED) return 0;
-  const TorSplit sr = tor_split(dim / 2, RF64::RMAX), sc = tor_split(dim / 2, RC64::RMAX);
+  const TorSplit sr = tor_batch_split<RF64>(dim / 2), sc = tor_split(dim / 2, RC64::RMAX);
   return (size_t)B * (((size_t)1 << sr.T1) + ((size_t)1 << sc.T1)) * (sizeof(cplx) + sizeof(double));
 }
 
@@ -1151,7 +1535,7 @@ int tor_launch(int tier, int dim, const cplx* d_in, int B, cplx* d_out, double* 
   if (!ws) return GBSK_ERR_WORKSPACE;
   // at most 2^30 workgroups per launch: large batches run as consecutive
   // sub-batches on the same stream, reusing the workspace
-  const TorSplit s64 = tor_split(nm, RF64::RMAX), sc = tor_split(nm, RC64::RMAX), sdd = tor_split(nm, RDD::RMAX);
+  const TorSplit s64 = tor_batch_split<RF64>(nm), sc = tor_split(nm, RC64::RMAX), sdd = tor_batch_split<RDD>(nm);
   int t1 = s64.T1 > sc.T1 ? s64.T1 : sc.T1;
   t1 = sdd.T1 > t1 ? sdd.T1 : t1;
   const int64_t max_b = ((int64_t)1 << 30) >> t1;
@@ -1182,6 +1566,21 @@ int tor_launch(int tier, int dim, const cplx* d_in, int B, cplx* d_out, double* 
 // a wider leading split (tor_single_split) so that one evaluation fills the
 // GPU. Real path only: off the physical domain the value is NaN and a bound is
 // +inf (no complex fallback).
+int tor_schedule_get(int family, int* grouped_max, int* bfs_max) {
+  if (family < 0 || family > 1) return GBSK_ERR_ARG;
+  if (grouped_max) *grouped_max = tor_tune[family].grouped_max;
+  if (bfs_max) *bfs_max = tor_tune[family].bfs_max;
+  return 0;
+}
+
+int tor_schedule_set(int family, int grouped_max, int bfs_max) {
+  if (family < 0 || family > 1 || grouped_max < 0 || grouped_max > 10 || bfs_max < 0 || bfs_max > TOR_MAX_MODES)
+    return GBSK_ERR_ARG;
+  tor_tune[family].grouped_max = grouped_max;
+  tor_tune[family].bfs_max = bfs_max;
+  return 0;
+}
+
 int tor_single_max_modes() { return TOR_MAX_MODES; }
 
 static int tor_single_rmax(int tier) { return (tier == GBSK_DD || tier == GBSK_DD_CERTIFIED) ? RDD::RMAX : RF64::RMAX; }

@@ -262,6 +262,21 @@ int gbs_perm_set_lane_log2(int family, int n, int log2_terms);
 /* the split a launch of n x n permanents uses now (n = 1..28) */
 int gbs_perm_split(int family, int n, int* lane_log2, int* workgroups);
 
+/* ---- 11. tuning: the batched torontonian's schedule (no reference
+ * counterpart; it replaces the reference's fixed per-size dispatch,
+ * core/host_api.cu:177-229). family 0: fp64 / kappa / certified, 1: dd /
+ * certified-dd. grouped_max_modes (0..10): mode counts up to which several
+ * instances share a wave (lane groups of 2^(n - R) lanes, R = the family's
+ * per-lane trailing modes); this split differs from the one-instance-per-wave
+ * kernel's, so it moves values' last bits. bfs_max_modes (0..32): up to this
+ * mode count the one-instance-per-wave kernel walks its middle include/exclude
+ * tree breadth-first (one pass per level), depth-first above; both give
+ * identical values. The built-in schedule comes from measurements with
+ * gbskernels_amd/autotune.py; setting it is for tuning runs and must not
+ * overlap evaluations. */
+int gbs_tor_get_schedule(int family, int* grouped_max_modes, int* bfs_max_modes);
+int gbs_tor_set_schedule(int family, int grouped_max_modes, int bfs_max_modes);
+
 #ifdef __cplusplus
 }
 #endif

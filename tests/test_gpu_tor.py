@@ -223,3 +223,45 @@ def test_tor_dd_reads_the_real_part_under_imaginary_noise(oracle):
     # the fp64 pass sees the complex input (fallback path): the noise itself
     # moves the value by about 1e-15 times the conditioning of I - O_S
     assert np.all(np.abs(got_auto - exact) <= 1e-10 * oracle.tor_abs(O))
+
+
+def test_tor_schedules_bfs_identical_grouped_correct(ext, oracle):
+    """The breadth-first middle walk returns the depth-first walk's values and
+    auxiliaries bit for bit (every tier); the grouped small-count kernel
+    (several instances per wave, trailing modes first: a different split)
+    agrees with the oracle, keeps the fp64 / kappa / certified values identical
+    to each other, and its certificates enclose."""
+    import ctypes
+    from gbskernels_amd import _lib
+    L = _lib.lib()
+    saved = []
+    for fam in (0, 1):
+        g, b = ctypes.c_int(), ctypes.c_int()
+        _lib.check(L.gbs_tor_get_schedule(fam, ctypes.byref(g), ctypes.byref(b)), "get_schedule")
+        saved.append((g.value, b.value))
+
+    def run_all(O):
+        return (ext.tor(O), *ext.tor_kappa(O), *ext.tor_certified(O), ext.tor_dd(O), *ext.tor_dd_certified(O))
+    try:
+        for nm in (3, 6, 8, 10, 12):
+            print(f"tor schedules nm={nm}", flush=True)
+            O = physical_stack(nm, 5, 800 + nm)
+            res = {}
+            for label, gm, bfs in (("dfs", 0, 0), ("bfs", 0, 32), ("grouped", 10, 32)):
+                for fam in (0, 1):
+                    _lib.check(L.gbs_tor_set_schedule(fam, gm, bfs), "set_schedule")
+                res[label] = run_all(O)
+            for a, b in zip(res["dfs"], res["bfs"]):
+                assert np.array_equal(a, b)
+            v, kv, ka, cv, cb, dv, ddv, ddb = res["grouped"]
+            assert np.array_equal(v, kv) and np.array_equal(v, cv) and np.array_equal(dv, ddv)
+            exact = oracle.tor_dd(O)
+            absn = oracle.tor_abs(O)
+            assert np.all(np.abs(v - exact) <= tor_bound(absn, nm))
+            assert rel_err(ka, absn).max() < 1e-12
+            assert np.all(np.abs(cv - exact) <= cb)
+            assert np.all(np.abs(dv - exact) <= 1e-13 * np.abs(exact) + 1e-27 * absn)
+            assert np.all(np.abs(ddv - exact) <= ddb)
+    finally:
+        for fam, (g, b) in enumerate(saved):
+            _lib.check(L.gbs_tor_set_schedule(fam, g, b), "set_schedule")
