@@ -22,6 +22,7 @@ GBSK_ERR_DIM = 10002
 GBSK_ERR_FUNC = 10003
 GBSK_ERR_WORKSPACE = 10004
 GBSK_ERR_ALLOC = 10005
+GBSK_ERR_SCRATCH = 10006
 
 FUNCS = {"perm": 0, "haf": 1, "lhaf": 2, "tor": 3}
 TIERS = {"fp64": 0, "dd": 1, "kappa": 2, "certified": 3, "dd_certified": 4}
@@ -66,6 +67,7 @@ SIGNATURES = {
     "gbs_dev_free": ([_cp], _i),
     "gbs_workspace_bytes": ([_i, _i, _i, _i], ctypes.c_size_t),
     "gbs_eval_device": ([_i, _i, _cp, _i, _i, _cp, _cp, _cp, ctypes.c_size_t, _cp], _i),
+    "gbs_eval_device_noalloc": ([_i, _i, _cp, _i, _i, _cp, _cp, _cp], _i),
     "gbs_max_dim": ([_i, _i], _i),
     "gbs_error_string": ([_i], ctypes.c_char_p),
     "gbs_is_device_build": ([], _i),

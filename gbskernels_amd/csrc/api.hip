@@ -209,6 +209,38 @@ static int tor_single_host(int tier, const double* hO, int n, int g, double* out
   return 0;
 }
 
+// ------------------------------------------------- allocation-free evaluation
+// gbs_eval_device_noalloc's workspace: a file-scope device array that every GPU
+// the library runs on holds until the process ends. Its name is a host-side
+// stand-in, not a device address: host code uses only the per-device address
+// hipGetSymbolAddress returns, fetched once per device (with that device
+// current) on the entry point's first call there. The kernels write every
+// workspace byte they read, so nothing in it needs clearing; calls that overlap
+// on one GPU share the array and race.
+constexpr size_t SCRATCH_BYTES = (size_t)GBSK_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+constexpr int SCRATCH_DEVICES = 64;
+static std::mutex scratch_mu;
+static void* scratch_ptr[SCRATCH_DEVICES];
+
+static int scratch_address(void** out) {
+  int dev = 0;
+  int rc = (int)hipGetDevice(&dev);
+  if (rc) return rc;
+  if (dev < 0 || dev >= SCRATCH_DEVICES) return GBSK_ERR_ARG;
+  std::lock_guard<std::mutex> g(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    rc = (int)hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (rc) return rc;
+    if (!p) return GBSK_ERR_ARG;
+    scratch_ptr[dev] = p;
+  }
+  *out = scratch_ptr[dev];
+  return 0;
+}
+
 }  // namespace gbsk
 
 using namespace gbsk;
@@ -336,6 +368,21 @@ int gbs_eval_device(int func, int tier, const gbs_complex* d_in, int dim, int B,
   return launch(func, tier, (const cplx*)d_in, dim, B, (cplx*)d_out, d_aux, d_ws, ws_size, (hipStream_t)stream);
 }
 
+int gbs_eval_device_noalloc(int func, int tier, const gbs_complex* d_in, int dim, int B, gbs_complex* d_out,
+                            double* d_aux, void* stream) {
+  if (dim < 0) return GBSK_ERR_DIM;
+  if (func == GBSK_TOR && (dim & 1)) return GBSK_ERR_DIM;
+  const int cap = gbs_max_dim(func, tier);
+  if (cap == 0) return GBSK_ERR_FUNC;
+  if (dim > cap) return GBSK_ERR_DIM;
+  if (B <= 0) return 0;
+  if (ws_bytes(func, tier, dim, B) > SCRATCH_BYTES) return GBSK_ERR_SCRATCH;
+  void* ws = nullptr;
+  const int rc = scratch_address(&ws);
+  if (rc) return rc;
+  return launch(func, tier, (const cplx*)d_in, dim, B, (cplx*)d_out, d_aux, ws, SCRATCH_BYTES, (hipStream_t)stream);
+}
+
 int gbs_perm_get_lane_log2(int family, int n, int* log2_terms) {
   if (!log2_terms) return GBSK_ERR_ARG;
   const int v = perm_lane_log2_get(family, n);
@@ -380,6 +427,9 @@ const char* gbs_error_string(int rc) {
     case GBSK_ERR_FUNC: return "unknown or unsupported (function, tier)";
     case GBSK_ERR_WORKSPACE: return "workspace smaller than gbs_workspace_bytes()";
     case GBSK_ERR_ALLOC: return "host allocation failed";
+    case GBSK_ERR_SCRATCH:
+      return "batch needs more workspace than gbs_eval_device_noalloc's fixed scratch (GBSK_NOALLOC_SCRATCH_BYTES); "
+             "split the batch or use gbs_eval_device";
   }
   return hipGetErrorString((hipError_t)rc);
 }

@@ -39,6 +39,7 @@ typedef struct {
 #define GBSK_ERR_FUNC 10003
 #define GBSK_ERR_WORKSPACE 10004
 #define GBSK_ERR_ALLOC 10005
+#define GBSK_ERR_SCRATCH 10006 /* gbs_eval_device_noalloc: batch needs more than its fixed scratch */
 
 /* function codes (core/host_api.cu:474-482: 0 perm, 1 haf, 2 lhaf, 3 tor) */
 #define GBSK_PERM 0
@@ -111,6 +112,18 @@ int gbs_dev_free(gbs_complex* d);
 size_t gbs_workspace_bytes(int func, int tier, int dim, int B);
 int gbs_eval_device(int func, int tier, const gbs_complex* d_in, int dim, int B, gbs_complex* d_out, double* d_aux,
                     void* d_ws, size_t ws_bytes, void* stream);
+/* gbs_eval_device without a workspace argument and without any allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The workspace is a fixed device array of
+ * GBSK_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the
+ * process ends; a batch needing more (gbs_workspace_bytes) returns
+ * GBSK_ERR_SCRATCH. Calls that overlap on one GPU (any streams, any threads)
+ * share that array and race: serialise them. Values are those of
+ * gbs_eval_device, bit for bit. */
+#define GBSK_NOALLOC_SCRATCH_BYTES (64u << 20)
+int gbs_eval_device_noalloc(int func, int tier, const gbs_complex* d_in, int dim, int B, gbs_complex* d_out,
+                            double* d_aux, void* stream);
 /* largest supported matrix side for (func, tier), 0 if the pair is unsupported */
 int gbs_max_dim(int func, int tier);
 const char* gbs_error_string(int rc);
