@@ -604,7 +604,10 @@ def perm_tier_cells(L, torch, dev, stream):
     certified and certified-dd kernels at n = 16 (B = 4096), 20 (B = 512) and
     24 (B = 64), config-2 style rows of a Haar unitary."""
     from gbskernels_amd.workloads import haar_unitary, perm_rows
-    cells = []
+    from gbskernels_amd.workloads import perm_haar_rows_workload
+    # the headline's n = 24 launch as a cell of its own (profiler runs)
+    _, m24 = perm_haar_rows_workload(take16=0, take24=1024)
+    cells = [device_cell(L, torch, dev, stream, "perm_fp64_n24", 0, 0, m24, flops=perm_flops(24))]
     for n, B in ((16, 4096), (20, 512), (24, 64)):
         rng = np.random.default_rng(40 + n)
         U = haar_unitary(64, rng)
@@ -643,8 +646,8 @@ def tor_flops(nm: int) -> float:
 def tor_cells(L, torch, dev, stream):
     """Torontonian of threshold-detector O matrices (two-mode squeezed pairs,
     r ~ U[0.3, 1.0], Haar interferometer, loss 0.7), 4096 per mode count
-    (512 distinct, tiled; certified-dd at 20 modes: 512), fp64 / certified /
-    certified-dd; plus the hafnian of config 1: 4096 real symmetric 8x8
+    (512 distinct, tiled; dd and certified-dd at 20 modes: 512), fp64 /
+    certified / dd / certified-dd; plus the hafnian of config 1: 4096 real symmetric 8x8
     matrices G + G^T, default_rng(0)."""
     from gbskernels_amd.workloads import threshold_O
     cells = []
@@ -652,10 +655,10 @@ def tor_cells(L, torch, dev, stream):
         rng = np.random.default_rng(4000 + nm)
         uniq = np.stack([threshold_O(nm, rng) for _ in range(512)])
         m = np.ascontiguousarray(np.tile(uniq, (8, 1, 1)))
-        for tier, tname in ((0, "fp64"), (3, "certified"), (4, "dd_certified")):
-            B = m.shape[0] if (tier != 4 or nm <= 16) else 512
+        for tier, tname in ((0, "fp64"), (3, "certified"), (1, "dd"), (4, "dd_certified")):
+            B = m.shape[0] if (tier not in (1, 4) or nm <= 16) else 512
             cells.append(device_cell(L, torch, dev, stream, f"tor_{tname}_nm{nm}", 3, tier, m[:B],
-                                     flops=tor_flops(nm) * (10 if tier == 4 else 1)))
+                                     flops=tor_flops(nm) * (10 if tier in (1, 4) else 1)))
     G = np.random.default_rng(0).standard_normal((4096, 8, 8))
     cells.append(device_cell(L, torch, dev, stream, "haf_fp64_config1_n8", 1, 0, G + G.transpose(0, 2, 1),
                              flops=haf_flops(8)))

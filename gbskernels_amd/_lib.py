@@ -92,6 +92,8 @@ SIGNATURES = {
     "gbs_perm_get_lane_log2": ([_i, _i, _cp], _i),
     "gbs_perm_set_lane_log2": ([_i, _i, _i], _i),
     "gbs_perm_split": ([_i, _i, _cp, _cp], _i),
+    "gbs_haf_get_schedule": ([_cp, _cp], _i),
+    "gbs_haf_set_schedule": ([_i, _i], _i),
     "gbs_tor_get_schedule": ([_i, _cp, _cp], _i),
     "gbs_tor_set_schedule": ([_i, _i, _i], _i),
     "gbs_lhaf_repeated_cert_host": ([_cp, _cp, _i, _cp, _i, _cp, _cp], _i),

@@ -393,6 +393,14 @@ int gbs_perm_get_lane_log2(int family, int n, int* log2_terms) {
 
 int gbs_perm_set_lane_log2(int family, int n, int log2_terms) { return perm_lane_log2_set(family, n, log2_terms); }
 
+int gbs_haf_get_schedule(int* small_max_n, int* small_target_wgs) {
+  return haf_schedule_get(small_max_n, small_target_wgs);
+}
+
+int gbs_haf_set_schedule(int small_max_n, int small_target_wgs) {
+  return haf_schedule_set(small_max_n, small_target_wgs);
+}
+
 int gbs_tor_get_schedule(int family, int* grouped_max_modes, int* bfs_max_modes) {
   return tor_schedule_get(family, grouped_max_modes, bfs_max_modes);
 }

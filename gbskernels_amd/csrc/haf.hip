@@ -4,6 +4,23 @@
 
 namespace gbsk {
 
+// built-in schedule of the fused small-size kernel (measured on MI355X)
+int haf_small_max_n = 12;
+int haf_small_target_wgs = 2048;
+
+int haf_schedule_get(int* small_max_n, int* small_target_wgs) {
+  if (small_max_n) *small_max_n = haf_small_max_n;
+  if (small_target_wgs) *small_target_wgs = haf_small_target_wgs;
+  return 0;
+}
+
+int haf_schedule_set(int small_max_n, int small_target_wgs) {
+  if (small_max_n < 0 || small_max_n > HAF_SMALL_MAX_N || small_target_wgs < 1) return GBSK_ERR_ARG;
+  haf_small_max_n = small_max_n;
+  haf_small_target_wgs = small_target_wgs;
+  return 0;
+}
+
 // N == 0: value 1 (kappa indicator 1, certificate 0); odd N: haf 0 exactly, lhaf NaN
 __global__ void haf_trivial_kernel(int N, bool loop, int B, cplx* out, double* aux, bool kappa) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

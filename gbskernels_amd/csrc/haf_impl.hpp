@@ -914,10 +914,14 @@ __global__ __launch_bounds__(64) void haf_small_kernel(const HafSmallJobs jobs) 
   }
 }
 
-// instances per workgroup of the fused small-size kernel: enough workgroups to
-// give every SIMD two waves where the batch allows, at most HAF_SMALL_MAX_T
+// Schedule of the fused small-size kernel (host state, haf.hip; neither knob
+// changes a value): it runs sizes N <= haf_small_max_n, with as many instances
+// per workgroup (at most HAF_SMALL_MAX_T) as keep at least haf_small_target_wgs
+// workgroups in the launch.
+extern int haf_small_max_n;
+extern int haf_small_target_wgs;
 inline int haf_small_T(int B) {
-  const int t = B / 2048;
+  const int t = B / (haf_small_target_wgs > 0 ? haf_small_target_wgs : 1);
   return t < 1 ? 1 : (t > HAF_SMALL_MAX_T ? HAF_SMALL_MAX_T : t);
 }
 
@@ -982,7 +986,8 @@ int run_haf_multi(const HafJob* all, int nall, hipStream_t st) {
   if constexpr (MODE == HM_PLAIN || MODE == HM_KAPPA) {
     HafJob small[HAF_MAX_JOBS];
     int ns = 0;
-    for (int j = 0; j < nall; ++j) (all[j].N <= HAF_SMALL_MAX_N ? small[ns++] : jobs[njobs++]) = all[j];
+    const int small_n = haf_small_max_n < HAF_SMALL_MAX_N ? haf_small_max_n : HAF_SMALL_MAX_N;
+    for (int j = 0; j < nall; ++j) (all[j].N <= small_n ? small[ns++] : jobs[njobs++]) = all[j];
     if (ns) launch_small<Ar, LOOP, MODE>(small, ns, st);
     if (njobs == 0) return (int)hipGetLastError();
   } else {

@@ -43,6 +43,12 @@ int tor_single_max_modes();
 size_t tor_single_workspace_bytes(int tier, int nm);
 int tor_single_launch(int tier, int nm, const cplx* d_in, cplx* d_out, double* d_aux, void* ws, hipStream_t st);
 
+// fused small-size hafnian kernel: sizes N <= small_max_n (0..12), instances
+// per workgroup chosen to keep at least small_target_wgs workgroups (values
+// never depend on either)
+int haf_schedule_get(int* small_max_n, int* small_target_wgs);
+int haf_schedule_set(int small_max_n, int small_target_wgs);
+
 // torontonian schedule knobs (family 0: fp64 / kappa / certified, 1: dd /
 // certified-dd): grouped small-count kernel up to grouped_max modes (changes
 // last bits), breadth-first middle walk up to bfs_max modes (same values)

@@ -34,14 +34,14 @@ constexpr int TOR_MAX_MODES = 32;        // the kernels: dim 64 (one instance at
 constexpr int TOR_BATCH_MAX_MODES = 20;  // batched entry points: dim 40
 constexpr int TOR_T2 = 6;          // cooperative middle modes (64 tasks)
 // built-in schedule (tor_tune): the grouped small-count kernel up to these mode
-// counts per family, the breadth-first middle walk up to these (measured on
-// MI355X at 4096 instances: fp64 nm = 6 / 8 / 12 3.5x / 2.8x / 1.13x faster,
-// even at 16 and 4 % slower at 20, where its larger LDS costs occupancy; dd
-// 1.15x at 16)
+// counts per family (off: the breadth-first whole-wave kernel was faster at
+// every mode count), the breadth-first middle walk up to these (every count:
+// measured on MI355X, call times fp64 nm = 6 / 8 / 12 / 16 / 20 4.3x / 3.6x /
+// 2.0x / 1.6x / 1.5x faster than depth-first, dd 1.5x-4x, certified 1.2x-3.8x)
 constexpr int TOR_GROUPED_MAX_F64 = 0;
 constexpr int TOR_GROUPED_MAX_DD = 0;
-constexpr int TOR_BFS_MAX_F64 = 16;
-constexpr int TOR_BFS_MAX_DD = 20;
+constexpr int TOR_BFS_MAX_F64 = 32;
+constexpr int TOR_BFS_MAX_DD = 32;
 
 // real arithmetic policies: binary64 and double-double (common.hpp)
 struct RF64 {
@@ -697,41 +697,68 @@ __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int
   int my_cnt = 0;
   int ntask = 0;
   if constexpr (BFS) {
-    // Breadth-first walk over the T2 middle modes (register-task policies):
-    // level d holds the Schur complements of all 2^d include / exclude
-    // patterns of middle modes 0..d-1 (node index = pattern bits, first mode
-    // most significant, include = 1), each level formed in one pass over the 64
-    // lanes; even and odd levels alternate between two LDS buffers. Pivots,
-    // complements and determinants are the depth-first walk's operations, so
-    // the values are identical; the leaf level is formed in the lanes'
-    // registers (lane k = leaf k = task k of the depth-first order).
+    // Breadth-first walk over the T2 middle modes (register-task policies).
+    // Level d has the 2^d include / exclude patterns of middle modes 0..d-1
+    // (node index = pattern bits, first mode most significant, include = 1).
+    // Only include children are formed: excluding a mode drops the first two
+    // rows and columns, and the packed lower-triangle index tri(i, j) does not
+    // depend on the matrix size, so an exclude child is its parent read with a
+    // shift of two rows. A node reads the matrix formed by its last include
+    // (the level-0 matrix if none), shifted by two rows per exclude since.
+    // Each level's new matrices are formed in one pass over the 64 lanes.
+    // Pivots, complements and determinants are the depth-first walk's
+    // operations, so the values are identical; the leaf level is formed in the
+    // lanes' registers (lane k = leaf k = task k of the depth-first order).
     static_assert(Ar::SYM, "breadth-first walk of the symmetric policies");
     constexpr int NODES = 1 << (T2 - 1);
-    __shared__ T bpiv[NODES][3];      // ra, rc, b01 of the current level's nodes
-    __shared__ T bdet[2][1 << T2];    // determinants: current / next level
-    const int base = nm - T1;         // modes of the level-0 matrix
-    int even = 0;                     // size of the even-level buffer
-    for (int d = 0; d < T2; d += 2) {
-      const int z = (1 << d) * lev_size<true>(2 * (base - d));
-      even = z > even ? z : even;
+    __shared__ T bpiv[NODES][3];         // ra, rc, b01 of the current level's nodes
+    __shared__ T bdet[2][1 << T2];       // determinants: current / next level
+    __shared__ int boff[NODES], bsh[NODES];  // each node's matrix offset and row shift
+    __shared__ int region[TOR_T2 + 1];   // first entry of level d's formed matrices
+    const int base = nm - T1;            // modes of the level-0 matrix
+    // (row, column) of every packed lower-triangle index a formed matrix can
+    // hold (at most 2 (TOR_T2 + 5) - 2 rows below level 0)
+    constexpr int TIJ = 256;
+    static_assert(lev_size<true>(2 * (TOR_T2 + 5) - 2) <= TIJ, "triangle table");
+    __shared__ unsigned char tij_r[TIJ], tij_c[TIJ];
+    for (int e = lane; e < TIJ; e += 64) {
+      int i, j;
+      tri_decode(e, i, j);
+      tij_r[e] = (unsigned char)i;
+      tij_c[e] = (unsigned char)j;
     }
-    T* const bufs[2] = {lev, lev + even};
-    if (lane == 0) bdet[0][0] = det;
+    if (lane == 0) {
+      bdet[0][0] = det;
+      int r = lev_size<true>(2 * base);  // level 1's region follows the level-0 matrix
+      region[0] = 0;
+      for (int d = 1; d <= T2; ++d) {
+        region[d] = r;
+        r += (1 << (d - 1)) * lev_size<true>(2 * (base - d));
+      }
+    }
     __syncthreads();
     for (int d = 0; d < T2; ++d) {
-      const int Dd = 2 * (base - d), Ld = lev_size<true>(Dd), Ly = lev_size<true>(Dd - 2);
-      const T* const Xl = bufs[d & 1];
-      // pass 1: each node's two pivots and its children's determinants
+      // pass 1: each node's matrix (offset, shift), pivots and its children's
+      // determinants
       if (lane < (1 << d)) {
-        const T* X = Xl + lane * Ld;
-        const T a = X[0];
-        const T b01 = X[1];
+        int off = 0, sh = 2 * d;
+        if (lane != 0) {
+          const int p = __builtin_ctz((unsigned)lane);  // excludes since the last include
+          const int lv = d - p;                         // level of the matrix that include formed
+          off = region[lv] + (lane >> (p + 1)) * lev_size<true>(2 * (base - lv));
+          sh = 2 * p;
+        }
+        const T* X = lev + off;
+        const T a = X[tri(sh, sh)];
+        const T b01 = X[tri(sh + 1, sh)];
         const T ra = Ar::recip(a);
-        const T c = Ar::sub(X[2], Ar::mul(Ar::mul(b01, b01), ra));
+        const T c = Ar::sub(X[tri(sh + 1, sh + 1)], Ar::mul(Ar::mul(b01, b01), ra));
         const T rc = Ar::recip(c);
         bpiv[lane][0] = ra;
         bpiv[lane][1] = rc;
         bpiv[lane][2] = b01;
+        boff[lane] = off;
+        bsh[lane] = sh;
         bad |= !Ar::pos(a) || !Ar::pos(c);
         const T dn = bdet[d & 1][lane];
         bdet[(d + 1) & 1][2 * lane] = dn;
@@ -739,49 +766,51 @@ __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int
       }
       __syncthreads();
       if (d + 1 == T2) break;
-      // pass 2: every child's complement (exclude: the parent without its first
-      // mode; include: the Schur complement of the two pivots)
-      T* const Yl = bufs[(d + 1) & 1];
-      for (int idx = lane; idx < (2 << d) * Ly; idx += 64) {
-        const int child = idx / Ly, e = idx - child * Ly, nu = child >> 1;
-        const T* X = Xl + nu * Ld;
-        int i, j;
-        tri_decode(e, i, j);
-        i += 2;
-        j += 2;
-        T v;
-        if ((child & 1) == 0) {
-          v = X[tri(i, j)];
-        } else {
-          const T ra = bpiv[nu][0], rc = bpiv[nu][1], b01 = bpiv[nu][2];
-          const T li0 = Ar::mul(X[tri(i, 0)], ra);
-          const T li1 = Ar::mul(Ar::fms(li0, b01, X[tri(i, 1)]), rc);
-          const T x0j = X[tri(j, 0)];
-          const T x1j = Ar::fms(Ar::mul(x0j, ra), b01, X[tri(j, 1)]);
-          v = Ar::fms(li0, x0j, X[tri(i, j)]);
-          v = Ar::fms(li1, x1j, v);
-        }
-        Yl[idx] = v;
+      // pass 2: each node's include child (the Schur complement of its two
+      // pivots), formed at region[d + 1] + node * Ly. The flat index (node,
+      // entry) is split without an integer division (a float reciprocal, exact
+      // to one step below 2^24, then corrected) and the entry's (row, column)
+      // comes from the shared table.
+      const int Ly = lev_size<true>(2 * (base - d - 1));
+      T* const Y = lev + region[d + 1];
+      const int total = (1 << d) * Ly;
+      const float rLy = 1.0f / (float)Ly;
+      for (int idx = lane; idx < total; idx += 64) {
+        int nu = (int)((float)idx * rLy);
+        if (nu * Ly > idx)
+          --nu;
+        else if ((nu + 1) * Ly <= idx)
+          ++nu;
+        const int e = idx - nu * Ly, sh = bsh[nu];
+        const T* X = lev + boff[nu];
+        const int i = tij_r[e] + 2 + sh, j = tij_c[e] + 2 + sh;
+        const T ra = bpiv[nu][0], rc = bpiv[nu][1], b01 = bpiv[nu][2];
+        const T li0 = Ar::mul(X[tri(i, sh)], ra);
+        const T li1 = Ar::mul(Ar::fms(li0, b01, X[tri(i, sh + 1)]), rc);
+        const T x0j = X[tri(j, sh)];
+        const T x1j = Ar::fms(Ar::mul(x0j, ra), b01, X[tri(j, sh + 1)]);
+        T v = Ar::fms(li0, x0j, X[tri(i, j)]);
+        Y[idx] = Ar::fms(li1, x1j, v);
       }
       __syncthreads();
     }
     // leaves: lane k forms task k from its parent (level T2 - 1) in registers
     ntask = 1 << T2;
     if (lane < ntask) {
-      const int nu = lane >> 1;
-      const T* X = bufs[(T2 - 1) & 1] + nu * lev_size<true>(2 * R + 2);
+      const int nu = lane >> 1, sh = bsh[nu];
+      const T* X = lev + boff[nu];
       const bool inc = lane & 1;
       const T ra = bpiv[nu][0], rc = bpiv[nu][1], b01 = bpiv[nu][2];
 #pragma unroll
       for (int e = 0; e < SLOT; ++e) {
-        const int i = tri_row(e) + 2, j = tri_col(e) + 2;
+        const int i = tri_row(e) + 2 + sh, j = tri_col(e) + 2 + sh;
         if (!inc) {
           Mreg[e] = X[tri(i, j)];
         } else {
-          const T li0 = Ar::mul(X[tri(i, 0)], ra);
-          const T li1 = Ar::mul(Ar::fms(li0, b01, X[tri(i, 1)]), rc);
-          const T x0j = X[tri(j, 0)];
-          const T x1j = Ar::fms(Ar::mul(x0j, ra), b01, X[tri(j, 1)]);
+          const T li0 = Ar::mul(X[tri(i, sh)], ra);
+          const T li1 = Ar::mul(Ar::fms(li0, b01, X[tri(i, sh + 1)]), rc);
+          const T x0j = X[tri(j, sh)];
+          const T x1j = Ar::fms(Ar::mul(x0j, ra), b01, X[tri(j, sh + 1)]);
           T v = Ar::fms(li0, x0j, X[tri(i, j)]);
           Mreg[e] = Ar::fms(li1, x1j, v);
         }
@@ -1332,16 +1361,16 @@ __global__ void tor_n0_kernel(int B, cplx* out, double* aux, double auxval) {
   if (aux) aux[b] = auxval;
 }
 
-// LDS of the breadth-first walk: the even and odd levels' buffers (level d:
-// 2^d matrices of the modes after T1 + d), in entries
+// LDS entries of the breadth-first walk: the level-0 matrix and, per level
+// d = 1 .. T2 - 1, the 2^(d-1) matrices its include children form
 static size_t tor_bfs_entries(int nm, const TorSplit& sp) {
-  size_t even = 0, odd = 0;
-  for (int d = 0; d < sp.T2; ++d) {
-    const size_t D = 2 * (size_t)(nm - sp.T1 - d), z = ((size_t)1 << d) * (D * (D + 1) / 2);
-    size_t& m = (d & 1) ? odd : even;
-    m = z > m ? z : m;
+  const size_t base = (size_t)(nm - sp.T1);
+  size_t t = base * (2 * base + 1);  // lev_size(2 base)
+  for (int d = 1; d < sp.T2; ++d) {
+    const size_t D = 2 * (base - d);
+    t += ((size_t)1 << (d - 1)) * (D * (D + 1) / 2);
   }
-  return even + odd;
+  return t;
 }
 
 template <class Ar>

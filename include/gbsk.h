@@ -275,7 +275,19 @@ int gbs_perm_set_lane_log2(int family, int n, int log2_terms);
 /* the split a launch of n x n permanents uses now (n = 1..28) */
 int gbs_perm_split(int family, int n, int* lane_log2, int* workgroups);
 
-/* ---- 11. tuning: the batched torontonian's schedule (no reference
+/* ---- 11. tuning: the hafnian's fused small-size kernel (no reference
+ * counterpart; it replaces the reference's small-cap dispatch,
+ * core/host_api.cu:162-167). Matrices of side N <= small_max_n (0..12) of the
+ * fp64, kappa and dd tiers run in one fused launch (all subset sizes and the
+ * final sum), several instances per workgroup, as many as keep at least
+ * small_target_wgs workgroups; larger sides run one launch per subset size.
+ * Both paths give identical values. The built-in values come from measurements
+ * with gbskernels_amd/autotune.py; setting them is for tuning runs and must not
+ * overlap evaluations. */
+int gbs_haf_get_schedule(int* small_max_n, int* small_target_wgs);
+int gbs_haf_set_schedule(int small_max_n, int small_target_wgs);
+
+/* ---- 12. tuning: the batched torontonian's schedule (no reference
  * counterpart; it replaces the reference's fixed per-size dispatch,
  * core/host_api.cu:177-229). family 0: fp64 / kappa / certified, 1: dd /
  * certified-dd. grouped_max_modes (0..10): mode counts up to which several

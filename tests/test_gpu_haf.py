@@ -207,3 +207,28 @@ def test_dd_certified_cancellation(ext, oracle):
     assert bound[0] / abs(exact) < 1e-14
     v0, b0 = ext.haf_dd_certified(np.zeros((2, 0, 0), complex))
     assert np.all(v0 == 1) and np.all(b0 == 0)
+
+
+@pytest.mark.parametrize("func", ["haf", "lhaf"])
+def test_fused_small_kernel_equals_per_size_launches(ext, func):
+    """The fused small-size kernel (all subset sizes and the final sum in one
+    launch, several instances per workgroup) returns the per-subset-size
+    launches' values and kappa sums bit for bit, at every side it covers, for
+    every workgroup packing."""
+    import ctypes
+    from gbskernels_amd import _lib
+    L = _lib.lib()
+    m0, t0 = ctypes.c_int(), ctypes.c_int()
+    _lib.check(L.gbs_haf_get_schedule(ctypes.byref(m0), ctypes.byref(t0)), "get_schedule")
+    try:
+        for N in (2, 4, 6, 8, 10, 12):
+            A = rng_symmetric(37, N, 700 + N)
+            _lib.check(L.gbs_haf_set_schedule(0, t0.value), "set_schedule")
+            ref = [getattr(ext, func)(A), *getattr(ext, f"{func}_kappa")(A), getattr(ext, f"{func}_dd")(A)]
+            for target in (1, 12, 18, 2048):  # 4, 3, 2, 1 instances per workgroup
+                _lib.check(L.gbs_haf_set_schedule(12, target), "set_schedule")
+                got = [getattr(ext, func)(A), *getattr(ext, f"{func}_kappa")(A), getattr(ext, f"{func}_dd")(A)]
+                for a, b in zip(ref, got):
+                    assert np.array_equal(a, b), (N, target)
+    finally:
+        _lib.check(L.gbs_haf_set_schedule(m0.value, t0.value), "set_schedule")

@@ -48,3 +48,20 @@ def test_write_table_rewrites_only_the_given_entries(tmp_path):
 def test_batches_cover_the_config_sizes():
     assert autotune.batch_for(16, 0) == 65536 and autotune.batch_for(24, 0) == 1024
     assert autotune.batch_for(28, 0) == 64 and autotune.batch_for(28, 1) == 16
+
+
+def test_rewrite_constant_touches_only_the_named_built_in(tmp_path):
+    """--tor / --haf --write rewrite one integer constant each, in place."""
+    from gbskernels_amd import autotune
+    for src, names in ((autotune.TOR_SOURCE, ("TOR_BFS_MAX_F64", "TOR_BFS_MAX_DD", "TOR_GROUPED_MAX_F64",
+                                              "TOR_GROUPED_MAX_DD")),
+                       (autotune.HAF_SOURCE, ("haf_small_max_n", "haf_small_target_wgs"))):
+        out = tmp_path / src.name
+        out.write_text(src.read_text())
+        for k, name in enumerate(names):
+            autotune._rewrite_constant(src, name, 7 + k, out)
+        text = out.read_text()
+        for k, name in enumerate(names):
+            assert f"{name} = {7 + k};" in text
+        changed = [a for a, b in zip(src.read_text().splitlines(), text.splitlines()) if a != b]
+        assert len(changed) == len(names)
