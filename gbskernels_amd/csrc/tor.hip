@@ -544,9 +544,21 @@ __device__ inline typename Ar::T eliminate(typename Ar::T* X, int D, int p) {
   const typename Ar::T rp = Ar::recip(piv);
   const int n = D - p - 1;
   __syncthreads();
-  for (int e = threadIdx.x; e < n * n; e += 64) {
-    const int i = p + 1 + e / n, j = p + 1 + e % n;
-    X[i * D + j] = Ar::fms(Ar::mul(X[i * D + p], rp), X[p * D + j], X[i * D + j]);
+  if (n > 0) {
+    // lane entries e = lane, lane + 64, ... of the n x n trailing block, the
+    // (row, column) stepped without a division per entry
+    int i = (int)threadIdx.x / n, j = (int)threadIdx.x - i * n;
+    const int di = 64 / n, dj = 64 - di * n;
+    for (int e = threadIdx.x; e < n * n; e += 64) {
+      const int r = p + 1 + i, c = p + 1 + j;
+      X[r * D + c] = Ar::fms(Ar::mul(X[r * D + p], rp), X[p * D + c], X[r * D + c]);
+      i += di;
+      j += dj;
+      if (j >= n) {
+        j -= n;
+        ++i;
+      }
+    }
   }
   __syncthreads();
   return piv;
@@ -638,12 +650,21 @@ __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int
   // A = I - O, mode-major: W[(2i+s)][(2j+t)] = delta - O[mi + s*nm][mj + t*nm]
   // for active modes mi = amode[i], mj = amode[j]
   const cplx* O = mats + b * dim * dim;
-  for (int e = lane; e < D * D; e += 64) {
-    const int r = e / D, c = e % D;
-    const int gr = amode[r >> 1] + (r & 1) * nm, gc = amode[c >> 1] + (c & 1) * nm;
-    const cplx o = O[gr * dim + gc];
-    W[e] = (r == c) ? Ar::one_minus(o) : Ar::neg(o);
-    if (!Ar::domain(o)) bad_flag = 1;  // complex O: not the real kernels' domain
+  {
+    int r = lane / D, c = lane - r * D;  // (row, column) of entry e, stepped per iteration
+    const int dr = 64 / D, dc = 64 - dr * D;
+    for (int e = lane; e < D * D; e += 64) {
+      const int gr = amode[r >> 1] + (r & 1) * nm, gc = amode[c >> 1] + (c & 1) * nm;
+      const cplx o = O[gr * dim + gc];
+      W[e] = (r == c) ? Ar::one_minus(o) : Ar::neg(o);
+      if (!Ar::domain(o)) bad_flag = 1;  // complex O: not the real kernels' domain
+      r += dr;
+      c += dc;
+      if (c >= D) {
+        c -= D;
+        ++r;
+      }
+    }
   }
   __syncthreads();
 
