@@ -16,7 +16,9 @@ The value is evals/s of the whole job (65536 + 1024 evaluations per step).
 The roofline object describes the dominant kernel (n = 24): canonical flops
 2^(n-1) (8n - 4) per evaluation (complex MAC = 8 real flops) over its average
 launch duration measured with HIP events on the launch stream, against the
-MI355X FP64 vector peak of 78.6 TFLOP/s. cpu_baseline times the CPU oracle
+MI355X FP64 vector peak of 78.6 TFLOP/s; executed_tflops / executed_frac give
+the FP64 work the kernel issues (hardware counters, EXECUTED_F64) over the
+same time. cpu_baseline times the CPU oracle
 (a restatement of the reference's cpu_ref Glynn walk) on the host cores on a
 bounded sample of the same workload.
 """
@@ -41,6 +43,25 @@ PEAK_HBM = 8.0e12
 # HBM bytes of one perm_kernel<24, fp64> launch (1024 x 24x24), measured with
 # hardware counters (FETCH_SIZE, WRITE_SIZE) calibrated against a plain copy
 TRAFFIC_N24 = 13742848
+# Executed FP64 work per evaluation of a cell, measured with hardware counters
+# (SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 summed over the cell's dispatches,
+# divided by its batch; these kernels' instruction streams do not depend on
+# the input values): (flop, FP64 VALU wave instructions). flop counts 64
+# lanes per wave instruction, 2 per FMA. Canonical flops count the reference
+# algorithm as written; executed flops count what the kernels issue.
+EXECUTED_F64 = {}
+# FP64 VALU wave instructions per second at the 78.6 TFLOP/s peak: one per CU
+# per clock (256 CUs, 2.4 GHz; an FMA over 64 lanes is 128 flop)
+FP64_ISSUE_PER_S = 256 * 2.4e9
+
+
+def executed(name: str, evals: int, seconds: float) -> dict:
+    """executed TFLOP/s and FP64 issue fraction of a cell, from its counters"""
+    if name not in EXECUTED_F64 or seconds <= 0:
+        return {}
+    flop, insts = EXECUTED_F64[name]
+    return {"tflops_executed": flop * evals / seconds / 1e12,
+            "fp64_issue_frac": insts * evals / seconds / FP64_ISSUE_PER_S}
 
 
 def workload():
@@ -97,6 +118,9 @@ def main() -> int:
     ap.add_argument("--cells", type=lambda t: [c for c in t.split(",") if c], default=None,
                     help="comma-separated side-kernel cells (e.g. haf_fp64_n24,tor_fp64_nm20): time only "
                          "these, print their record and exit (for profiler runs)")
+    ap.add_argument("--counter-pass", action="store_true",
+                    help="with --cells: run each named cell once, in the order given, with a one-element "
+                         "fill kernel after each (splits a hardware-counter run by cell), and exit")
     ap.add_argument("--detail", default="gpurun_out/bench_detail.json",
                     help="file receiving the full measurement record (JSON)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -131,11 +155,28 @@ def main() -> int:
         return 0
     stream = torch.cuda.current_stream(dev)
     sp = stream.cuda_stream
+    if args.cells and args.counter_pass:
+        made = {}
+        for make in (perm_tier_cells, haf_cells, tor_cells):
+            for c in (make(L, torch, dev, stream, headline=True) if make is perm_tier_cells
+                      else make(L, torch, dev, stream)):
+                if c.name in args.cells:
+                    made[c.name] = c
+        torch.cuda.synchronize()
+        for name in args.cells:
+            made[name].go()
+            torch.cuda.synchronize()
+            torch.full((1,), 7.0, device=dev)  # marker kernel between cells
+            torch.cuda.synchronize()
+            print(f"bench: counter pass ran {name} (batch {made[name].meta['batch']})", file=sys.stderr, flush=True)
+        print(json.dumps({"counter_pass": [[n, made[n].meta["batch"]] for n in args.cells]}))
+        return 0
     if args.cells:
         out = {}
         for gi, make in enumerate((perm_tier_cells, haf_cells, tor_cells)):
             print(f"bench: cells {make.__name__}", file=sys.stderr, flush=True)
-            cells = [c for c in make(L, torch, dev, stream) if c.name in args.cells]
+            made = make(L, torch, dev, stream, headline=True) if make is perm_tier_cells else make(L, torch, dev, stream)
+            cells = [c for c in made if c.name in args.cells]
             out.update(run_cells(cells, torch, stream, args.reps, seed=100 + gi))
         print(json.dumps({"cells": out}))
         return 0
@@ -208,6 +249,11 @@ def main() -> int:
                                 "algorithmic input 9437184 B",
                 "kernel": f"perm_kernel<{dom_n},fp64>", "flops_per_launch": dom_flops,
                 "avg_launch_ms": kern_ms[dom_n], "pipe": "FP64 VALU (no MFMA)"}
+    ex = executed(f"perm_fp64_n{dom_n}", m24.shape[0], dom_s)
+    if ex:
+        roofline["executed_tflops"] = ex["tflops_executed"]
+        roofline["executed_frac"] = ex["tflops_executed"] * 1e12 / PEAK_FP64
+        roofline["fp64_issue_frac"] = ex["fp64_issue_frac"]
 
     per_kernel = {}
     for n, m in cases:
@@ -216,6 +262,7 @@ def main() -> int:
                                          "median_ms": kern_med[n][0], "iqr_ms": kern_med[n][1],
                                          "reps": args.steps, "evals_per_s": m.shape[0] / (kern_ms[n] * 1e-3),
                                          "tflops": f / (kern_ms[n] * 1e-3) / 1e12}
+        per_kernel[f"perm_fp64_n{n}"].update(executed(f"perm_fp64_n{n}", m.shape[0], kern_ms[n] * 1e-3))
         if dd_kern_ms:
             per_kernel[f"perm_dd_n{n}"] = {"batch": m.shape[0], "avg_ms": dd_kern_ms[n],
                                            "evals_per_s": m.shape[0] / (dd_kern_ms[n] * 1e-3)}
@@ -240,8 +287,6 @@ def main() -> int:
     for gi, (gname, make) in enumerate(cell_groups):
         def timed_group(make=make, gi=gi):
             cells = make(L, torch, dev, stream)
-            if args.cells:
-                cells = [c for c in cells if c.name in args.cells]
             return run_cells(cells, torch, stream, args.reps, seed=100 + gi)
         per_kernel.update(side(gname, timed_group))
     repeated = side("repeated-row sieve", repeated_bench) if args.repeated else None
@@ -282,7 +327,8 @@ def main() -> int:
                                "Haar 64x64 unitary (default_rng(1))",
                    "global_batch": evals_per_step, "tier": "fp64"},
         "roofline": {k: roofline[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic", "kernel",
-                                              "avg_launch_ms", "pipe")},
+                                              "avg_launch_ms", "pipe", "executed_tflops", "executed_frac")
+                     if k in roofline},
         "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "value_1core", "sample_short")}
                          if cpu else None),
         "dd": ({"evals_per_s": round(detail["dd"]["evals_per_s"], 1)} if dd_wall else None),
@@ -566,6 +612,7 @@ def run_cells(cells, torch, stream, reps, seed):
                     "reps": reps, "evals_per_s": c.meta["batch"] / (med * 1e-3)})
         if c.post is not None:
             rec.update(c.post(med * 1e-3))
+        rec.update(executed(c.name, c.meta["batch"], med * 1e-3))
         out[c.name] = rec
     return {k: out[k] for k in sorted(out)}
 
@@ -599,15 +646,17 @@ def device_cell(L, torch, dev, stream, name, func, tier, m, meta_extra=None, flo
     return Cell(name, go, meta, post)
 
 
-def perm_tier_cells(L, torch, dev, stream):
+def perm_tier_cells(L, torch, dev, stream, headline=False):
     """The permanent's other tiers (fp64 and dd are the headline's): kappa,
     certified and certified-dd kernels at n = 16 (B = 4096), 20 (B = 512) and
     24 (B = 64), config-2 style rows of a Haar unitary."""
     from gbskernels_amd.workloads import haar_unitary, perm_rows
     from gbskernels_amd.workloads import perm_haar_rows_workload
-    # the headline's n = 24 launch as a cell of its own (profiler runs)
-    _, m24 = perm_haar_rows_workload(take16=0, take24=1024)
-    cells = [device_cell(L, torch, dev, stream, "perm_fp64_n24", 0, 0, m24, flops=perm_flops(24))]
+    cells = []
+    if headline:  # the headline's two launches as cells of their own (profiler runs)
+        m16, m24 = perm_haar_rows_workload()
+        cells += [device_cell(L, torch, dev, stream, "perm_fp64_n24", 0, 0, m24, flops=perm_flops(24)),
+                  device_cell(L, torch, dev, stream, "perm_fp64_n16", 0, 0, m16, flops=perm_flops(16))]
     for n, B in ((16, 4096), (20, 512), (24, 64)):
         rng = np.random.default_rng(40 + n)
         U = haar_unitary(64, rng)

@@ -126,6 +126,15 @@ __device__ inline double c_abs(cplx z) {
 
 // ---------------------------------------------------------- wave utilities
 __device__ inline double shfl_xor_d(double v, int mask) { return __shfl_xor(v, mask, 64); }
+// the partner lane's value (lane ^ 1) through a DPP quad permutation
+// [1, 0, 3, 2]: a VALU move, no LDS traffic (a shuffle is an LDS permute);
+// every lane of the wave must execute it
+__device__ inline double swap_pair_d(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, 0xB1, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0xB1, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, (long long)(((unsigned long long)hi << 32) | lo));
+}
 
 // Sum over groups of G consecutive lanes (G a power of two), butterfly order:
 // every lane of the group ends holding the same, deterministically ordered sum.

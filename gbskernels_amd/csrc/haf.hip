@@ -6,7 +6,7 @@ namespace gbsk {
 
 // built-in schedule of the fused small-size kernel (measured on MI355X)
 int haf_small_max_n = 12;
-int haf_small_target_wgs = 2048;
+int haf_small_target_wgs = 1024;
 
 int haf_schedule_get(int* small_max_n, int* small_target_wgs) {
   if (small_max_n) *small_max_n = haf_small_max_n;

@@ -23,6 +23,7 @@
 //    in a fixed order by a final pass: the value depends on N alone.
 #pragma once
 
+#include <type_traits>
 #include <utility>
 
 #include "cert.hpp"
@@ -51,6 +52,7 @@ struct ArF64 {
   __device__ static T div_int(T a, int j) { return cplx{a.re / j, a.im / j}; }
   __device__ static cplx to_c(T a) { return a; }
   __device__ static T shfl_xor(T a, int m) { return cplx{shfl_xor_d(a.re, m), shfl_xor_d(a.im, m)}; }
+  __device__ static T swap_pair(T a) { return cplx{swap_pair_d(a.re), swap_pair_d(a.im)}; }
 };
 
 __device__ inline dd dd_mul_d_acc(dd a, double b, dd acc) { return dd_add(acc, dd_mul_d(a, b)); }
@@ -75,6 +77,9 @@ struct ArDD {
   __device__ static cplx to_c(T a) { return ddc_to_c(a); }
   __device__ static T shfl_xor(T a, int m) {
     return ddc{dd{shfl_xor_d(a.re.hi, m), shfl_xor_d(a.re.lo, m)}, dd{shfl_xor_d(a.im.hi, m), shfl_xor_d(a.im.lo, m)}};
+  }
+  __device__ static T swap_pair(T a) {
+    return ddc{dd{swap_pair_d(a.re.hi), swap_pair_d(a.re.lo)}, dd{swap_pair_d(a.im.hi), swap_pair_d(a.im.lo)}};
   }
 };
 
@@ -320,12 +325,18 @@ __device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, in
   // b = ceil((k-1)/2): v_{2j}, v_{2j+1} at power j. Stored apart (kvs) and
   // added to the trace part p_k / 2 before the Newton recurrence.
   auto dot_d = [&](const T (&row)[S], const double (&erow)[ANYC ? S : 1], T& u, double& eu) {
+    // d is re-read from LDS on every call: the opaque zero offset stops the
+    // compiler from hoisting the S loop-invariant diagonal entries (4 S
+    // VGPRs) out of the power loop, which halved the loop kernels' occupancy
+    int zo = 0;
+    asm volatile("" : "+v"(zo));
+    const cplx* __restrict__ Ad = As + zo;
     u = Ar::zero();
     eu = CERT ? (double)S * CERT_UFL : 0.0;
 #pragma unroll
     for (int c = 0; c < S; ++c) {
       const int Rc = 2 * p[c >> 1] + (c & 1);
-      const cplx dc = As[Rc * N + Rc];
+      const cplx dc = Ad[Rc * N + Rc];
       if constexpr (DDC) {
         double o;
         u = cddc_fma_c(row[c], dc, u, &o);
@@ -338,8 +349,8 @@ __device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, in
   };
   auto add_v = [&](int k, const T& ua, double eua, const T& ub, double eub) {
     if (k > n) return;  // uniform over the wave (reductions hold barriers)
-    const T x = Ar::shfl_xor(ua, 1);
-    const double ex = ANYC ? shfl_xor_d(eua, 1) : 0.0;
+    const T x = Ar::swap_pair(ua);
+    const double ex = ANYC ? swap_pair_d(eua) : 0.0;
     const double hk = 0.5 * (double)k;
     if constexpr (DDC) {
       double o;
@@ -385,7 +396,7 @@ __device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, in
     double e2 = 0.0;
 #pragma unroll
     for (int c = 0; c < S; ++c) {
-      const T y = Ar::shfl_xor(P[c ^ 1], 1);
+      const T y = Ar::swap_pair(P[c ^ 1]);
       if constexpr (DDC) {
         double eo;
         t2 = cddc_fma(P[c], y, t2, &eo);  // exact operands
@@ -435,27 +446,58 @@ __device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, in
 #pragma unroll
       for (int c = 0; c < S; ++c) eA[c] = CERT ? (double)S * CERT_UFL : 0.0;
     }
+    // the column offsets pass through an opaque move on every power: without
+    // it the compiler keeps all S x M entry addresses live across the power
+    // loop (up to 100 more VGPRs, one wave per SIMD where two fit)
+    int colv[M];
 #pragma unroll
-    for (int t = 0; t < S; ++t) {
-      const cplx* At = As + (2 * p[t >> 1] + (t & 1)) * N;
-      // a-priori bound of the fma-chain dot product: sum_t (eP_t + gamma_2S |P_t|) |C_tc|
-      const double wt = CERT ? fma(gam, cert_norm1(Ar::to_c(P[t])), eP[t]) : 0.0;
+    for (int u = 0; u < M; ++u) {
+      colv[u] = 2 * p[u];
+      asm volatile("" : "+v"(colv[u]));
+    }
+    // acc[c] = sum_t P[t] C[t][c] as an fma chain in t = 0, 1, .., S - 1 for
+    // every c. B_S is symmetric, so C[t][c] = A[R_t][R_(c^1)] = C[c^1][t^1]:
+    // each entry is read from LDS once and feeds both of its positions (half
+    // the LDS reads; LDS bandwidth, not the FP64 pipe, bounds this loop). At
+    // step t the entries (t, c) with c^1 >= t are read in ascending c^1; entry
+    // (t, c) is term t of acc[c] and, as C[c^1][t^1], term c^1 of acc[t^1].
+    // So acc[y] receives terms 0 .. y^1 - 1 at those steps, then term y^1 and
+    // terms y^1 + 1 .. S - 1 in order at step y^1: the chain order of reading
+    // row by row (for symmetric input the values are the same bits).
+    auto wrow = [&](int t) -> double {  // a-priori bound weight of row t (certified)
+      return CERT ? fma(gam, cert_norm1(Ar::to_c(P[t])), eP[t]) : 0.0;
+    };
+    // one term of the fma-chain dot product acc[c] += P[t] C[t][c] (and, in the
+    // certified modes, its bound: sum_t (eP_t + gamma_2S |P_t|) |C_tc|)
+    auto term = [&](int c, int t, const cplx& x) {
+      if constexpr (DDC) {
+        double o;
+        acc[c] = cddc_fma_c(P[t], x, acc[c], &o);
+        eA[c] = fma(eP[t], cert_norm1(x), eA[c] + o);
+      } else {
+        acc[c] = Ar::fma_c(P[t], x, acc[c]);
+        if (CERT) eA[c] = fma(wrow(t), cert_norm1(x), eA[c]);
+      }
+    };
+    if constexpr (DDC) {
+      // certified dd keeps the row-by-row reads (the same chains, so the same
+      // bits as the dd tier on symmetric input): the interleaved form made its
+      // compilation take over half an hour
 #pragma unroll
-      for (int u = 0; u < M; ++u) {
-        const cplx a1 = At[2 * p[u] + 1], a0 = At[2 * p[u]];
-        if constexpr (DDC) {
-          double o1, o0;
-          acc[2 * u] = cddc_fma_c(P[t], a1, acc[2 * u], &o1);
-          acc[2 * u + 1] = cddc_fma_c(P[t], a0, acc[2 * u + 1], &o0);
-          eA[2 * u] = fma(eP[t], cert_norm1(a1), eA[2 * u] + o1);
-          eA[2 * u + 1] = fma(eP[t], cert_norm1(a0), eA[2 * u + 1] + o0);
-          continue;
-        }
-        acc[2 * u] = Ar::fma_c(P[t], a1, acc[2 * u]);
-        acc[2 * u + 1] = Ar::fma_c(P[t], a0, acc[2 * u + 1]);
-        if (CERT) {
-          eA[2 * u] = fma(wt, cert_norm1(a1), eA[2 * u]);
-          eA[2 * u + 1] = fma(wt, cert_norm1(a0), eA[2 * u + 1]);
+      for (int t = 0; t < S; ++t) {
+        const cplx* At = As + (2 * p[t >> 1] + (t & 1)) * N;
+#pragma unroll
+        for (int cc = 0; cc < S; ++cc) term(cc ^ 1, t, At[colv[cc >> 1] + (cc & 1)]);
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < S; ++t) {
+        const cplx* At = As + (2 * p[t >> 1] + (t & 1)) * N;
+#pragma unroll
+        for (int cc = t; cc < S; ++cc) {
+          const cplx x = At[colv[cc >> 1] + (cc & 1)];  // C[t][cc ^ 1] = A[R_t][R_cc]
+          term(cc ^ 1, t, x);
+          if (cc != t) term(t ^ 1, cc, x);
         }
       }
     }
@@ -472,10 +514,10 @@ __device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, in
     double eo = 0.0, ee = 0.0;
 #pragma unroll
     for (int c = 0; c < S; ++c) {
-      const T yo = Ar::shfl_xor(P[c ^ 1], 1);
-      const T ye = Ar::shfl_xor(acc[c ^ 1], 1);
+      const T yo = Ar::swap_pair(P[c ^ 1]);
+      const T ye = Ar::swap_pair(acc[c ^ 1]);
       if constexpr (DDC) {
-        const double eyo = shfl_xor_d(eP[c ^ 1], 1), eye = shfl_xor_d(eA[c ^ 1], 1);
+        const double eyo = swap_pair_d(eP[c ^ 1]), eye = swap_pair_d(eA[c ^ 1]);
         double oo, oe;
         const T ton = cddc_fma(acc[c], yo, todd, &oo);
         const T ten = cddc_fma(acc[c], ye, teven, &oe);
@@ -488,7 +530,7 @@ __device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, in
       const T ton = Ar::fma(acc[c], yo, todd);
       const T ten = Ar::fma(acc[c], ye, teven);
       if constexpr (CERT) {
-        const double eyo = shfl_xor_d(eP[c ^ 1], 1), eye = shfl_xor_d(eA[c ^ 1], 1);
+        const double eyo = swap_pair_d(eP[c ^ 1]), eye = swap_pair_d(eA[c ^ 1]);
         eo = cert_fma(acc[c], eA[c], yo, eyo, todd, eo, ton);
         ee = cert_fma(acc[c], eA[c], ye, eye, teven, ee, ten);
       }
@@ -596,8 +638,37 @@ __device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, in
 // m = S/2 and writes the phase partial (and sum |term| in kappa mode, the
 // bound in certified mode).
 //   part[b * np + off_S + chunk]      chunk partial (np = haf_nparts(n))
+// Waves per SIMD the register allocator must leave room for, per fp64 phase
+// kernel: bit S / 2 of a mask asks for 2 (a wave capped at 256 registers, the
+// excess spilled) where the allocator alone would take more and run one wave
+// per SIMD. Plain and kappa share a mask; certified has its own. Chosen per
+// subset size from per-phase kernel times on MI355X (N = 16 and 24): forced
+// where it was faster (S = 20, 22: 1.3x; loop S = 18: 1.55x; certified
+// S = 14, 16 and loop S = 10 .. 14: 1.1 - 1.55x), left to the allocator where
+// the spill cost more (S = 24, loop S = 16, certified loop S = 16).
+#ifndef GBSK_HAF_OCC2_MASK
+#define GBSK_HAF_OCC2_MASK ((1u << 10) | (1u << 11))
+#endif
+#ifndef GBSK_HAF_OCC2_LOOP_MASK
+#define GBSK_HAF_OCC2_LOOP_MASK ((1u << 9) | (1u << 10) | (1u << 11))
+#endif
+#ifndef GBSK_HAF_OCC2_CERT_MASK
+#define GBSK_HAF_OCC2_CERT_MASK ((1u << 7) | (1u << 8))
+#endif
+#ifndef GBSK_HAF_OCC2_CERT_LOOP_MASK
+#define GBSK_HAF_OCC2_CERT_LOOP_MASK ((1u << 5) | (1u << 6) | (1u << 7))
+#endif
 template <class Ar, bool LOOP, int S, int MODE>
-__global__ __launch_bounds__(64) void haf_phase_kernel(const HafPhaseJobs<typename Ar::T> jobs) {
+constexpr int haf_phase_min_waves() {
+  if (!std::is_same<Ar, ArF64>::value) return 1;
+  const unsigned mask = MODE == HM_CERT ? (LOOP ? (unsigned)GBSK_HAF_OCC2_CERT_LOOP_MASK : (unsigned)GBSK_HAF_OCC2_CERT_MASK)
+                                        : (LOOP ? (unsigned)GBSK_HAF_OCC2_LOOP_MASK : (unsigned)GBSK_HAF_OCC2_MASK);
+  return ((mask >> (S / 2)) & 1u) ? 2 : 1;
+}
+
+template <class Ar, bool LOOP, int S, int MODE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(haf_phase_min_waves<Ar, LOOP, S, MODE>())))
+void haf_phase_kernel(const HafPhaseJobs<typename Ar::T> jobs) {
   using T = typename Ar::T;
   constexpr bool KAPPA = (MODE == HM_KAPPA);
   constexpr bool CERT = (MODE == HM_CERT);
