@@ -49,7 +49,19 @@ TRAFFIC_N24 = 13742848
 # the input values): (flop, FP64 VALU wave instructions). flop counts 64
 # lanes per wave instruction, 2 per FMA. Canonical flops count the reference
 # algorithm as written; executed flops count what the kernels issue.
-EXECUTED_F64 = {}
+EXECUTED_F64 = {
+    "perm_fp64_n24": (2.014479e+09, 1.888513e+07),
+    "perm_fp64_n16": (5169792, 48586),
+    "haf_fp64_n16": (7939584, 65744),
+    "haf_fp64_n24": (5.18053e+08, 4206890),
+    "lhaf_fp64_n16": (9162176, 78319),
+    "lhaf_fp64_n24": (5.749229e+08, 4794570),
+    "haf_fp64_config1_n8": (57952, 511.5),
+    "tor_fp64_nm8": (50944, 506),
+    "tor_fp64_nm12": (554752, 5279),
+    "tor_fp64_nm16": (9244160, 88138),
+    "tor_fp64_nm20": (1.55469e+08, 1487511),
+}
 # FP64 VALU wave instructions per second at the 78.6 TFLOP/s peak: one per CU
 # per clock (256 CUs, 2.4 GHz; an FMA over 64 lanes is 128 flop)
 FP64_ISSUE_PER_S = 256 * 2.4e9

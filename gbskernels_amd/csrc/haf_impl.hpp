@@ -859,6 +859,10 @@ __global__ __launch_bounds__(64 * HAF_REDUCE_WAVES) void haf_reduce_kernel(const
 constexpr int HAF_SMALL_MAX_N = 12;
 constexpr int HAF_SMALL_MAX_PAIRS = HAF_SMALL_MAX_N / 2;
 constexpr int HAF_SMALL_MAX_T = 4;  // instances per workgroup
+// terms of one instance (every nonempty subset of its pairs, 2^n - 1 <= 63)
+// and chunk partials (haf_nparts(n) <= 8 for n <= 6)
+__host__ __device__ constexpr int haf_small_terms(int pairs) { return (1 << pairs) - 1; }
+constexpr int HAF_SMALL_MAX_PARTS = 8;
 static_assert(HAF_CHUNK_ITERS == 1, "the fused small-size kernel replays one-round chunks");
 
 struct HafSmallJobs {
@@ -875,11 +879,12 @@ struct HafSmallJobs {
 // one phase (subset size S) of the ninst staged instances: terms[i][off + q]
 // receives instance i's subset-q term as the phase kernel's group accumulator
 // holds it (zero + term), tabs the kappa magnitude (0 + |term|)
-template <class Ar, bool LOOP, int MODE, int S>
+template <class Ar, bool LOOP, int MODE, int PAIRS, int S>
 __device__ __forceinline__ void haf_small_phase(const cplx* As, int N, int ninst, void* scratch,
-                                                typename Ar::T (*terms)[64], double (*tabs)[64]) {
+                                                typename Ar::T (*terms)[haf_small_terms(PAIRS)],
+                                                double (*tabs)[haf_small_terms(PAIRS)]) {
   using T = typename Ar::T;
-  using Sc = HafScratch<Ar, LOOP, S, MODE, HAF_SMALL_MAX_PAIRS + 1>;
+  using Sc = HafScratch<Ar, LOOP, S, MODE, PAIRS + 1>;
   constexpr int G = 64 / S;
   Sc& sh = *reinterpret_cast<Sc*>(scratch);
   const int lane = threadIdx.x;
@@ -906,26 +911,32 @@ __device__ __forceinline__ void haf_small_phase(const cplx* As, int N, int ninst
   __syncthreads();
 }
 
-template <class Ar, bool LOOP, int MODE, int... Ss>
+template <class Ar, bool LOOP, int MODE, int PAIRS, int... Ss>
 __device__ __forceinline__ void haf_small_phases(const cplx* As, int N, int ninst, void* scratch,
-                                                 typename Ar::T (*terms)[64], double (*tabs)[64],
+                                                 typename Ar::T (*terms)[haf_small_terms(PAIRS)],
+                                                 double (*tabs)[haf_small_terms(PAIRS)],
                                                  std::integer_sequence<int, Ss...>) {
-  ((2 * (Ss + 1) <= N ? (haf_small_phase<Ar, LOOP, MODE, 2 * (Ss + 1)>(As, N, ninst, scratch, terms, tabs), 0) : 0),
+  ((2 * (Ss + 1) <= N ? (haf_small_phase<Ar, LOOP, MODE, PAIRS, 2 * (Ss + 1)>(As, N, ninst, scratch, terms, tabs), 0)
+                      : 0),
    ...);
 }
 
-template <class Ar, bool LOOP, int MODE>
+// NMAX: the size class (4, 8 or 12) the LDS arrays and phases are built for;
+// a smaller class takes less LDS and fewer registers (more waves per CU)
+template <class Ar, bool LOOP, int MODE, int NMAX>
 __global__ __launch_bounds__(64) void haf_small_kernel(const HafSmallJobs jobs) {
   using T = typename Ar::T;
   constexpr bool KAPPA = (MODE == HM_KAPPA);
-  constexpr int NN = HAF_SMALL_MAX_N * HAF_SMALL_MAX_N;
+  constexpr int PAIRS = NMAX / 2;
+  constexpr int NN = NMAX * NMAX;
+  constexpr int TERMS = haf_small_terms(PAIRS);
   // the largest phase scratch is the S = 2 one (32 lane groups)
-  constexpr size_t SCR = sizeof(HafScratch<Ar, LOOP, 2, MODE, HAF_SMALL_MAX_PAIRS + 1>);
+  constexpr size_t SCR = sizeof(HafScratch<Ar, LOOP, 2, MODE, PAIRS + 1>);
   __shared__ cplx As[HAF_SMALL_MAX_T * NN];
-  __shared__ T terms[HAF_SMALL_MAX_T][64];
-  __shared__ double tabs[KAPPA ? HAF_SMALL_MAX_T : 1][64];
-  __shared__ T parts[HAF_SMALL_MAX_T][64];
-  __shared__ double pabs[KAPPA ? HAF_SMALL_MAX_T : 1][64];
+  __shared__ T terms[HAF_SMALL_MAX_T][TERMS];
+  __shared__ double tabs[KAPPA ? HAF_SMALL_MAX_T : 1][TERMS];
+  __shared__ T parts[HAF_SMALL_MAX_T][HAF_SMALL_MAX_PARTS];
+  __shared__ double pabs[KAPPA ? HAF_SMALL_MAX_T : 1][HAF_SMALL_MAX_PARTS];
   __shared__ __align__(16) unsigned char scratch[SCR];
   const int lane = threadIdx.x;
   int jb = 0;
@@ -942,8 +953,8 @@ __global__ __launch_bounds__(64) void haf_small_kernel(const HafSmallJobs jobs) 
     As[e] = v;
   }
   __syncthreads();
-  haf_small_phases<Ar, LOOP, MODE>(As, N, ninst, scratch, terms, KAPPA ? tabs : nullptr,
-                                   std::make_integer_sequence<int, HAF_SMALL_MAX_PAIRS>{});
+  haf_small_phases<Ar, LOOP, MODE, PAIRS>(As, N, ninst, scratch, terms, KAPPA ? tabs : nullptr,
+                                          std::make_integer_sequence<int, PAIRS>{});
   // chunk partials, in the phase kernels' slot order and association
   const int np = haf_nparts(n);
   for (int idx = lane; idx < ninst * np; idx += 64) {
@@ -998,22 +1009,32 @@ inline int haf_small_T(int B) {
 
 template <class Ar, bool LOOP, int MODE>
 static void launch_small(const HafJob* jobs, int njobs, hipStream_t st) {
-  HafSmallJobs sj{};
-  int wg = 0;
-  for (int j = 0; j < njobs; ++j) {
-    const int k = sj.n++;
-    sj.mats[k] = jobs[j].in;
-    sj.out[k] = jobs[j].out;
-    sj.aux[k] = jobs[j].aux;
-    sj.N[k] = jobs[j].N;
-    sj.B[k] = jobs[j].B;
-    sj.T[k] = haf_small_T(jobs[j].B);
-    sj.wg_begin[k] = wg;
-    wg += (jobs[j].B + sj.T[k] - 1) / sj.T[k];
+  // one launch per size class present (N <= 4, <= 8, <= 12)
+  for (int cls : {4, 8, 12}) {
+    HafSmallJobs sj{};
+    int wg = 0;
+    for (int j = 0; j < njobs; ++j) {
+      const int N = jobs[j].N;
+      if (N > cls || (cls > 4 && N <= cls - 4)) continue;
+      const int k = sj.n++;
+      sj.mats[k] = jobs[j].in;
+      sj.out[k] = jobs[j].out;
+      sj.aux[k] = jobs[j].aux;
+      sj.N[k] = N;
+      sj.B[k] = jobs[j].B;
+      sj.T[k] = haf_small_T(jobs[j].B);
+      sj.wg_begin[k] = wg;
+      wg += (jobs[j].B + sj.T[k] - 1) / sj.T[k];
+    }
+    if (sj.n == 0 || wg == 0) continue;
+    sj.wg_begin[sj.n] = wg;
+    if (cls == 4)
+      hipLaunchKernelGGL((haf_small_kernel<Ar, LOOP, MODE, 4>), dim3((unsigned)wg), dim3(64), 0, st, sj);
+    else if (cls == 8)
+      hipLaunchKernelGGL((haf_small_kernel<Ar, LOOP, MODE, 8>), dim3((unsigned)wg), dim3(64), 0, st, sj);
+    else
+      hipLaunchKernelGGL((haf_small_kernel<Ar, LOOP, MODE, 12>), dim3((unsigned)wg), dim3(64), 0, st, sj);
   }
-  if (sj.n == 0 || wg == 0) return;
-  sj.wg_begin[sj.n] = wg;
-  hipLaunchKernelGGL((haf_small_kernel<Ar, LOOP, MODE>), dim3((unsigned)wg), dim3(64), 0, st, sj);
 }
 
 template <class Ar, bool LOOP, int MODE, int S>
