@@ -116,6 +116,9 @@ def main() -> int:
                     help="also time the single-large torontonian at 16..32 modes (0 skips)")
     ap.add_argument("--sampler", type=int, default=1,
                     help="also time the resident GBS sampler on the reference's sampler configurations (0 skips)")
+    ap.add_argument("--reference-table", type=int, default=1,
+                    help="also time the reference's kernel-only benchmark shapes (B = 4096) against its "
+                         "published medians (0 skips)")
     ap.add_argument("--accuracy", type=int, default=1,
                     help="also measure relative error against the mpmath ref tier, physical and "
                          "cancelling inputs, every tier (0 skips)")
@@ -304,6 +307,7 @@ def main() -> int:
     repeated = side("repeated-row sieve", repeated_bench) if args.repeated else None
     tor_single = side("single-large torontonian", tor_single_bench) if args.tor_single else None
     sampler = side("resident sampler", sampler_bench) if args.sampler else None
+    reference = side("reference kernel table", reference_table) if args.reference_table else None
     served = side("served mix", lambda: served_bench(args)) if args.served_requests > 0 else None
     streams = (side("config streams", lambda: config_streams(args))
                if args.config3_requests > 0 or args.config5_requests > 0 else None)
@@ -322,11 +326,12 @@ def main() -> int:
         "dd": ({"evals_per_s": evals_per_step * args.dd_steps / dd_wall,
                 "ms_per_step": dd_wall / args.dd_steps * 1e3, "steps": args.dd_steps} if dd_wall else None),
         "accuracy": accuracy, "served": served, "streams": streams, "repeated_ab": repeated,
-        "tor_single": tor_single, "sampler": sampler,
+        "tor_single": tor_single, "sampler": sampler, "reference_table": reference,
     }
     detail_path = write_detail(detail, args.detail)
     # the detailed legs as earlier lines (one per leg), the headline last
-    for k in ("per_kernel", "accuracy", "served", "streams", "repeated_ab", "tor_single", "sampler"):
+    for k in ("per_kernel", "accuracy", "served", "streams", "repeated_ab", "tor_single", "sampler",
+              "reference_table"):
         if detail[k] is not None:
             print(json.dumps({"bench_detail": k, k: detail[k]}))
 
@@ -346,6 +351,9 @@ def main() -> int:
         "dd": ({"evals_per_s": round(detail["dd"]["evals_per_s"], 1)} if dd_wall else None),
         "parity": ({k: float(f"{v:.3g}") for k, v in parity.items() if k != "samples"} if parity else None),
         "served": served_summary(served, streams),
+        "vs_reference_kernels": ({k: (round(reference[k], 2) if isinstance(reference[k], float) else reference[k])
+                                  for k in ("cells_compared", "geomean_speedup", "min_speedup")}
+                                 if isinstance(reference, dict) and "rows" in reference else None),
         "detail": detail_path,
     }
     text = json.dumps(line)
@@ -472,6 +480,102 @@ def tor_single_bench():
                 row["reference_evals_per_s"] = ref[2 * n]
             rows.append(row)
     return {"workload": "core/bench_kernels.cu:368-410 inputs (numpy generator), public path", "rows": rows}
+
+
+# The reference's kernel-only table (core/bench_kernels.cu, B = 4096): each of
+# our tiers against the reference function family it replaces (the best of its
+# per-thread / cooperative / warp / small-cap variants at that size), on the
+# same input recipes (complex U(-1, 1); symmetric for haf / lhaf / tor, the
+# torontonian x 0.05; certified x 0.3, symmetrized by averaging), drawn with
+# numpy here. Matrix sides are the reference's `matrix_dim`.
+REFERENCE_ROWS = (
+    ("perm fp64", 0, 0, "plain", (8, 12, 16, 20), ("perm", "perm_coop", "perm_warp")),
+    ("haf fp64", 1, 0, "sym", (8, 12, 16, 20), ("haf", "haf_small", "haf_coop")),
+    ("lhaf fp64", 2, 0, "sym", (8, 12, 16), ("lhaf", "lhaf_coop")),
+    ("tor fp64, complex O", 3, 0, "tor", (8, 12, 16, 20), ("tor", "tor_coop")),
+    ("tor fp64, real O", 3, 0, "tor_real", (8, 12, 16), ("tor_real_chol", "tor_recursive")),
+    ("perm dd", 0, 1, "plain", (8, 12, 16), ("perm_dd",)),
+    ("haf dd", 1, 1, "sym", (8, 12, 16), ("haf_dd",)),
+    ("lhaf dd", 2, 1, "sym", (8, 12), ("lhaf_dd",)),
+    ("tor dd", 3, 1, "tor", (8, 12, 16), ("tor_dd",)),
+    ("perm certified", 0, 3, "cert", (8, 12, 16), ("perm_certified",)),
+    ("haf certified", 1, 3, "cert_sym", (8, 12, 16), ("haf_certified",)),
+    ("lhaf certified", 2, 3, "cert_sym", (8, 12), ("lhaf_certified",)),
+    ("haf certified-dd", 1, 4, "cert_sym", (8, 12), ("haf_dd_certified",)),
+)
+
+
+def reference_inputs(kind: str, D: int, B: int) -> np.ndarray:
+    """B x D x D inputs of one recipe (numpy default_rng(1234 + D))"""
+    rng = np.random.default_rng(1234 + D)
+    if kind == "tor_real":
+        m = 0.05 * rng.uniform(-1.0, 1.0, (B, D, D))
+        iu = np.triu_indices(D, 1)
+        m[:, iu[1], iu[0]] = m[:, iu[0], iu[1]]
+        return m.astype(np.complex128)
+    scale = {"plain": 1.0, "sym": 1.0, "tor": 0.05, "cert": 0.3, "cert_sym": 0.3}[kind]
+    m = scale * (rng.uniform(-1.0, 1.0, (B, D, D)) + 1j * rng.uniform(-1.0, 1.0, (B, D, D)))
+    if kind in ("sym", "tor"):  # the upper triangle copied down
+        iu = np.triu_indices(D, 1)
+        m[:, iu[1], iu[0]] = m[:, iu[0], iu[1]]
+    elif kind == "cert_sym":
+        m = 0.5 * (m + m.transpose(0, 2, 1))
+    return np.ascontiguousarray(m)
+
+
+def reference_table():
+    """Our kernels on the reference's kernel-only benchmark (B = 4096): the
+    reference's protocol (2 untimed warm-ups, wall clock around the launch and
+    a device synchronize, median of 7), next to its published medians on an
+    RTX 4090 and an A100 (tests/golden/kernel_throughput_reference.json)."""
+    import torch
+    from gbskernels_amd import _lib
+    L = _lib.lib()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    pub = json.loads((Path(__file__).resolve().parent / "tests" / "golden" /
+                      "kernel_throughput_reference.json").read_text())["rows"]
+    B, rows = 4096, []
+    for label, func, tier, kind, dims, ref_funcs in REFERENCE_ROWS:
+        for D in dims:
+            m = reference_inputs(kind, D, B)
+            d_in = torch.from_numpy(m.view(np.float64)).to(dev)
+            d_out = torch.empty(2 * B, dtype=torch.float64, device=dev)
+            d_aux = torch.empty(B, dtype=torch.float64, device=dev)
+            wsb = int(L.gbs_workspace_bytes(func, tier, D, B))
+            ws = torch.empty(max(wsb, 256), dtype=torch.uint8, device=dev)
+
+            def go():
+                rc = L.gbs_eval_device(func, tier, d_in.data_ptr(), D, B, d_out.data_ptr(), d_aux.data_ptr(),
+                                       ws.data_ptr(), ws.numel(), stream.cuda_stream)
+                if rc:
+                    raise RuntimeError(f"gbs_eval_device rc={rc} ({label}, {D})")
+            for _ in range(2):
+                go()
+                torch.cuda.synchronize()
+            ts = []
+            for _ in range(7):
+                t0 = time.perf_counter()
+                go()
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t0)
+            ours = B / float(np.median(ts))
+            best = {}
+            for r in pub:
+                if r["func"] in ref_funcs and r["matrix_dim"] == D:
+                    best[r["gpu"]] = max(best.get(r["gpu"], 0.0), r["evals_per_sec_median"])
+            row = {"row": label, "matrix_dim": D, "batch": B, "evals_per_s": ours, "reference": best}
+            if best:
+                row["speedup_vs_best_published"] = ours / max(best.values())
+            if tier in (3, 4):
+                row["certified_finite"] = float(np.mean(np.isfinite(d_aux.cpu().numpy())))
+            rows.append(row)
+            del d_in, d_out, d_aux, ws
+    sp = [r["speedup_vs_best_published"] for r in rows if "speedup_vs_best_published" in r]
+    return {"protocol": "B = 4096, 2 warm-ups, wall clock launch + synchronize, median of 7 "
+                        "(core/bench_kernels.cu:58-119)", "rows": rows,
+            "cells_compared": len(sp), "geomean_speedup": float(np.exp(np.mean(np.log(sp)))) if sp else None,
+            "min_speedup": float(min(sp)) if sp else None}
 
 
 def sampler_bench():

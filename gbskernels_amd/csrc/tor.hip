@@ -973,11 +973,13 @@ __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int
   }
 }
 
-// RERUN scans the partial slots with a fixed grid (at most TOR_RERUN_GRID
-// workgroups): the 64 lanes test 64 slots at once and the workgroup then
+// RERUN: min(slots, TOR_RERUN_GRID) workgroups, each owning a contiguous range
+// of partial slots; the 64 lanes test 64 slots at once and the workgroup then
 // recomputes the NaN ones in order. When no instance fell off the real path
 // the pass is a quick scan (at 20 modes, one workgroup per slot -- 8.4M --
-// took 5 ms to find nothing).
+// took 5 ms to find nothing); when every slot did (a complex O batch), up to
+// 8192 workgroups recompute in parallel (one slot each at 4096 instances
+// without leading modes: 64 in a row per workgroup took 1.3 ms at 4 modes).
 constexpr int64_t TOR_RERUN_GRID = 8192;
 
 template <class Ar, int R, int T2, bool KAPPA, bool RERUN, bool BFS>
@@ -986,10 +988,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(tor_min_wave
     double* __restrict__ part_abs) {
   if constexpr (RERUN) {
     const int64_t total = (int64_t)B << T1;
-    for (int64_t base = (int64_t)blockIdx.x * 64; base < total; base += (int64_t)gridDim.x * 64) {
+    const int64_t per = (total + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * per, hi = lo + per < total ? lo + per : total;
+    for (int64_t base = lo; base < hi; base += 64) {
       const int64_t mine = base + threadIdx.x;
       bool nan = false;
-      if (mine < total) {
+      if (mine < hi) {
         const typename Ar::P prev = part[mine];
         nan = prev.re != prev.re;
       }
@@ -1265,10 +1269,20 @@ __device__ inline dd cdd_wave_sum(dd s, double& e) {
   return s;
 }
 
+// With cpart, an fp64 reduce also marks the complex fallback's partials of
+// its instance: all npc of them NaN (to be recomputed) when the real sum is
+// NaN, else 0 (skipped); every lane holds the sum after the butterfly.
+__device__ inline void tor_mark_fallback(const cplx& s, int64_t b, int lane, cplx* __restrict__ cpart, int npc) {
+  if (cpart == nullptr) return;
+  const cplx m = (s.re != s.re) ? cplx{NAN, NAN} : cplx{0.0, 0.0};
+  for (int i = lane; i < npc; i += 64) cpart[b * npc + i] = m;
+}
+
 template <bool DD, bool KAPPA>
 __global__ __launch_bounds__(256) void tor_reduce_kernel(const void* __restrict__ partv,
                                                          const double* __restrict__ part_abs, int np, int B,
-                                                         cplx* __restrict__ out, double* __restrict__ aux) {
+                                                         cplx* __restrict__ out, double* __restrict__ aux,
+                                                         cplx* __restrict__ cpart, int npc) {
   const int64_t b = tor_red_instance();
   const int lane = threadIdx.x & 63;
   if (b >= B) return;
@@ -1289,6 +1303,7 @@ __global__ __launch_bounds__(256) void tor_reduce_kernel(const void* __restrict_
     s = group_sum<64>(s);
     if (KAPPA) a = group_sum<64>(a);
     if (lane == 0) out[b] = s;
+    tor_mark_fallback(s, b, lane, cpart, npc);
   }
   if (KAPPA && lane == 0) aux[b] = a;
 }
@@ -1298,7 +1313,8 @@ __global__ __launch_bounds__(256) void tor_reduce_kernel(const void* __restrict_
 __global__ __launch_bounds__(256) void tor_reduce_cert_kernel(const cplx* __restrict__ part,
                                                               const double* __restrict__ part_e, int np, int B,
                                                               double depth, cplx* __restrict__ out,
-                                                              double* __restrict__ aux) {
+                                                              double* __restrict__ aux, cplx* __restrict__ cpart,
+                                                              int npc) {
   const int64_t b = tor_red_instance();
   const int lane = threadIdx.x & 63;
   if (b >= B) return;
@@ -1309,6 +1325,7 @@ __global__ __launch_bounds__(256) void tor_reduce_cert_kernel(const cplx* __rest
     e = cert_round(s, e + part_e[b * np + i]);
   }
   cert_group_sum<64>(s, e);
+  tor_mark_fallback(s, b, lane, cpart, npc);
   if (lane != 0) return;
   out[b] = s;
   const double f = cert_finish(e, depth);
@@ -1338,15 +1355,6 @@ __global__ __launch_bounds__(256) void tor_reduce_ddcert_kernel(const dd* __rest
   out[b] = cplx{v, 0.0};
   const double f = cert_finish(e, depth);
   aux[b] = (f == f) ? f : INFINITY;
-}
-
-// complex fallback bookkeeping: every partial of an instance whose real-kernel
-// value is NaN is marked NaN (to be recomputed), the others 0 (skipped)
-__global__ void tor_mark_kernel(const cplx* __restrict__ out, int np, int B, cplx* __restrict__ cpart) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)B * np) return;
-  const cplx v = out[e / np];
-  cpart[e] = (v.re != v.re) ? cplx{NAN, NAN} : cplx{0.0, 0.0};
 }
 
 // instances the complex kernel recomputed take its sum (same association as
@@ -1450,8 +1458,8 @@ static int launch_tor(int nm, const TorSplit& sp, const cplx* d_in, int B, typen
   }
   const int np = 1 << sp.T1;
   const bool bfs = Ar::TASK_REGS && nm <= tor_tune_of<Ar>().bfs_max;
-  const int64_t slots = (int64_t)B * np, scan = (slots + 63) / 64;
-  const dim3 grid((unsigned)(RERUN ? (scan < TOR_RERUN_GRID ? scan : TOR_RERUN_GRID) : slots));
+  const int64_t slots = (int64_t)B * np;
+  const dim3 grid((unsigned)(RERUN ? (slots < TOR_RERUN_GRID ? slots : TOR_RERUN_GRID) : slots));
   if (sp.T2 == TOR_T2) {
     switch (sp.R) {
 #define TOR_CASE(RR)                                                                                             \
@@ -1503,16 +1511,15 @@ static int run_tor_fp64(int nm, const cplx* d_in, int B, cplx* d_out, double* d_
   double* part_abs = reinterpret_cast<double*>(part + (size_t)B * npr);
   int rc = launch_tor<ArR, KAPPA, false>(nm, sr, d_in, B, part, part_abs, st);
   if (rc) return rc;
-  if constexpr (CERT)
-    hipLaunchKernelGGL(tor_reduce_cert_kernel, dim3(tor_red_grid(B)), dim3(256), 0, st, part, part_abs, npr, B,
-                       tor_cert_depth(nm, npr), d_out, d_aux);
-  else
-    hipLaunchKernelGGL((tor_reduce_kernel<false, KAPPA>), dim3(tor_red_grid(B)), dim3(256), 0, st, part, part_abs,
-                       npr, B, d_out, d_aux);
+  // the reduce also marks the partials the complex kernel recomputes
   cplx* cpart = reinterpret_cast<cplx*>(part_abs + (size_t)B * npr);
   double* cpart_abs = reinterpret_cast<double*>(cpart + (size_t)B * npc);
-  hipLaunchKernelGGL(tor_mark_kernel, dim3((unsigned)(((size_t)B * npc + 255) / 256)), dim3(256), 0, st, d_out, npc,
-                     B, cpart);
+  if constexpr (CERT)
+    hipLaunchKernelGGL(tor_reduce_cert_kernel, dim3(tor_red_grid(B)), dim3(256), 0, st, part, part_abs, npr, B,
+                       tor_cert_depth(nm, npr), d_out, d_aux, cpart, npc);
+  else
+    hipLaunchKernelGGL((tor_reduce_kernel<false, KAPPA>), dim3(tor_red_grid(B)), dim3(256), 0, st, part, part_abs,
+                       npr, B, d_out, d_aux, cpart, npc);
   rc = launch_tor<RC64, KAPPA, true>(nm, sc, d_in, B, cpart, cpart_abs, st);
   if (rc) return rc;
   hipLaunchKernelGGL((tor_merge_kernel<KAPPA, CERT>), dim3(tor_red_grid(B)), dim3(256), 0, st, cpart, cpart_abs, npc,
@@ -1527,7 +1534,7 @@ static int run_tor_dd(int nm, const cplx* d_in, int B, cplx* d_out, void* ws, hi
   int rc = launch_tor<RDD, false, false>(nm, sp, d_in, B, part, nullptr, st);
   if (rc) return rc;
   hipLaunchKernelGGL((tor_reduce_kernel<true, false>), dim3(tor_red_grid(B)), dim3(256), 0, st, part, nullptr, np, B,
-                     d_out, nullptr);
+                     d_out, nullptr, nullptr, 0);
   return (int)hipGetLastError();
 }
 
@@ -1684,12 +1691,12 @@ int tor_single_launch(int tier, int nm, const cplx* d_in, cplx* d_out, double* d
       rc = launch_tor<RF64, false, false>(nm, sp, d_in, 1, part, nullptr, st);
       if (!rc)
         hipLaunchKernelGGL((tor_reduce_kernel<false, false>), dim3(1), dim3(256), 0, st, part, nullptr, np, 1, d_out,
-                           nullptr);
+                           nullptr, nullptr, 0);
     } else {
       rc = launch_tor<RCERT, false, false>(nm, sp, d_in, 1, part, part_abs, st);
       if (!rc)
         hipLaunchKernelGGL(tor_reduce_cert_kernel, dim3(1), dim3(256), 0, st, part, part_abs, np, 1,
-                           tor_cert_depth(nm, np), d_out, d_aux);
+                           tor_cert_depth(nm, np), d_out, d_aux, nullptr, 0);
     }
   } else {
     dd* part = reinterpret_cast<dd*>(ws);
@@ -1698,7 +1705,7 @@ int tor_single_launch(int tier, int nm, const cplx* d_in, cplx* d_out, double* d
       rc = launch_tor<RDD, false, false>(nm, sp, d_in, 1, part, nullptr, st);
       if (!rc)
         hipLaunchKernelGGL((tor_reduce_kernel<true, false>), dim3(1), dim3(256), 0, st, part, nullptr, np, 1, d_out,
-                           nullptr);
+                           nullptr, nullptr, 0);
     } else {
       rc = launch_tor<RCDD, false, false>(nm, sp, d_in, 1, part, part_e, st);
       const double depth = 160.0 * nm + 64.0 + 96.0 + 4.0 * np + 64.0;

@@ -83,3 +83,43 @@ def test_sieve_canonical_flops():
     assert bench.sieve_flops((2,) * 6) == 365 * (4 * 36 + 48 + 22 * 6 + 20)
     assert bench.sieve_flops((6,) * 6) == 58825 * (4 * 36 + 48 + 22 * 18 + 20)
     assert bench.sieve_flops((0, 3, 0)) == 2 * (4 + 8 + 22 + 20)
+
+
+def test_reference_table_inputs_and_fixture():
+    """The reference-table leg's input recipes (core/bench_kernels.cu:58-72,
+    306-326) and the published rows every compared cell maps to."""
+    import json
+    import bench
+    from pathlib import Path
+    m = bench.reference_inputs("sym", 8, 3)
+    assert m.shape == (3, 8, 8) and np.array_equal(m, m.transpose(0, 2, 1))
+    assert np.abs(m.real).max() <= 1.0 and np.abs(m.imag).max() <= 1.0 and np.abs(m.imag).max() > 0.5
+    t = bench.reference_inputs("tor_real", 16, 2)
+    assert np.array_equal(t, t.transpose(0, 2, 1)) and not t.imag.any() and np.abs(t).max() <= 0.05
+    assert np.abs(bench.reference_inputs("tor", 8, 2)).max() <= 0.05 * np.sqrt(2)
+    c = bench.reference_inputs("cert_sym", 8, 2)
+    assert np.array_equal(c, c.transpose(0, 2, 1)) and np.abs(c.real).max() <= 0.3
+    assert not np.array_equal(bench.reference_inputs("plain", 8, 1)[0], bench.reference_inputs("plain", 8, 1)[0].T)
+    rows = json.loads((Path(bench.__file__).parent / "tests" / "golden" /
+                       "kernel_throughput_reference.json").read_text())["rows"]
+    assert all(r["evals_per_sec_median"] > 0 for r in rows)
+    have = {(r["func"], r["matrix_dim"], r["gpu"]) for r in rows}
+    for label, func, tier, kind, dims, ref_funcs in bench.REFERENCE_ROWS:
+        for D in dims:
+            for gpu in ("NVIDIA_GeForce_RTX_4090", "NVIDIA_A100-SXM4-40GB"):
+                assert any((f, D, gpu) in have for f in ref_funcs), (label, D, gpu)
+    # the published haf N = 8 medians (BASELINE kernel table)
+    got = {r["gpu"]: r["evals_per_sec_median"] for r in rows if r["func"] == "haf" and r["matrix_dim"] == 8}
+    assert got == {"NVIDIA_GeForce_RTX_4090": 5562051.0, "NVIDIA_A100-SXM4-40GB": 10541870.0}
+
+
+def test_executed_flop_table():
+    """Counter-measured executed work: FP64 wave instructions x 64 lanes x at
+    most 2 flops, and the helper's rates."""
+    import bench
+    for name, (flop, insts) in bench.EXECUTED_F64.items():
+        assert insts * 64 <= flop <= insts * 128, name
+    ex = bench.executed("perm_fp64_n24", 1024, 0.04)
+    assert ex["tflops_executed"] == pytest.approx(2.014479e9 * 1024 / 0.04 / 1e12)
+    assert 0.0 < ex["fp64_issue_frac"] < 1.0
+    assert bench.executed("no_such_cell", 1, 1.0) == {}
