@@ -108,7 +108,7 @@ def test_reference_table_inputs_and_fixture():
         for D in dims:
             for gpu in ("NVIDIA_GeForce_RTX_4090", "NVIDIA_A100-SXM4-40GB"):
                 assert any((f, D, gpu) in have for f in ref_funcs), (label, D, gpu)
-    # the published haf N = 8 medians (BASELINE kernel table)
+    # the published haf N = 8 medians (the reference's kernel table)
     got = {r["gpu"]: r["evals_per_sec_median"] for r in rows if r["func"] == "haf" and r["matrix_dim"] == 8}
     assert got == {"NVIDIA_GeForce_RTX_4090": 5562051.0, "NVIDIA_A100-SXM4-40GB": 10541870.0}
 
