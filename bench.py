@@ -136,14 +136,14 @@ def main() -> int:
     ap.add_argument("--counter-pass", action="store_true",
                     help="with --cells: run each named cell once, in the order given, with a one-element "
                          "fill kernel after each (splits a hardware-counter run by cell), and exit")
-    ap.add_argument("--detail", default="gpurun_out/bench_detail.json",
+    ap.add_argument("--detail", default="bench_detail.json",
                     help="file receiving the full measurement record (JSON)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (a known-size kernel that "
                          "calibrates hardware byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+        print(f"bench.py: measures one GPU only; --gpus {args.gpus} is not supported "
               "(instances are independent: several GPUs would each take a share of the batch)", file=sys.stderr)
         return 2
 
