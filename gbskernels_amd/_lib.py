@@ -72,6 +72,7 @@ SIGNATURES = {
     "gbs_error_string": ([_i], ctypes.c_char_p),
     "gbs_is_device_build": ([], _i),
     "gbs_check_div_sqrt": ([_cp, _cp, ctypes.c_longlong, _cp, _cp, _cp], _i),
+    "gbs_check_rsqrt": ([_cp, ctypes.c_longlong, _cp], _i),
     "gbs_batch_create": ([ctypes.POINTER(_cp), ctypes.c_size_t, _i, _i], _i),
     "gbs_batch_run": ([_cp, _i, _cp, _cp, _cp, _cp, _cp, _cp, _cp], _i),
     "gbs_batch_get_stats": ([_cp, ctypes.POINTER(BatchStats)], _i),

@@ -467,6 +467,24 @@ int gbs_check_div_sqrt(const double* a, const double* b, long long n, double* q,
   return rc ? rc : rs;
 }
 
+int gbs_check_rsqrt(const double* a, long long n, double* r) {
+  if (n <= 0) return 0;
+  if (!a || !r) return GBSK_ERR_ARG;
+  Ctx& c = global_ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  int rc = c.init();
+  const size_t bytes = (size_t)n * sizeof(double);
+  double* d = nullptr;  // a, r
+  if (!rc) rc = (int)hipMalloc(&d, 2 * bytes);
+  if (rc) return rc;
+  rc = (int)hipMemcpyAsync(d, a, bytes, hipMemcpyHostToDevice, c.stream);
+  if (!rc) rc = tor_rsqrt_probe(d, n, d + n, c.stream);
+  if (!rc) rc = (int)hipMemcpyAsync(r, d + n, bytes, hipMemcpyDeviceToHost, c.stream);
+  const int rs = (int)hipStreamSynchronize(c.stream);
+  (void)hipFree(d);
+  return rc ? rc : rs;
+}
+
 int gbs_tor_recursive_single_host(const double* O, int n, int g, double* out) {
   return tor_single_host(GBSK_FP64, O, n, g, out, nullptr);
 }

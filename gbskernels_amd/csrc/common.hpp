@@ -126,6 +126,22 @@ __device__ inline double c_abs(cplx z) {
 
 // ---------------------------------------------------------- wave utilities
 __device__ inline double shfl_xor_d(double v, int mask) { return __shfl_xor(v, mask, 64); }
+// 1 / sqrt(a), straight-line: the hardware estimate refined by two Newton
+// steps, y' = y + (y / 2) (1 - (a y) y) with the residual formed by one fma. A
+// step takes relative error d to 3 d^2 / 2 plus at most 3u/2 of its own
+// rounding; the estimate's d0 is only ~2^-24 for some operands on MI355X (one
+// step left 38u), so after two the error is 3u/2 + O(u^2) for a in
+// [2^-1000, 2^1000]: charged 2u there, checked on the device (gbs_check_rsqrt).
+// 8 instructions where the correctly rounded 1.0 / sqrt(a) takes about 30 (a
+// range branch around it made the torontonian walk spill). Outside that range
+// the result is unspecified: callers flag non-positive pivots, and the
+// certified tier refuses pivots outside it.
+__device__ inline double rsqrt_nr(double a) {
+  const double y0 = __builtin_amdgcn_rsq(a);
+  const double y1 = fma(0.5 * y0, fma(-(a * y0), y0, 1.0), y0);
+  return fma(0.5 * y1, fma(-(a * y1), y1, 1.0), y1);
+}
+
 // the partner lane's value (lane ^ 1) through a DPP quad permutation
 // [1, 0, 3, 2]: a VALU move, no LDS traffic (a shuffle is an LDS permute);
 // every lane of the wave must execute it

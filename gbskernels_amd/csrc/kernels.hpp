@@ -76,5 +76,7 @@ int sieve_launch_device(bool cert, const cplx* d_A, const cplx* d_g, int M, cons
 // certified hafnian's Newton step does
 int tor_div_sqrt_probe(const double* d_a, const double* d_b, int64_t n, double* d_q, double* d_s, hipStream_t st);
 int haf_div_probe(const double* d_a, int64_t n, double* d_out, hipStream_t st);
+// r = 1 / sqrt(a) as the torontonian's trailing walk evaluates it (rsqrt_nr)
+int tor_rsqrt_probe(const double* d_a, int64_t n, double* d_r, hipStream_t st);
 
 }  // namespace gbsk

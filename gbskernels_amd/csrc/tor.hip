@@ -67,6 +67,7 @@ struct RF64 {
   __device__ static T div(T a, T b) { return a / b; }
   __device__ static T recip(T a) { return 1.0 / a; }
   __device__ static T sqrt_(T a) { return sqrt(a); }
+  __device__ static T rsqrt(T a) { return rsqrt_nr(a); }  // 1 / sqrt(a), relative error <= 2u
   __device__ static T fms(T a, T b, T c) { return fma(-a, b, c); }  // c - a b
   __device__ static bool pos(T a) { return a > 0.0; }
   __device__ static double to_d(T a) { return a; }
@@ -101,6 +102,7 @@ struct RDD {
   __device__ static T div(T a, T b) { return dd_div(a, b); }
   __device__ static T recip(T a) { return dd_div(dd{1.0, 0.0}, a); }
   __device__ static T sqrt_(T a) { return dd_sqrt(a); }
+  __device__ static T rsqrt(T a) { return recip(sqrt_(a)); }
   __device__ static T fms(T a, T b, T c) { return dd_sub(c, dd_mul(a, b)); }
   __device__ static bool pos(T a) { return a.hi > 0.0; }
   __device__ static double to_d(T a) { return dd_to_d(a); }
@@ -164,6 +166,7 @@ struct RC64 {
   __device__ static T div(T a, T b) { return c_div(a, b); }
   __device__ static T recip(T a) { return c_div(cplx{1.0, 0.0}, a); }
   __device__ static T sqrt_(T a) { return c_sqrt(a); }
+  __device__ static T rsqrt(T a) { return recip(sqrt_(a)); }
   __device__ static T fms(T a, T b, T c) { return c_sub(c, c_mul(a, b)); }
   __device__ static bool pos(T a) { return a.re != 0.0 || a.im != 0.0; }
   __device__ static double to_d(T a) { return a.re; }
@@ -189,6 +192,10 @@ struct RC64 {
 //               e = 2 (ea + (1 + u)|v| eb) / |b| + u|v| + UFL
 //   sqrt(a)     needs a >= 4 ea (so a_exact > 0):
 //               e = (1 + u) ea / v + u v + UFL   (|sqrt x - sqrt y| <= |x - y| / sqrt y)
+//   1/sqrt(a)   (rsqrt_nr, relative error <= 2u on [2^-1000, 2^1000]) needs
+//               4 ea <= a and 2^-1000 <= a <= 2^600:
+//               e = 0.78 ea v^3 + 2u v + UFL     (a_exact >= 3a/4, so
+//               |x^-1/2 - a^-1/2| <= ea / (2 (3a/4)^(3/2)) = 0.7698 ea a^(-3/2))
 // A failed requirement makes the bound +inf (refusal); pivots are divided by
 // or square-rooted, so a certified bound also certifies that every exact
 // pivot is positive (the real formula applies).
@@ -246,6 +253,11 @@ struct RCERT {
     if (a.e == 0.0) return cv{v, fma(CERT_U, v, CERT_UFL)};
     if (!(a.v >= 4.0 * a.e)) return cv{v, INFINITY};
     return cv{v, fma(CERT_U, v, fma(1.0 + 2.0 * CERT_U, a.e / v, CERT_UFL))};
+  }
+  __device__ static T rsqrt(T a) {
+    const double v = rsqrt_nr(a.v);
+    if (!(a.v >= 4.0 * a.e) || !(a.v >= 0x1p-1000) || !(a.v <= 0x1p600)) return cv{v, INFINITY};
+    return cv{v, fma(2.0 * CERT_U, v, fma(0.78 * a.e, v * v * v, CERT_UFL))};
   }
   __device__ static bool pos(T a) { return a.v > 0.0; }
   template <int W>
@@ -333,6 +345,8 @@ struct RCDD {
     if (!(ly >= 2.0 * ey)) return cdd{y, INFINITY};
     return cdd{y, ey + 2.0 * a.e / ly};
   }
+  __device__ static T rsqrt(T a) { return recip(sqrt_(a)); }
+
   __device__ static bool pos(T a) { return a.v.hi > 0.0; }
   template <int W>
   __device__ static T sumw(T x) {
@@ -430,7 +444,7 @@ struct Bottom {
           L[r][c] = Ar::ISD ? Ar::mul(s, L[c][c]) : Ar::div(s, L[c][c]);
         } else {
           bad |= !Ar::pos(s);
-          L[r][r] = Ar::ISD ? Ar::recip(Ar::sqrt_(s)) : Ar::sqrt_(s);
+          L[r][r] = Ar::ISD ? Ar::rsqrt(s) : Ar::sqrt_(s);
         }
       }
     }
@@ -521,7 +535,7 @@ __device__ inline void bottom_walk(const typename Ar::T* M, typename Ar::T det, 
                                    typename Ar::T& acc, double& aacc) {
   constexpr int W = 2 * R > 0 ? 2 * R : 1;
   // ISD: the walk carries 1/sqrt of the determinant (upstream pivots here)
-  const typename Ar::T d0 = Ar::ISD ? Ar::recip(Ar::sqrt_(det)) : det;
+  const typename Ar::T d0 = Ar::ISD ? Ar::rsqrt(det) : det;
   if constexpr (R == 0) {
     tor_leaf<Ar>(d0, count, nm, acc, aacc);
   } else if constexpr (Ar::SYM) {
@@ -1667,6 +1681,22 @@ int tor_div_sqrt_probe(const double* d_a, const double* d_b, int64_t n, double* 
   const int64_t blocks = (n + 255) / 256;
   hipLaunchKernelGGL(tor_div_sqrt_probe_kernel, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0, st,
                      d_a, d_b, n, d_q, d_s);
+  return (int)hipGetLastError();
+}
+
+// The same for the reciprocal square root of the trailing walk's pivots
+// (RF64::rsqrt = RCERT::rsqrt's value): RCERT charges it 2u|result|, which a
+// host checks against an extended-precision 1 / sqrt(a).
+__global__ void tor_rsqrt_probe_kernel(const double* __restrict__ a, int64_t n, double* __restrict__ r) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    r[i] = RCERT::rsqrt(cv{a[i], 0.0}).v;
+}
+
+int tor_rsqrt_probe(const double* d_a, int64_t n, double* d_r, hipStream_t st) {
+  if (n <= 0) return 0;
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(tor_rsqrt_probe_kernel, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0, st,
+                     d_a, n, d_r);
   return (int)hipGetLastError();
 }
 

@@ -139,6 +139,13 @@ int gbs_is_device_build(void);
  * certified hafnian's Newton-step division. A caller compares them bit for bit
  * with IEEE-754 round-to-nearest results. Synchronous; allocates 5 n doubles. */
 int gbs_check_div_sqrt(const double* a, const double* b, long long n, double* q, double* s, double* h);
+/* Companion check (no reference counterpart): r[i] = 1 / sqrt(a[i]) as the
+ * torontonian's trailing walk evaluates its pivots' reciprocal square roots
+ * (the hardware estimate refined by one Newton step for a in [2^-1000,
+ * 2^1000], the correctly rounded expression elsewhere). The certified tier
+ * charges it 2u|r|; a caller compares with an extended-precision 1 / sqrt(a).
+ * Synchronous; allocates 2 n doubles. */
+int gbs_check_rsqrt(const double* a, long long n, double* r);
 
 /* ---- 5. heterogeneous batches (no reference counterpart) ------------------
  * Replaces the per-bucket loop of Workspace._ragged / _dispatch_bucket

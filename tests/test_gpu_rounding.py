@@ -4,7 +4,13 @@ square root it charges one rounding of u|result| must be correctly rounded
 very expressions the certified torontonian (RCERT::div, RCERT::sqrt_) and the
 certified hafnian's Newton step (division by the index) use; numpy's float64
 division and square root are correctly rounded on x86-64 (SSE2 divsd/sqrtsd,
-no flush-to-zero), so the device results must match them bit for bit."""
+no flush-to-zero), so the device results must match them bit for bit.
+
+The torontonian's trailing walk takes its pivots' reciprocal square roots from
+the hardware estimate refined by one Newton step (gbs_check_rsqrt); the
+certified tier charges that 2u|result| on [2^-1000, 2^1000] (and refuses
+pivots outside it), checked here against 1 / sqrt(a) in x87 extended precision
+(64-bit significand)."""
 import ctypes
 
 import numpy as np
@@ -88,3 +94,48 @@ def test_edge_operands_correctly_rounded():
     a, b = _edge_cases()
     _check(a, b)
     _check(-a, b)
+
+
+U = 2.0 ** -53
+
+
+def _rsqrt(a):
+    from gbskernels_amd import _lib
+    a = np.ascontiguousarray(a, np.float64)
+    r = np.empty(a.size, np.float64)
+    _lib.check(_lib.lib().gbs_check_rsqrt(_lib.ptr(a), ctypes.c_longlong(a.size), _lib.ptr(r)), "gbs_check_rsqrt")
+    return r
+
+
+def _check_rsqrt(a):
+    """worst relative error, in u, over the operands in [2^-1000, 2^1000]"""
+    assert np.finfo(np.longdouble).nmant >= 63, "needs x87 extended precision"
+    a = a[(a >= 2.0 ** -1000) & (a <= 2.0 ** 1000)]
+    r = _rsqrt(a)
+    exact = 1.0 / np.sqrt(a.astype(np.longdouble))
+    rel = np.abs((r.astype(np.longdouble) - exact) / exact)
+    worst = float(rel.max())
+    assert worst <= 2 * U, f"1/sqrt: relative error {worst / U:.3f} u > 2u at a={a[np.argmax(rel)]!r}"
+    return worst / U
+
+
+def test_rsqrt_within_two_ulp():
+    rng = np.random.default_rng(53)
+    n = 10_000_000
+    # pivots of I - O_S live in (0, 2]: dense there, then the whole range
+    piv = rng.uniform(0.0, 2.0, n // 2)
+    wide = np.ldexp(1.0 + rng.random(n // 2), rng.integers(-1074, 1024, n // 2))
+    worst = _check_rsqrt(np.concatenate([piv, wide]))
+    assert worst > 0.3  # the check sees the refinement's rounding (not a copy of the reference)
+
+
+def test_rsqrt_edges():
+    pw = np.ldexp(1.0, np.arange(-1074, 1024))
+    with np.errstate(over="ignore"):
+        near = np.concatenate([pw * (1 + 2.0 ** -52), pw * (1 - 2.0 ** -53), pw * 1.5, pw * 3])
+    ends = np.array([2.0 ** -1000, 2.0 ** 1000, np.nextafter(2.0 ** -1000, 1.0), np.nextafter(2.0 ** 1000, 0.0)])
+    _check_rsqrt(np.concatenate([pw, near[np.isfinite(near)], ends]))
+    # non-positive and NaN operands give a non-finite result (such pivots are
+    # flagged before their reciprocal root matters; zero gives NaN: 0 * inf)
+    r = _rsqrt(np.array([-1.0, -2.0 ** -600, np.nan, 0.0]))
+    assert not np.isfinite(r).any()
