@@ -612,6 +612,12 @@ __device__ inline void drop_first_mode(const T* X, int Dx, T* Y) {
 template <class Ar>
 constexpr int tor_min_waves() { return std::is_same<Ar, RF64>::value ? 3 : 1; }
 
+// Whether the breadth-first walk is split in two parts (half its LDS): not for
+// certified dd, whose walk spills 720 bytes per lane to scratch: more
+// workgroups per CU there only thrashed the caches (20 modes 1.15x slower).
+template <class Ar>
+constexpr bool tor_bfs_split() { return !std::is_same<Ar, RCDD>::value; }
+
 template <class Ar, int R, int T2, bool KAPPA, bool RERUN, bool BFS>
 __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int nm, int T1, int B,
                                               typename Ar::P* __restrict__ part, double* __restrict__ part_abs,
@@ -744,13 +750,19 @@ __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int
     // Pivots, complements and determinants are the depth-first walk's
     // operations, so the values are identical; the leaf level is formed in the
     // lanes' registers (lane k = leaf k = task k of the depth-first order).
+    // With all six middle modes the tree is walked in two parts split on the
+    // first mode (Q = 1), one after the other, each filling its 32 lanes'
+    // leaves: only one part's include matrices are held, 1950 instead of 3437
+    // entries at 11 modes below the leading ones (LDS allowed 5 workgroups per
+    // CU, the registers 12; now 8: 1.3x at 16-20 modes, certified 1.5x).
     static_assert(Ar::SYM, "breadth-first walk of the symmetric policies");
+    constexpr int Q = (T2 == TOR_T2 && tor_bfs_split<Ar>()) ? 1 : 0;  // split bits
     constexpr int NODES = 1 << (T2 - 1);
-    __shared__ T bpiv[NODES][3];         // ra, rc, b01 of the current level's nodes
-    __shared__ T bdet[2][1 << T2];       // determinants: current / next level
+    __shared__ T bpiv[NODES][3];            // ra, rc, b01 of the current level's nodes (part-local index)
+    __shared__ T bdet[(2 << T2) - 1];        // determinants, level d node n at 2^d - 1 + n
     __shared__ int boff[NODES], bsh[NODES];  // each node's matrix offset and row shift
-    __shared__ int region[TOR_T2 + 1];   // first entry of level d's formed matrices
-    const int base = nm - T1;            // modes of the level-0 matrix
+    __shared__ int region[TOR_T2 + 1];       // first entry of level d's formed matrices
+    const int base = nm - T1;                // modes of the level-0 matrix
     // (row, column) of every packed lower-triangle index a formed matrix can
     // hold (at most 2 (TOR_T2 + 5) - 2 rows below level 0)
     constexpr int TIJ = 256;
@@ -763,24 +775,29 @@ __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int
       tij_c[e] = (unsigned char)j;
     }
     if (lane == 0) {
-      bdet[0][0] = det;
-      int r = lev_size<true>(2 * base);  // level 1's region follows the level-0 matrix
+      // level 0, then per level d >= 1 the include matrices one part holds:
+      // 2^(d-1-Q) of them above the split, one on the split path
+      bdet[0] = det;
+      int r = lev_size<true>(2 * base);
       region[0] = 0;
       for (int d = 1; d <= T2; ++d) {
         region[d] = r;
-        r += (1 << (d - 1)) * lev_size<true>(2 * (base - d));
+        r += (d > Q ? (1 << (d - 1 - Q)) : 1) * lev_size<true>(2 * (base - d));
       }
     }
     __syncthreads();
-    for (int d = 0; d < T2; ++d) {
-      // pass 1: each node's matrix (offset, shift), pivots and its children's
-      // determinants
-      if (lane < (1 << d)) {
+    // pass 1 on the nodes n0 .. n0 + cnt - 1 of level d (part h): each node's
+    // matrix (offset, shift), pivots and its children's determinants
+    auto pivots = [&](int d, int h, int n0, int cnt) {
+      if (lane < cnt) {
+        const int n = n0 + lane;
         int off = 0, sh = 2 * d;
-        if (lane != 0) {
-          const int p = __builtin_ctz((unsigned)lane);  // excludes since the last include
-          const int lv = d - p;                         // level of the matrix that include formed
-          off = region[lv] + (lane >> (p + 1)) * lev_size<true>(2 * (base - lv));
+        if (n != 0) {
+          const int p = __builtin_ctz((unsigned)n);  // excludes since the last include
+          const int lv = d - p;                      // level of the matrix that include formed
+          const int g = n >> (p + 1);                // its parent at level lv - 1
+          const int loc = lv > Q ? g - (h << (lv - 1 - Q)) : 0;
+          off = region[lv] + loc * lev_size<true>(2 * (base - lv));
           sh = 2 * p;
         }
         const T* X = lev + off;
@@ -795,20 +812,21 @@ __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int
         boff[lane] = off;
         bsh[lane] = sh;
         bad |= !Ar::pos(a) || !Ar::pos(c);
-        const T dn = bdet[d & 1][lane];
-        bdet[(d + 1) & 1][2 * lane] = dn;
-        bdet[(d + 1) & 1][2 * lane + 1] = Ar::mul(dn, Ar::mul(a, c));
+        const T dn = bdet[(1 << d) - 1 + n];
+        bdet[(2 << d) - 1 + 2 * n] = dn;
+        bdet[(2 << d) - 1 + 2 * n + 1] = Ar::mul(dn, Ar::mul(a, c));
       }
       __syncthreads();
-      if (d + 1 == T2) break;
-      // pass 2: each node's include child (the Schur complement of its two
-      // pivots), formed at region[d + 1] + node * Ly. The flat index (node,
-      // entry) is split without an integer division (a float reciprocal, exact
-      // to one step below 2^24, then corrected) and the entry's (row, column)
-      // comes from the shared table.
+    };
+    // pass 2: the include child (the Schur complement of its two pivots) of
+    // each of the cnt nodes of pass 1, formed at region[d + 1] + node * Ly
+    // (part-local node index). The flat index (node, entry) is split without
+    // an integer division (a float reciprocal, exact to one step below 2^24,
+    // then corrected) and the entry's (row, column) comes from the shared table.
+    auto complements = [&](int d, int cnt) {
       const int Ly = lev_size<true>(2 * (base - d - 1));
       T* const Y = lev + region[d + 1];
-      const int total = (1 << d) * Ly;
+      const int total = cnt * Ly;
       const float rLy = 1.0f / (float)Ly;
       for (int idx = lane; idx < total; idx += 64) {
         int nu = (int)((float)idx * rLy);
@@ -828,30 +846,46 @@ __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int
         Y[idx] = Ar::fms(li1, x1j, v);
       }
       __syncthreads();
+    };
+    // the split path: level 0's node and its include child (level 1)
+    if constexpr (Q == 1) {
+      pivots(0, 0, 0, 1);
+      complements(0, 1);
     }
-    // leaves: lane k forms task k from its parent (level T2 - 1) in registers
     ntask = 1 << T2;
-    if (lane < ntask) {
-      const int nu = lane >> 1, sh = bsh[nu];
-      const T* X = lev + boff[nu];
-      const bool inc = lane & 1;
-      const T ra = bpiv[nu][0], rc = bpiv[nu][1], b01 = bpiv[nu][2];
-#pragma unroll
-      for (int e = 0; e < SLOT; ++e) {
-        const int i = tri_row(e) + 2 + sh, j = tri_col(e) + 2 + sh;
-        if (!inc) {
-          Mreg[e] = X[tri(i, j)];
-        } else {
-          const T li0 = Ar::mul(X[tri(i, sh)], ra);
-          const T li1 = Ar::mul(Ar::fms(li0, b01, X[tri(i, sh + 1)]), rc);
-          const T x0j = X[tri(j, sh)];
-          const T x1j = Ar::fms(Ar::mul(x0j, ra), b01, X[tri(j, sh + 1)]);
-          T v = Ar::fms(li0, x0j, X[tri(i, j)]);
-          Mreg[e] = Ar::fms(li1, x1j, v);
-        }
+#pragma unroll 1
+    for (int h = 0; h < (1 << Q); ++h) {
+      for (int d = Q; d < T2; ++d) {
+        const int cnt = 1 << (d - Q);
+        pivots(d, h, h * cnt, cnt);
+        if (d + 1 < T2) complements(d, cnt);
       }
-      my_det = bdet[T2 & 1][lane];
-      my_cnt = count + __popc((unsigned)lane);
+      // leaves of part h: lane k forms task k from its parent (level T2 - 1)
+      // in registers
+      const int k0 = h << (T2 - Q);
+      if (lane >= k0 && lane < k0 + (1 << (T2 - Q))) {
+        const int nu = (lane >> 1) - (k0 >> 1), sh = bsh[nu];
+        const T* X = lev + boff[nu];
+        const bool inc = lane & 1;
+        const T ra = bpiv[nu][0], rc = bpiv[nu][1], b01 = bpiv[nu][2];
+#pragma unroll
+        for (int e = 0; e < SLOT; ++e) {
+          const int i = tri_row(e) + 2 + sh, j = tri_col(e) + 2 + sh;
+          if (!inc) {
+            Mreg[e] = X[tri(i, j)];
+          } else {
+            const T li0 = Ar::mul(X[tri(i, sh)], ra);
+            const T li1 = Ar::mul(Ar::fms(li0, b01, X[tri(i, sh + 1)]), rc);
+            const T x0j = X[tri(j, sh)];
+            const T x1j = Ar::fms(Ar::mul(x0j, ra), b01, X[tri(j, sh + 1)]);
+            T v = Ar::fms(li0, x0j, X[tri(i, j)]);
+            Mreg[e] = Ar::fms(li1, x1j, v);
+          }
+        }
+        my_det = bdet[(1 << T2) - 1 + lane];
+        my_cnt = count + __popc((unsigned)lane);
+      }
+      __syncthreads();  // the next part overwrites the matrices above the split
     }
   } else {
     // cooperative walk over the T2 middle modes' include / exclude tree,
@@ -1405,13 +1439,15 @@ __global__ void tor_n0_kernel(int B, cplx* out, double* aux, double auxval) {
 }
 
 // LDS entries of the breadth-first walk: the level-0 matrix and, per level
-// d = 1 .. T2 - 1, the 2^(d-1) matrices its include children form
-static size_t tor_bfs_entries(int nm, const TorSplit& sp) {
+// d = 1 .. T2 - 1, the include matrices one part of the tree holds
+// (2^(d-1-Q) above the split, one on the split path; Q = 1 at T2 = TOR_T2)
+static size_t tor_bfs_entries(int nm, const TorSplit& sp, bool split) {
   const size_t base = (size_t)(nm - sp.T1);
-  size_t t = base * (2 * base + 1);  // lev_size(2 base)
+  const int Q = (sp.T2 == TOR_T2 && split) ? 1 : 0;  // the kernel's split bits (one part held at a time)
+  size_t t = base * (2 * base + 1);       // lev_size(2 base)
   for (int d = 1; d < sp.T2; ++d) {
     const size_t D = 2 * (base - d);
-    t += ((size_t)1 << (d - 1)) * (D * (D + 1) / 2);
+    t += (d > Q ? ((size_t)1 << (d - 1 - Q)) : (size_t)1) * (D * (D + 1) / 2);
   }
   return t;
 }
@@ -1424,7 +1460,7 @@ static size_t tor_lds_bytes(int nm, const TorSplit& sp, bool bfs) {
     const size_t Dd = 2 * (nm - sp.T1 - d);
     lev += Ar::SYM ? Dd * (Dd + 1) / 2 : Dd * Dd;  // the kernel's lev_size
   }
-  if (Ar::TASK_REGS && bfs) lev = tor_bfs_entries(nm, sp);
+  if (Ar::TASK_REGS && bfs) lev = tor_bfs_entries(nm, sp, tor_bfs_split<Ar>());
   const size_t slot = Ar::SYM ? (size_t)(2 * sp.R) * (2 * sp.R + 1) / 2 : (size_t)(2 * sp.R) * (2 * sp.R);
   const size_t T = sizeof(typename Ar::T);
   // task matrices in VGPRs: the level stack overlays the work matrix
