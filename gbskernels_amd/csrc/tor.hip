@@ -612,11 +612,20 @@ __device__ inline void drop_first_mode(const T* X, int Dx, T* Y) {
 template <class Ar>
 constexpr int tor_min_waves() { return std::is_same<Ar, RF64>::value ? 3 : 1; }
 
-// Whether the breadth-first walk is split in two parts (half its LDS): not for
-// certified dd, whose walk spills 720 bytes per lane to scratch: more
-// workgroups per CU there only thrashed the caches (20 modes 1.15x slower).
+#ifndef TOR_BFS_Q
+#define TOR_BFS_Q 1
+#endif
+
+// Bits the breadth-first walk is split on (2^Q parts walked one after the
+// other, one part's include matrices held): 1 (two parts) for fp64, dd and
+// certified; four parts held less LDS still but their extra passes cost more
+// than the occupancy gained (fp64 1.1-1.2x, dd 1.2-1.3x slower at 12-20
+// modes). 0 for certified dd, whose walk spills 720 bytes per lane to scratch:
+// more of its workgroups per CU only thrashed the caches (20 modes 1.15x slower).
 template <class Ar>
-constexpr bool tor_bfs_split() { return !std::is_same<Ar, RCDD>::value; }
+constexpr int tor_bfs_split() {
+  return std::is_same<Ar, RCDD>::value ? 0 : (std::is_same<Ar, RCERT>::value ? 1 : TOR_BFS_Q);
+}
 
 template <class Ar, int R, int T2, bool KAPPA, bool RERUN, bool BFS>
 __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int nm, int T1, int B,
@@ -750,13 +759,14 @@ __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int
     // Pivots, complements and determinants are the depth-first walk's
     // operations, so the values are identical; the leaf level is formed in the
     // lanes' registers (lane k = leaf k = task k of the depth-first order).
-    // With all six middle modes the tree is walked in two parts split on the
-    // first mode (Q = 1), one after the other, each filling its 32 lanes'
-    // leaves: only one part's include matrices are held, 1950 instead of 3437
-    // entries at 11 modes below the leading ones (LDS allowed 5 workgroups per
-    // CU, the registers 12; now 8: 1.3x at 16-20 modes, certified 1.5x).
+    // With all six middle modes the tree is walked in 2^Q parts split on the
+    // first Q modes (tor_bfs_split), one after the other, each filling its
+    // 2^(6-Q) lanes' leaves: only one part's include matrices (and the split
+    // path's) are held. At 11 modes below the leading ones: 3437 entries for
+    // the whole tree, 1950 at Q = 1, 1292 at Q = 2 (fp64: the LDS allowed 5
+    // workgroups per CU, the registers 12).
     static_assert(Ar::SYM, "breadth-first walk of the symmetric policies");
-    constexpr int Q = (T2 == TOR_T2 && tor_bfs_split<Ar>()) ? 1 : 0;  // split bits
+    constexpr int Q = (T2 == TOR_T2) ? tor_bfs_split<Ar>() : 0;  // split bits
     constexpr int NODES = 1 << (T2 - 1);
     __shared__ T bpiv[NODES][3];            // ra, rc, b01 of the current level's nodes (part-local index)
     __shared__ T bdet[(2 << T2) - 1];        // determinants, level d node n at 2^d - 1 + n
@@ -847,14 +857,21 @@ __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int
       }
       __syncthreads();
     };
-    // the split path: level 0's node and its include child (level 1)
-    if constexpr (Q == 1) {
+    // the split path: level 0's node and its include child (level 1), once;
+    // deeper path levels l < Q each time the part's node there changes
+    if constexpr (Q >= 1) {
       pivots(0, 0, 0, 1);
       complements(0, 1);
     }
     ntask = 1 << T2;
 #pragma unroll 1
     for (int h = 0; h < (1 << Q); ++h) {
+      for (int l = 1; l < Q; ++l) {
+        if (h % (1 << (Q - l)) == 0) {
+          pivots(l, 0, h >> (Q - l), 1);
+          complements(l, 1);
+        }
+      }
       for (int d = Q; d < T2; ++d) {
         const int cnt = 1 << (d - Q);
         pivots(d, h, h * cnt, cnt);
@@ -1440,10 +1457,11 @@ __global__ void tor_n0_kernel(int B, cplx* out, double* aux, double auxval) {
 
 // LDS entries of the breadth-first walk: the level-0 matrix and, per level
 // d = 1 .. T2 - 1, the include matrices one part of the tree holds
-// (2^(d-1-Q) above the split, one on the split path; Q = 1 at T2 = TOR_T2)
-static size_t tor_bfs_entries(int nm, const TorSplit& sp, bool split) {
+// (2^(d-1-Q) above the split, one on the split path; Q = tor_bfs_split at
+// T2 = TOR_T2)
+static size_t tor_bfs_entries(int nm, const TorSplit& sp, int split) {
   const size_t base = (size_t)(nm - sp.T1);
-  const int Q = (sp.T2 == TOR_T2 && split) ? 1 : 0;  // the kernel's split bits (one part held at a time)
+  const int Q = sp.T2 == TOR_T2 ? split : 0;  // the kernel's split bits (one part held at a time)
   size_t t = base * (2 * base + 1);       // lev_size(2 base)
   for (int d = 1; d < sp.T2; ++d) {
     const size_t D = 2 * (base - d);
