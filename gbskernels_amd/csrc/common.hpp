@@ -145,12 +145,18 @@ __device__ inline double rsqrt_nr(double a) {
 // the partner lane's value (lane ^ 1) through a DPP quad permutation
 // [1, 0, 3, 2]: a VALU move, no LDS traffic (a shuffle is an LDS permute);
 // every lane of the wave must execute it
-__device__ inline double swap_pair_d(double v) {
+// a double moved across lanes by one DPP pattern (CTRL: 0xB1 lane ^ 1 and
+// 0x4E lane ^ 2 inside each quad, 0x00 / 0x55 / 0xAA / 0xFF quad lane 0 / 1 / 2 /
+// 3 to the whole quad, 0x104 lane + 4 inside each row of 16, where lanes 12..15
+// keep an unspecified value)
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
   const long long b = __builtin_bit_cast(long long, v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, 0xB1, 0xF, 0xF, false);
-  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0xB1, 0xF, 0xF, false);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
   return __builtin_bit_cast(double, (long long)(((unsigned long long)hi << 32) | lo));
 }
+__device__ inline double swap_pair_d(double v) { return dpp_d<0xB1>(v); }
 
 // Sum over groups of G consecutive lanes (G a power of two), butterfly order:
 // every lane of the group ends holding the same, deterministically ordered sum.

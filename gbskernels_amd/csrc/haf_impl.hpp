@@ -53,6 +53,10 @@ struct ArF64 {
   __device__ static cplx to_c(T a) { return a; }
   __device__ static T shfl_xor(T a, int m) { return cplx{shfl_xor_d(a.re, m), shfl_xor_d(a.im, m)}; }
   __device__ static T swap_pair(T a) { return cplx{swap_pair_d(a.re), swap_pair_d(a.im)}; }
+  template <int CTRL>
+  __device__ static T dpp(T a) {
+    return cplx{dpp_d<CTRL>(a.re), dpp_d<CTRL>(a.im)};
+  }
 };
 
 __device__ inline dd dd_mul_d_acc(dd a, double b, dd acc) { return dd_add(acc, dd_mul_d(a, b)); }
@@ -80,6 +84,10 @@ struct ArDD {
   }
   __device__ static T swap_pair(T a) {
     return ddc{dd{swap_pair_d(a.re.hi), swap_pair_d(a.re.lo)}, dd{swap_pair_d(a.im.hi), swap_pair_d(a.im.lo)}};
+  }
+  template <int CTRL>
+  __device__ static T dpp(T a) {
+    return ddc{dd{dpp_d<CTRL>(a.re.hi), dpp_d<CTRL>(a.re.lo)}, dd{dpp_d<CTRL>(a.im.hi), dpp_d<CTRL>(a.im.lo)}};
   }
 };
 
@@ -114,6 +122,17 @@ __device__ inline void unrank(int q, int n, int (&p)[M]) {
 // group receives the same sum. All 64 lanes must call it (barriers).
 template <class Ar, int S>
 __device__ inline typename Ar::T group_reduce(typename Ar::T v, typename Ar::T* red) {
+  if constexpr (std::is_same<Ar, ArF64>::value && (S == 2 || S == 4)) {
+    // groups inside one quad: each group value arrives by a DPP quad broadcast
+    // (no LDS round trip, no barrier); the same adds in the same order
+    (void)red;
+    if constexpr (S == 2) {
+      return Ar::add(Ar::template dpp<0xA0>(v), Ar::template dpp<0xF5>(v));
+    } else {
+      const cplx s = Ar::add(Ar::template dpp<0x00>(v), Ar::template dpp<0x55>(v));
+      return Ar::add(Ar::add(s, Ar::template dpp<0xAA>(v)), Ar::template dpp<0xFF>(v));
+    }
+  }
   const int lane = threadIdx.x;
   red[lane] = v;
   __syncthreads();
@@ -194,6 +213,9 @@ __device__ inline void ddcert_group_reduce(ddc& v, double& e, ddc* red, double* 
 // The split depends on (N, S) only; chunk partials are summed in chunk order.
 constexpr int HAF_CHUNK_ITERS = 1;
 __host__ __device__ inline int haf_chunk_size(int S) { return HAF_CHUNK_ITERS * (64 / S); }
+__host__ __device__ constexpr int haf_cbinom(int n, int k) {
+  return (k < 0 || k > n) ? 0 : (k == 0 ? 1 : haf_cbinom(n - 1, k - 1) * n / k);
+}
 __host__ __device__ inline int haf_binom(int n, int k) {
   if (k < 0 || k > n) return 0;
   int r = 1;
@@ -596,7 +618,27 @@ __device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, in
   // Newton: e_j = (1/j) sum_{k=1..j} kg_k e_{j-k}, by the first lane of each group
   T coeff = Ar::zero();
   double ecoeff = 0.0;
-  if (r == 0 && lane_ok) {
+  if constexpr (!ANYC && std::is_same<Ar, ArF64>::value && Scratch::P <= 7) {
+    // few pairs (the fused small-size kernel): every lane runs the recurrence
+    // with kg and e in registers, unrolled, instead of one lane walking them in
+    // LDS; the same operations in the same order, so the same bits
+    const int gg = lane_ok ? g : 0;
+    T kg[Scratch::P + 1];
+#pragma unroll
+    for (int k = 1; k <= Scratch::P; ++k) kg[k] = kgs[gg][k <= n ? k : 1];
+    T e[Scratch::P + 1];
+    e[0] = Ar::one();
+#pragma unroll
+    for (int jj = 1; jj <= Scratch::P; ++jj) {
+      if (jj <= n) {  // uniform: n is the workgroup's
+        T a = Ar::zero();
+#pragma unroll
+        for (int k = 1; k <= jj; ++k) a = Ar::fma(kg[k], e[jj - k], a);
+        e[jj] = Ar::div_int(a, jj);
+        if (jj == n) coeff = e[jj];
+      }
+    }
+  } else if (r == 0 && lane_ok) {
     T* e = es[g];
     e[0] = Ar::one();
     if (ANYC) ese[g][0] = 0.0;
@@ -882,7 +924,9 @@ struct HafSmallJobs {
 template <class Ar, bool LOOP, int MODE, int PAIRS, int S>
 __device__ __forceinline__ void haf_small_phase(const cplx* As, int N, int ninst, void* scratch,
                                                 typename Ar::T (*terms)[haf_small_terms(PAIRS)],
-                                                double (*tabs)[haf_small_terms(PAIRS)]) {
+                                                double (*tabs)[haf_small_terms(PAIRS)],
+                                                typename Ar::T (*parts)[HAF_SMALL_MAX_PARTS],
+                                                double (*pabs)[HAF_SMALL_MAX_PARTS]) {
   using T = typename Ar::T;
   using Sc = HafScratch<Ar, LOOP, S, MODE, PAIRS + 1>;
   constexpr int G = 64 / S;
@@ -909,14 +953,48 @@ __device__ __forceinline__ void haf_small_phase(const cplx* As, int N, int ninst
     }
   }
   __syncthreads();
+  // this phase's chunk partials, in the phase kernel's association: chunk
+  // `slot` of instance i sums the G groups of one round (terms q0 .. q0 + G - 1,
+  // idle groups past the last subset adding zero) in group order. G is a
+  // compile-time constant here, so every term read is issued ahead of the add
+  // chain (reads past the last subset are clamped to it and then discarded).
+  // A chunk holds at most KMAX = min(G, C(PAIRS, S/2)) subsets, so only its
+  // first KMAX groups are read; the idle groups after them add zero in the tail.
+  constexpr int KMAX = G < haf_cbinom(PAIRS, S / 2) ? G : haf_cbinom(PAIRS, S / 2);
+  const int nch = haf_nchunks(n, S);
+  const int pbase = haf_part_offset(n, S);
+  for (int idx = lane; idx < ninst * nch; idx += 64) {
+    const int i = idx / nch, slot = idx - i * nch, q0 = slot * G;
+    const T* ti = terms[i] + off;
+    T acc = ti[q0];
+    double a = (MODE == HM_KAPPA) ? tabs[i][off + q0] : 0.0;
+#pragma unroll
+    for (int k = 1; k < KMAX; ++k) {
+      const bool on = q0 + k < count;
+      const int qk = on ? q0 + k : q0;
+      const T x = ti[qk];
+      acc = Ar::add(acc, on ? x : Ar::zero());
+      if (MODE == HM_KAPPA) a += on ? tabs[i][off + qk] : 0.0;
+    }
+    for (int k = KMAX; k < G; ++k) {
+      acc = Ar::add(acc, Ar::zero());
+      if (MODE == HM_KAPPA) a += 0.0;
+    }
+    parts[i][pbase + slot] = acc;
+    if (MODE == HM_KAPPA) pabs[i][pbase + slot] = a;
+  }
 }
 
 template <class Ar, bool LOOP, int MODE, int PAIRS, int... Ss>
 __device__ __forceinline__ void haf_small_phases(const cplx* As, int N, int ninst, void* scratch,
                                                  typename Ar::T (*terms)[haf_small_terms(PAIRS)],
                                                  double (*tabs)[haf_small_terms(PAIRS)],
+                                                 typename Ar::T (*parts)[HAF_SMALL_MAX_PARTS],
+                                                 double (*pabs)[HAF_SMALL_MAX_PARTS],
                                                  std::integer_sequence<int, Ss...>) {
-  ((2 * (Ss + 1) <= N ? (haf_small_phase<Ar, LOOP, MODE, PAIRS, 2 * (Ss + 1)>(As, N, ninst, scratch, terms, tabs), 0)
+  ((2 * (Ss + 1) <= N ? (haf_small_phase<Ar, LOOP, MODE, PAIRS, 2 * (Ss + 1)>(As, N, ninst, scratch, terms, tabs,
+                                                                              parts, pabs),
+                         0)
                       : 0),
    ...);
 }
@@ -953,45 +1031,56 @@ __global__ __launch_bounds__(64) void haf_small_kernel(const HafSmallJobs jobs) 
     As[e] = v;
   }
   __syncthreads();
-  haf_small_phases<Ar, LOOP, MODE, PAIRS>(As, N, ninst, scratch, terms, KAPPA ? tabs : nullptr,
-                                          std::make_integer_sequence<int, PAIRS>{});
-  // chunk partials, in the phase kernels' slot order and association
-  const int np = haf_nparts(n);
-  for (int idx = lane; idx < ninst * np; idx += 64) {
-    const int i = idx / np;
-    int slot = idx % np, m = 1;
-    while (slot >= haf_nchunks(n, 2 * m)) slot -= haf_nchunks(n, 2 * m++);
-    const int S = 2 * m, G = 64 / S, count = haf_binom(n, m);
-    const int off = haf_small_term_offset(n, S), q0 = slot * G;
-    T acc = terms[i][off + q0];
-    double a = KAPPA ? tabs[i][off + q0] : 0.0;
-    for (int k = 1; k < G; ++k) {
-      const bool on = q0 + k < count;
-      acc = Ar::add(acc, on ? terms[i][off + q0 + k] : Ar::zero());
-      if (KAPPA) a += on ? tabs[i][off + q0 + k] : 0.0;
-    }
-    parts[i][idx % np] = acc;
-    if (KAPPA) pabs[i][idx % np] = a;
-  }
+  haf_small_phases<Ar, LOOP, MODE, PAIRS>(As, N, ninst, scratch, terms, KAPPA ? tabs : nullptr, parts,
+                                          KAPPA ? pabs : nullptr, std::make_integer_sequence<int, PAIRS>{});
   __syncthreads();
-  // the reduce kernel's butterfly over the partials, one instance at a time
-  for (int i = 0; i < ninst; ++i) {
-    T s = Ar::zero();
-    double a = 0.0;
-    if (lane < np) {
-      s = parts[i][lane];
-      if (KAPPA) a = pabs[i][lane];
-    }
+  // the reduce kernel's butterfly over each instance's partials, for lane 0,
+  // whose result is stored: with at most 8 partials (lanes 0..7, exact zeros
+  // above), levels 1, 2 are quad swaps, level 4 brings lane 4 to lane 0, and
+  // levels 8, 16, 32 add the zero sums of lanes 8.., 16.., 32.. . Each as a DPP
+  // move or a zero, the same adds in the same order; the instances' chains are
+  // interleaved.
+  static_assert(HAF_SMALL_MAX_PARTS <= 8, "the butterfly below covers 8 partials");
+  const int np = haf_nparts(n);
+  T s[HAF_SMALL_MAX_T];
+  double a[KAPPA ? HAF_SMALL_MAX_T : 1];
 #pragma unroll
-    for (int k = 1; k < 64; k <<= 1) {
-      const T o = Ar::shfl_xor(s, k);
-      const double ao = KAPPA ? shfl_xor_d(a, k) : 0.0;
-      s = Ar::add(s, o);
-      if (KAPPA) a += ao;
+  for (int i = 0; i < HAF_SMALL_MAX_T; ++i) {
+    const bool on = i < ninst && lane < np;
+    const T x = parts[i][on ? lane : 0];
+    s[i] = on ? x : Ar::zero();
+    if (KAPPA) a[i] = on ? pabs[i][on ? lane : 0] : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < HAF_SMALL_MAX_T; ++i) {
+    s[i] = Ar::add(s[i], Ar::template dpp<0xB1>(s[i]));
+    if (KAPPA) a[i] += dpp_d<0xB1>(a[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < HAF_SMALL_MAX_T; ++i) {
+    s[i] = Ar::add(s[i], Ar::template dpp<0x4E>(s[i]));
+    if (KAPPA) a[i] += dpp_d<0x4E>(a[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < HAF_SMALL_MAX_T; ++i) {
+    s[i] = Ar::add(s[i], Ar::template dpp<0x104>(s[i]));
+    if (KAPPA) a[i] += dpp_d<0x104>(a[i]);
+  }
+#pragma unroll
+  for (int k = 8; k < 64; k <<= 1) {
+#pragma unroll
+    for (int i = 0; i < HAF_SMALL_MAX_T; ++i) {
+      s[i] = Ar::add(s[i], Ar::zero());
+      if (KAPPA) a[i] += 0.0;
     }
-    if (lane == 0) {
-      jobs.out[jb][b0 + i] = Ar::to_c(s);
-      if (KAPPA) jobs.aux[jb][b0 + i] = a;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < HAF_SMALL_MAX_T; ++i) {
+      if (i < ninst) {
+        jobs.out[jb][b0 + i] = Ar::to_c(s[i]);
+        if (KAPPA) jobs.aux[jb][b0 + i] = a[i];
+      }
     }
   }
 }
