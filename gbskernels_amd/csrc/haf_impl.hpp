@@ -38,6 +38,7 @@ constexpr int HAF_MAX_PAIRS = HAF_MAX_N / 2;
 // ------------------------------------------------------------ arithmetic policies
 struct ArF64 {
   using T = cplx;
+  using E = cplx;  // staged matrix entry
   __device__ static T zero() { return cplx{0.0, 0.0}; }
   __device__ static T one() { return cplx{1.0, 0.0}; }
   __device__ static T from(cplx z) { return z; }
@@ -59,10 +60,39 @@ struct ArF64 {
   }
 };
 
+// Real fp64 arithmetic for matrices whose imaginary parts are all zero (the
+// fused small-size kernel picks it per workgroup). Each operation is the real
+// part of ArF64's on such operands: c_fma's real part is
+// fma(x.re, y.re, fma(-x.im, y.im, a.re)) and the inner fma adds a zero product
+// to a.re, c_add / c_scale / div_int act per component, so the values agree
+// with ArF64's (an exact zero can at most differ in sign) and every imaginary
+// part stays zero.
+struct ArR64 {
+  using T = double;
+  using E = double;
+  __device__ static T zero() { return 0.0; }
+  __device__ static T one() { return 1.0; }
+  __device__ static T from(double z) { return z; }
+  __device__ static T add(T a, T b) { return a + b; }
+  __device__ static T neg(T a) { return -a; }
+  __device__ static T fma_c(T a, double z, T acc) { return fma(a, z, acc); }
+  __device__ static T fma(T a, T b, T acc) { return ::fma(a, b, acc); }
+  __device__ static T scale(T a, double s) { return a * s; }
+  __device__ static T div_int(T a, int j) { return a / j; }
+  __device__ static cplx to_c(T a) { return cplx{a, 0.0}; }
+  __device__ static T shfl_xor(T a, int m) { return shfl_xor_d(a, m); }
+  __device__ static T swap_pair(T a) { return swap_pair_d(a); }
+  template <int CTRL>
+  __device__ static T dpp(T a) {
+    return dpp_d<CTRL>(a);
+  }
+};
+
 __device__ inline dd dd_mul_d_acc(dd a, double b, dd acc) { return dd_add(acc, dd_mul_d(a, b)); }
 
 struct ArDD {
   using T = ddc;
+  using E = cplx;
   __device__ static T zero() { return ddc_zero(); }
   __device__ static T one() { return ddc{dd{1.0, 0.0}, dd{0.0, 0.0}}; }
   __device__ static T from(cplx z) { return ddc_from(z); }
@@ -122,14 +152,14 @@ __device__ inline void unrank(int q, int n, int (&p)[M]) {
 // group receives the same sum. All 64 lanes must call it (barriers).
 template <class Ar, int S>
 __device__ inline typename Ar::T group_reduce(typename Ar::T v, typename Ar::T* red) {
-  if constexpr (std::is_same<Ar, ArF64>::value && (S == 2 || S == 4)) {
+  if constexpr ((std::is_same<Ar, ArF64>::value || std::is_same<Ar, ArR64>::value) && (S == 2 || S == 4)) {
     // groups inside one quad: each group value arrives by a DPP quad broadcast
     // (no LDS round trip, no barrier); the same adds in the same order
     (void)red;
     if constexpr (S == 2) {
       return Ar::add(Ar::template dpp<0xA0>(v), Ar::template dpp<0xF5>(v));
     } else {
-      const cplx s = Ar::add(Ar::template dpp<0x00>(v), Ar::template dpp<0x55>(v));
+      const typename Ar::T s = Ar::add(Ar::template dpp<0x00>(v), Ar::template dpp<0x55>(v));
       return Ar::add(Ar::add(s, Ar::template dpp<0xAA>(v)), Ar::template dpp<0xFF>(v));
     }
   }
@@ -213,6 +243,14 @@ __device__ inline void ddcert_group_reduce(ddc& v, double& e, ddc* red, double* 
 // The split depends on (N, S) only; chunk partials are summed in chunk order.
 constexpr int HAF_CHUNK_ITERS = 1;
 __host__ __device__ inline int haf_chunk_size(int S) { return HAF_CHUNK_ITERS * (64 / S); }
+// a / b for 0 <= a < 2^16, 1 <= b <= 2^10, through one fp32 reciprocal: the
+// exact quotient (a + 1/2) / b lies at least 1/(2b) from an integer and the
+// fp32 product is within (a + 1/2) / b * 2^-21 < 2^-5 / b of it, so the
+// truncation is floor(a / b) (a few instructions where an integer division by
+// a runtime divisor takes about twenty)
+__device__ __forceinline__ int small_div(int a, int b) {
+  return (int)(((float)a + 0.5f) * __builtin_amdgcn_rcpf((float)b));
+}
 __host__ __device__ constexpr int haf_cbinom(int n, int k) {
   return (k < 0 || k > n) ? 0 : (k == 0 ? 1 : haf_cbinom(n - 1, k - 1) * n / k);
 }
@@ -292,11 +330,12 @@ struct HafScratch {
 // (-1)^(n - m) [x^n] exp(sum_k g_k x^k) in coeff and, in the certified modes,
 // its bound in ecoeff. The arithmetic depends on (As contents, N, q) only.
 template <class Ar, bool LOOP, int S, int MODE, class Scratch>
-__device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, int ab, int N, int q, Scratch& sh,
+__device__ __forceinline__ void haf_subset_term(const typename Ar::E* __restrict__ As0, int ab, int N, int q, Scratch& sh,
                                                 typename Ar::T& coeff_out, double& ecoeff_out) {
   // As0 is the workgroup's LDS array and ab this group's matrix offset in it
   // (an integer offset keeps every access a plain LDS address)
-  const cplx* __restrict__ As = As0 + ab;
+  using E = typename Ar::E;
+  const E* __restrict__ As = As0 + ab;
   using T = typename Ar::T;
   constexpr bool KAPPA = (MODE == HM_KAPPA);
   constexpr bool CERT = (MODE == HM_CERT);
@@ -333,8 +372,8 @@ __device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, in
   double eP[ANYC ? S : 1];
 #pragma unroll
   for (int u = 0; u < M; ++u) {
-    P[2 * u] = Ar::from(As[R * N + 2 * p[u] + 1]);
-    P[2 * u + 1] = Ar::from(As[R * N + 2 * p[u]]);
+    P[2 * u] = Ar::from(As[__mul24(R, N) + 2 * p[u] + 1]);
+    P[2 * u + 1] = Ar::from(As[__mul24(R, N) + 2 * p[u]]);
   }
   if (ANYC) {
 #pragma unroll
@@ -352,13 +391,13 @@ __device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, in
     // VGPRs) out of the power loop, which halved the loop kernels' occupancy
     int zo = 0;
     asm volatile("" : "+v"(zo));
-    const cplx* __restrict__ Ad = As + zo;
+    const E* __restrict__ Ad = As + zo;
     u = Ar::zero();
     eu = CERT ? (double)S * CERT_UFL : 0.0;
 #pragma unroll
     for (int c = 0; c < S; ++c) {
       const int Rc = 2 * p[c >> 1] + (c & 1);
-      const cplx dc = Ad[Rc * N + Rc];
+      const E dc = Ad[__mul24(Rc, N + 1)];
       if constexpr (DDC) {
         double o;
         u = cddc_fma_c(row[c], dc, u, &o);
@@ -401,7 +440,7 @@ __device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, in
   T uprev = Ar::zero();  // u_{j-1}
   double euprev = 0.0;
   if constexpr (LOOP) {
-    const T u0 = Ar::from(As[R * N + R]);
+    const T u0 = Ar::from(As[__mul24(R, N + 1)]);
     T u1;
     double eu1;
     dot_d(P, eP, u1, eu1);
@@ -413,7 +452,7 @@ __device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, in
   }
   // p_1 = tr C (C[r][r] = A[R][R^1]) and p_2 = tr(C C) through the partner row
   {
-    T d1 = Ar::from(As[R * N + (R ^ 1)]);
+    T d1 = Ar::from(As[__mul24(R, N) + (R ^ 1)]);
     T t2 = Ar::zero();
     double e2 = 0.0;
 #pragma unroll
@@ -491,14 +530,14 @@ __device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, in
     };
     // one term of the fma-chain dot product acc[c] += P[t] C[t][c] (and, in the
     // certified modes, its bound: sum_t (eP_t + gamma_2S |P_t|) |C_tc|)
-    auto term = [&](int c, int t, const cplx& x) {
+    auto term = [&](int c, int t, const E& x) {
       if constexpr (DDC) {
         double o;
         acc[c] = cddc_fma_c(P[t], x, acc[c], &o);
         eA[c] = fma(eP[t], cert_norm1(x), eA[c] + o);
       } else {
         acc[c] = Ar::fma_c(P[t], x, acc[c]);
-        if (CERT) eA[c] = fma(wrow(t), cert_norm1(x), eA[c]);
+        if constexpr (CERT) eA[c] = fma(wrow(t), cert_norm1(x), eA[c]);
       }
     };
     if constexpr (DDC) {
@@ -507,17 +546,17 @@ __device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, in
       // compilation take over half an hour
 #pragma unroll
       for (int t = 0; t < S; ++t) {
-        const cplx* At = As + (2 * p[t >> 1] + (t & 1)) * N;
+        const E* At = As + __mul24(2 * p[t >> 1] + (t & 1), N);
 #pragma unroll
         for (int cc = 0; cc < S; ++cc) term(cc ^ 1, t, At[colv[cc >> 1] + (cc & 1)]);
       }
     } else {
 #pragma unroll
       for (int t = 0; t < S; ++t) {
-        const cplx* At = As + (2 * p[t >> 1] + (t & 1)) * N;
+        const E* At = As + __mul24(2 * p[t >> 1] + (t & 1), N);
 #pragma unroll
         for (int cc = t; cc < S; ++cc) {
-          const cplx x = At[colv[cc >> 1] + (cc & 1)];  // C[t][cc ^ 1] = A[R_t][R_cc]
+          const E x = At[colv[cc >> 1] + (cc & 1)];  // C[t][cc ^ 1] = A[R_t][R_cc]
           term(cc ^ 1, t, x);
           if (cc != t) term(t ^ 1, cc, x);
         }
@@ -618,7 +657,7 @@ __device__ __forceinline__ void haf_subset_term(const cplx* __restrict__ As0, in
   // Newton: e_j = (1/j) sum_{k=1..j} kg_k e_{j-k}, by the first lane of each group
   T coeff = Ar::zero();
   double ecoeff = 0.0;
-  if constexpr (!ANYC && std::is_same<Ar, ArF64>::value && Scratch::P <= 7) {
+  if constexpr (!ANYC && (std::is_same<Ar, ArF64>::value || std::is_same<Ar, ArR64>::value) && Scratch::P <= 7) {
     // few pairs (the fused small-size kernel): every lane runs the recurrence
     // with kg and e in registers, unrolled, instead of one lane walking them in
     // LDS; the same operations in the same order, so the same bits
@@ -922,7 +961,7 @@ struct HafSmallJobs {
 // receives instance i's subset-q term as the phase kernel's group accumulator
 // holds it (zero + term), tabs the kappa magnitude (0 + |term|)
 template <class Ar, bool LOOP, int MODE, int PAIRS, int S>
-__device__ __forceinline__ void haf_small_phase(const cplx* As, int N, int ninst, void* scratch,
+__device__ __forceinline__ void haf_small_phase(const typename Ar::E* As, int N, int ninst, void* scratch,
                                                 typename Ar::T (*terms)[haf_small_terms(PAIRS)],
                                                 double (*tabs)[haf_small_terms(PAIRS)],
                                                 typename Ar::T (*parts)[HAF_SMALL_MAX_PARTS],
@@ -941,11 +980,12 @@ __device__ __forceinline__ void haf_small_phase(const cplx* As, int N, int ninst
   for (int base = 0; base < total; base += G) {  // wave-uniform
     const int item = base + g;
     const bool active = lane_ok && item < total;
-    const int inst = active ? item / count : 0;
-    const int q = active ? item % count : 0;
+    const int iq = small_div(item, count);
+    const int inst = active ? iq : 0;
+    const int q = active ? item - iq * count : 0;
     T coeff;
     double ecoeff;
-    haf_subset_term<Ar, LOOP, S, MODE>(As, inst * N * N, N, q, sh, coeff, ecoeff);
+    haf_subset_term<Ar, LOOP, S, MODE>(As, __mul24(inst, N * N), N, q, sh, coeff, ecoeff);
     (void)ecoeff;
     if (active && r == 0) {
       terms[inst][off + q] = Ar::add(Ar::zero(), coeff);
@@ -964,7 +1004,7 @@ __device__ __forceinline__ void haf_small_phase(const cplx* As, int N, int ninst
   const int nch = haf_nchunks(n, S);
   const int pbase = haf_part_offset(n, S);
   for (int idx = lane; idx < ninst * nch; idx += 64) {
-    const int i = idx / nch, slot = idx - i * nch, q0 = slot * G;
+    const int i = small_div(idx, nch), slot = idx - i * nch, q0 = slot * G;
     const T* ti = terms[i] + off;
     T acc = ti[q0];
     double a = (MODE == HM_KAPPA) ? tabs[i][off + q0] : 0.0;
@@ -976,9 +1016,14 @@ __device__ __forceinline__ void haf_small_phase(const cplx* As, int N, int ninst
       acc = Ar::add(acc, on ? x : Ar::zero());
       if (MODE == HM_KAPPA) a += on ? tabs[i][off + qk] : 0.0;
     }
-    for (int k = KMAX; k < G; ++k) {
-      acc = Ar::add(acc, Ar::zero());
-      if (MODE == HM_KAPPA) a += 0.0;
+    if constexpr (std::is_same<T, double>::value || std::is_same<T, cplx>::value) {
+      // fp64: x + 0 + 0 = x + 0 (the first zero turns -0 into +0, the rest are exact)
+      if (KMAX < G) {
+        acc = Ar::add(acc, Ar::zero());
+        if (MODE == HM_KAPPA) a += 0.0;
+      }
+    } else {
+      for (int k = KMAX; k < G; ++k) acc = Ar::add(acc, Ar::zero());
     }
     parts[i][pbase + slot] = acc;
     if (MODE == HM_KAPPA) pabs[i][pbase + slot] = a;
@@ -986,7 +1031,7 @@ __device__ __forceinline__ void haf_small_phase(const cplx* As, int N, int ninst
 }
 
 template <class Ar, bool LOOP, int MODE, int PAIRS, int... Ss>
-__device__ __forceinline__ void haf_small_phases(const cplx* As, int N, int ninst, void* scratch,
+__device__ __forceinline__ void haf_small_phases(const typename Ar::E* As, int N, int ninst, void* scratch,
                                                  typename Ar::T (*terms)[haf_small_terms(PAIRS)],
                                                  double (*tabs)[haf_small_terms(PAIRS)],
                                                  typename Ar::T (*parts)[HAF_SMALL_MAX_PARTS],
@@ -999,38 +1044,22 @@ __device__ __forceinline__ void haf_small_phases(const cplx* As, int N, int nins
    ...);
 }
 
-// NMAX: the size class (4, 8 or 12) the LDS arrays and phases are built for;
-// a smaller class takes less LDS and fewer registers (more waves per CU)
-template <class Ar, bool LOOP, int MODE, int NMAX>
-__global__ __launch_bounds__(64) void haf_small_kernel(const HafSmallJobs jobs) {
+// Everything after staging: the phases, the butterfly and the stores, in the
+// arithmetic Ar (ArR64 when the workgroup's matrices are real). The LDS arrays
+// are the kernel's, sized for its own (complex) Ar; a real Ar uses their first
+// halves.
+template <class Ar, bool LOOP, int MODE, int PAIRS>
+__device__ __forceinline__ void haf_small_body(cplx* outp, double* auxp, int ninst, int N,
+                                               const typename Ar::E* As, void* terms_raw,
+                                               double (*tabs)[haf_small_terms(PAIRS)], void* parts_raw,
+                                               double (*pabs)[HAF_SMALL_MAX_PARTS], void* scratch) {
   using T = typename Ar::T;
   constexpr bool KAPPA = (MODE == HM_KAPPA);
-  constexpr int PAIRS = NMAX / 2;
-  constexpr int NN = NMAX * NMAX;
   constexpr int TERMS = haf_small_terms(PAIRS);
-  // the largest phase scratch is the S = 2 one (32 lane groups)
-  constexpr size_t SCR = sizeof(HafScratch<Ar, LOOP, 2, MODE, PAIRS + 1>);
-  __shared__ cplx As[HAF_SMALL_MAX_T * NN];
-  __shared__ T terms[HAF_SMALL_MAX_T][TERMS];
-  __shared__ double tabs[KAPPA ? HAF_SMALL_MAX_T : 1][TERMS];
-  __shared__ T parts[HAF_SMALL_MAX_T][HAF_SMALL_MAX_PARTS];
-  __shared__ double pabs[KAPPA ? HAF_SMALL_MAX_T : 1][HAF_SMALL_MAX_PARTS];
-  __shared__ __align__(16) unsigned char scratch[SCR];
+  T(*terms)[TERMS] = reinterpret_cast<T(*)[TERMS]>(terms_raw);
+  T(*parts)[HAF_SMALL_MAX_PARTS] = reinterpret_cast<T(*)[HAF_SMALL_MAX_PARTS]>(parts_raw);
   const int lane = threadIdx.x;
-  int jb = 0;
-  while (jb + 1 < jobs.n && (int)blockIdx.x >= jobs.wg_begin[jb + 1]) ++jb;
-  const int N = jobs.N[jb], n = N / 2, Tj = jobs.T[jb];
-  const int64_t b0 = (int64_t)((int)blockIdx.x - jobs.wg_begin[jb]) * Tj;
-  const int ninst = (int)min((int64_t)Tj, (int64_t)jobs.B[jb] - b0);
-  const cplx* __restrict__ src = jobs.mats[jb] + b0 * N * N;
-  // stage (the ordinary hafnian zeroes the diagonal, as the phase kernels do)
-  for (int e = lane; e < ninst * N * N; e += 64) {
-    cplx v = src[e];
-    const int rc = e % (N * N);
-    if (!LOOP && (rc / N) == (rc % N)) v = cplx{0.0, 0.0};
-    As[e] = v;
-  }
-  __syncthreads();
+  const int n = N / 2;
   haf_small_phases<Ar, LOOP, MODE, PAIRS>(As, N, ninst, scratch, terms, KAPPA ? tabs : nullptr, parts,
                                           KAPPA ? pabs : nullptr, std::make_integer_sequence<int, PAIRS>{});
   __syncthreads();
@@ -1078,11 +1107,82 @@ __global__ __launch_bounds__(64) void haf_small_kernel(const HafSmallJobs jobs) 
 #pragma unroll
     for (int i = 0; i < HAF_SMALL_MAX_T; ++i) {
       if (i < ninst) {
-        jobs.out[jb][b0 + i] = Ar::to_c(s[i]);
-        if (KAPPA) jobs.aux[jb][b0 + i] = a[i];
+        outp[i] = Ar::to_c(s[i]);
+        if (KAPPA) auxp[i] = a[i];
       }
     }
   }
+}
+
+// NMAX: the size class (4, 8 or 12) the LDS arrays and phases are built for;
+// a smaller class takes less LDS and fewer registers (more waves per CU).
+// fp64 and kappa run a workgroup whose staged imaginary parts are all zero in
+// real arithmetic (ArR64: the same values, a quarter of the products).
+template <class Ar, bool LOOP, int MODE, int NMAX>
+__global__ __launch_bounds__(64) void haf_small_kernel(const HafSmallJobs jobs) {
+  using T = typename Ar::T;
+  constexpr bool KAPPA = (MODE == HM_KAPPA);
+  constexpr bool REAL = std::is_same<Ar, ArF64>::value && (MODE == HM_PLAIN || MODE == HM_KAPPA);
+  constexpr int PAIRS = NMAX / 2;
+  constexpr int NN = NMAX * NMAX;
+  constexpr int TERMS = haf_small_terms(PAIRS);
+  // the largest phase scratch is the S = 2 one (32 lane groups)
+  constexpr size_t SCR = sizeof(HafScratch<Ar, LOOP, 2, MODE, PAIRS + 1>);
+  static_assert(sizeof(HafScratch<ArR64, LOOP, 2, MODE, PAIRS + 1>) <= SCR, "real scratch fits");
+  __shared__ cplx As[HAF_SMALL_MAX_T * NN];
+  __shared__ T terms[HAF_SMALL_MAX_T][TERMS];
+  __shared__ double tabs[KAPPA ? HAF_SMALL_MAX_T : 1][TERMS];
+  __shared__ T parts[HAF_SMALL_MAX_T][HAF_SMALL_MAX_PARTS];
+  __shared__ double pabs[KAPPA ? HAF_SMALL_MAX_T : 1][HAF_SMALL_MAX_PARTS];
+  __shared__ __align__(16) unsigned char scratch[SCR];
+  const int lane = threadIdx.x;
+  int jb = 0;
+  while (jb + 1 < jobs.n && (int)blockIdx.x >= jobs.wg_begin[jb + 1]) ++jb;
+  const int N = jobs.N[jb], Tj = jobs.T[jb];
+  const int64_t b0 = (int64_t)((int)blockIdx.x - jobs.wg_begin[jb]) * Tj;
+  const int ninst = (int)min((int64_t)Tj, (int64_t)jobs.B[jb] - b0);
+  const int ne = ninst * N * N;
+  const cplx* __restrict__ src = jobs.mats[jb] + b0 * N * N;
+  // the output pointers are read from the argument block now, while the
+  // matrices load, not after the phases (the asm makes each value ready here)
+  cplx* const outp = jobs.out[jb] + b0;
+  double* const auxp = KAPPA ? jobs.aux[jb] + b0 : nullptr;
+  asm volatile("" : : "s"(outp), "s"(auxp));
+  // stage through registers (the ordinary hafnian zeroes the diagonal, as the
+  // phase kernels do); a real workgroup stores the real parts only, packed
+  // into the front of the same LDS array
+  constexpr int ITER = (HAF_SMALL_MAX_T * NN + 63) / 64;
+  cplx v[ITER];
+  bool nonreal = false;
+#pragma unroll
+  for (int k = 0; k < ITER; ++k) {
+    const int e = lane + 64 * k;
+    v[k] = cplx{0.0, 0.0};
+    if (e < ne) v[k] = src[e];
+    const int rc = e - __mul24(small_div(e, N * N), N * N);  // e < 2^16, N * N <= 2^10
+    if (!LOOP && rc == __mul24(small_div(rc, N), N + 1)) v[k] = cplx{0.0, 0.0};  // diagonal
+    nonreal |= !(v[k].im == 0.0);
+  }
+  if constexpr (REAL) {
+    if (!__any(nonreal)) {  // wave-uniform
+      double* Asr = reinterpret_cast<double*>(As);
+#pragma unroll
+      for (int k = 0; k < ITER; ++k) {
+        const int e = lane + 64 * k;
+        if (e < ne) Asr[e] = v[k].re;
+      }
+      __syncthreads();
+      haf_small_body<ArR64, LOOP, MODE, PAIRS>(outp, auxp, ninst, N, Asr, terms, tabs, parts, pabs, scratch);
+      return;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < ITER; ++k) {
+    const int e = lane + 64 * k;
+    if (e < ne) As[e] = v[k];
+  }
+  __syncthreads();
+  haf_small_body<Ar, LOOP, MODE, PAIRS>(outp, auxp, ninst, N, As, terms, tabs, parts, pabs, scratch);
 }
 
 // Schedule of the fused small-size kernel (host state, haf.hip; neither knob

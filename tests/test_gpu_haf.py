@@ -232,3 +232,44 @@ def test_fused_small_kernel_equals_per_size_launches(ext, func):
                     assert np.array_equal(a, b), (N, target)
     finally:
         _lib.check(L.gbs_haf_set_schedule(m0.value, t0.value), "set_schedule")
+
+
+@pytest.mark.parametrize("func", ["haf", "lhaf"])
+def test_fused_small_kernel_real_workgroups(ext, func):
+    """A workgroup of the fused kernel whose staged matrices are all real runs
+    in real arithmetic. Its fp64 values and kappa sums equal the per-size
+    launches' (complex arithmetic) bit for bit, imaginary parts +0, at every
+    side and packing, in a batch where one instance is complex (its workgroup
+    keeps complex arithmetic); -0.0 imaginary parts count as real (values equal)."""
+    import ctypes
+    from gbskernels_amd import _lib
+    L = _lib.lib()
+    m0, t0 = ctypes.c_int(), ctypes.c_int()
+    _lib.check(L.gbs_haf_get_schedule(ctypes.byref(m0), ctypes.byref(t0)), "get_schedule")
+
+    def bits(x):
+        return np.ascontiguousarray(x).view(np.uint64)
+
+    try:
+        for N in (2, 4, 6, 8, 10, 12):
+            g = np.random.default_rng(900 + N)
+            R = g.standard_normal((41, N, N))
+            A = (R + R.transpose(0, 2, 1)).astype(np.complex128)
+            C = g.standard_normal((N, N))
+            A[17] += 1e-3j * (C + C.T)
+            Aneg = np.zeros((8, N, N), np.complex128)
+            Aneg.real = A[:8].real
+            Aneg.imag = -0.0
+            _lib.check(L.gbs_haf_set_schedule(0, t0.value), "set_schedule")
+            ref = [getattr(ext, func)(A), *getattr(ext, f"{func}_kappa")(A)]
+            ref_neg = getattr(ext, func)(Aneg)
+            for target in (1, 12, 18, 2048):  # 4, 3, 2, 1 instances per workgroup
+                _lib.check(L.gbs_haf_set_schedule(12, target), "set_schedule")
+                got = [getattr(ext, func)(A), *getattr(ext, f"{func}_kappa")(A)]
+                for a, b in zip(ref, got):
+                    assert np.array_equal(bits(a), bits(b)), (N, target)
+                real = np.arange(41) != 17
+                assert np.all(got[0][real].imag == 0)
+                assert np.array_equal(getattr(ext, func)(Aneg), ref_neg), (N, target)
+    finally:
+        _lib.check(L.gbs_haf_set_schedule(m0.value, t0.value), "set_schedule")
