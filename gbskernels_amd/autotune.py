@@ -270,7 +270,7 @@ def tune_haf(L, torch, reps: int, margin: float, write) -> None:
     """Fused small-size kernel against the per-subset-size launches for each
     side N <= 12 (4096 instances, fp64): small_max_n = the largest side up to
     which the fused kernel is never slower than (1 + margin) x the other path;
-    then the workgroup target at N = 8."""
+    then the workgroup target at N = 8 (0: the built-in per-bucket cost model)."""
     import ctypes
     from .workloads import gbs_adjacency, haf_subset
     m0, t0 = ctypes.c_int(), ctypes.c_int()
@@ -298,7 +298,7 @@ def tune_haf(L, torch, reps: int, margin: float, write) -> None:
         best = None
         print("| workgroup target | N = 8 fused ms |")
         print("|---|---|")
-        for tgt in (256, 512, 1024, 2048, 4096):
+        for tgt in (0, 256, 512, 1024, 2048, 4096):
             L.gbs_haf_set_schedule(12, tgt)
             t = _median_ms(torch, stream, go, reps)
             print(f"| {tgt} | {t:.4f} |", flush=True)

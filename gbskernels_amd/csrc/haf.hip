@@ -6,7 +6,7 @@ namespace gbsk {
 
 // built-in schedule of the fused small-size kernel (measured on MI355X)
 int haf_small_max_n = 12;
-int haf_small_target_wgs = 1024;
+int haf_small_target_wgs = 0;  // 0: per-bucket packing from the cost model
 
 int haf_schedule_get(int* small_max_n, int* small_target_wgs) {
   if (small_max_n) *small_max_n = haf_small_max_n;
@@ -15,7 +15,7 @@ int haf_schedule_get(int* small_max_n, int* small_target_wgs) {
 }
 
 int haf_schedule_set(int small_max_n, int small_target_wgs) {
-  if (small_max_n < 0 || small_max_n > HAF_SMALL_MAX_N || small_target_wgs < 1) return GBSK_ERR_ARG;
+  if (small_max_n < 0 || small_max_n > HAF_SMALL_MAX_N || small_target_wgs < 0) return GBSK_ERR_ARG;
   haf_small_max_n = small_max_n;
   haf_small_target_wgs = small_target_wgs;
   return 0;

@@ -1186,20 +1186,75 @@ __global__ __launch_bounds__(64) void haf_small_kernel(const HafSmallJobs jobs) 
 }
 
 // Schedule of the fused small-size kernel (host state, haf.hip; neither knob
-// changes a value): it runs sizes N <= haf_small_max_n, with as many instances
-// per workgroup (at most HAF_SMALL_MAX_T) as keep at least haf_small_target_wgs
-// workgroups in the launch.
+// changes a value): it runs sizes N <= haf_small_max_n. A positive
+// haf_small_target_wgs packs as many instances per workgroup (at most
+// HAF_SMALL_MAX_T) as keep at least that many workgroups in the launch; 0 (the
+// built-in) picks the packing T per bucket from a cost model fitted on MI355X:
+// a wave of T instances makes haf_small_rounds(N, T) subset rounds; each
+// further wave sharing a SIMD adds 0.38 of its rounds (one wave leaves issue
+// slots idle that a second one partly fills; from N = 8, B = 4096: two waves
+// of 4 rounds took 0.92 x one wave of 6). A smaller T competes only if all its
+// workgroups fit on the device at once: the workgroups take equal time, so
+// the ones left over would run as a whole second pass (N = 12, B = 4096 at
+// T = 2: 2048 workgroups where 1792 fit, 1.25x slower than T = 4).
 extern int haf_small_max_n;
 extern int haf_small_target_wgs;
-inline int haf_small_T(int B) {
-  const int t = B / (haf_small_target_wgs > 0 ? haf_small_target_wgs : 1);
-  return t < 1 ? 1 : (t > HAF_SMALL_MAX_T ? HAF_SMALL_MAX_T : t);
+inline int haf_small_rounds(int N, int T) {
+  const int n = N / 2;
+  int r = 0;
+  for (int m = 1; m <= n; ++m) {
+    const int G = 64 / (2 * m);
+    r += (T * haf_binom(n, m) + G - 1) / G;
+  }
+  return r;
+}
+// cus: compute units; per_cu: workgroups (one wave each) of the kernel a CU holds
+inline int haf_small_T(int B, int N, int cus, int per_cu) {
+  if (haf_small_target_wgs > 0) {
+    const int t = B / haf_small_target_wgs;
+    return t < 1 ? 1 : (t > HAF_SMALL_MAX_T ? HAF_SMALL_MAX_T : t);
+  }
+  int best = HAF_SMALL_MAX_T;
+  double best_cost = 0.0;
+  for (int T = HAF_SMALL_MAX_T; T >= 1; --T) {  // ties keep the larger T
+    const int wg = (B + T - 1) / T;
+    if (T < HAF_SMALL_MAX_T && wg > cus * per_cu) break;  // a second pass would double the time
+    const double waves = ceil((double)wg / (4.0 * cus));  // per SIMD
+    const double cost = haf_small_rounds(N, T) * (1.0 + 0.38 * (waves - 1.0));
+    if (T == HAF_SMALL_MAX_T || cost < best_cost) {
+      best = T;
+      best_cost = cost;
+    }
+  }
+  return best;
+}
+// compute units of the current device and how many workgroups of `kernel`
+// (64 threads, static LDS) one of them holds; queried once per kernel (one
+// device model per process)
+inline void haf_small_occupancy(const void* kernel, int* cus, int* per_cu) {
+  int dev = 0, c = 0, nb = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 64, 0) != hipSuccess || c < 1 || nb < 1) {
+    (void)hipGetLastError();
+    c = 256;
+    nb = 1;
+  }
+  *cus = c;
+  *per_cu = nb;
 }
 
 template <class Ar, bool LOOP, int MODE>
 static void launch_small(const HafJob* jobs, int njobs, hipStream_t st) {
   // one launch per size class present (N <= 4, <= 8, <= 12)
+  struct Occ {
+    int cus, per_cu;
+  };
+  static const Occ occ[3] = {
+      [] { Occ o; haf_small_occupancy((const void*)haf_small_kernel<Ar, LOOP, MODE, 4>, &o.cus, &o.per_cu); return o; }(),
+      [] { Occ o; haf_small_occupancy((const void*)haf_small_kernel<Ar, LOOP, MODE, 8>, &o.cus, &o.per_cu); return o; }(),
+      [] { Occ o; haf_small_occupancy((const void*)haf_small_kernel<Ar, LOOP, MODE, 12>, &o.cus, &o.per_cu); return o; }()};
   for (int cls : {4, 8, 12}) {
+    const Occ& oc = occ[cls / 4 - 1];
     HafSmallJobs sj{};
     int wg = 0;
     for (int j = 0; j < njobs; ++j) {
@@ -1211,7 +1266,7 @@ static void launch_small(const HafJob* jobs, int njobs, hipStream_t st) {
       sj.aux[k] = jobs[j].aux;
       sj.N[k] = N;
       sj.B[k] = jobs[j].B;
-      sj.T[k] = haf_small_T(jobs[j].B);
+      sj.T[k] = haf_small_T(jobs[j].B, N, oc.cus, oc.per_cu);
       sj.wg_begin[k] = wg;
       wg += (jobs[j].B + sj.T[k] - 1) / sj.T[k];
     }
