@@ -56,7 +56,7 @@ EXECUTED_F64 = {
     "haf_fp64_n24": (5.18053e+08, 4206890),
     "lhaf_fp64_n16": (9162176, 78319),
     "lhaf_fp64_n24": (5.749229e+08, 4794570),
-    "haf_fp64_config1_n8": (57952, 511.5),
+    "haf_fp64_config1_n8": (19136, 184.0),
     "tor_fp64_nm8": (42880, 443),
     "tor_fp64_nm12": (409600, 4145),
     "tor_fp64_nm16": (6921728, 69994),
