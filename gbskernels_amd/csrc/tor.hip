@@ -627,10 +627,24 @@ constexpr int tor_bfs_split() {
   return std::is_same<Ar, RCDD>::value ? 0 : (std::is_same<Ar, RCERT>::value ? 1 : TOR_BFS_Q);
 }
 
+// Results a whole-wave launch of the fp64 tier writes itself when an instance
+// has one partial (T1 = 0), in place of the reduce or merge launch, whose
+// arithmetic on one partial is 0 + value (the other lanes' and the
+// butterfly's zeros change no bit after it). out == nullptr: partials only.
+//   real kernel: out / aux (kappa) and the complex rerun's marks cpart[npc]
+//   rerun kernel: out / aux (kappa: the complex sum |term|, cert_inf: +inf)
+struct TorDirect {
+  cplx* out = nullptr;
+  double* aux = nullptr;
+  cplx* cpart = nullptr;
+  int npc = 0;
+  bool cert_inf = false;
+};
+
 template <class Ar, int R, int T2, bool KAPPA, bool RERUN, bool BFS>
 __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int nm, int T1, int B,
                                               typename Ar::P* __restrict__ part, double* __restrict__ part_abs,
-                                              int64_t wg) {
+                                              int64_t wg, const TorDirect& dir) {
   using T = typename Ar::T;
   extern __shared__ double lds_raw[];
   T* lds = reinterpret_cast<T*>(lds_raw);  // split: T1 (argument) + T2 + R = nm
@@ -1029,7 +1043,21 @@ __device__ __forceinline__ void tor_workgroup(const cplx* __restrict__ mats, int
   __syncthreads();
   if (lane == 0) {
     const size_t np = (size_t)1 << T1;
-    part[b * np + p1] = bad_flag ? Ar::part_nan() : Ar::part(acc);
+    const typename Ar::P pv = bad_flag ? Ar::part_nan() : Ar::part(acc);
+    if constexpr (std::is_same<typename Ar::P, cplx>::value && !Ar::CERT) {
+      if (dir.out != nullptr) {  // T1 = 0: b is the slot
+        const cplx s{0.0 + pv.re, 0.0 + pv.im};
+        dir.out[b] = s;
+        if (KAPPA) dir.aux[b] = 0.0 + (bad_flag ? NAN : aacc);
+        if (!RERUN && dir.cpart != nullptr) {
+          const cplx m = (s.re != s.re) ? cplx{NAN, NAN} : cplx{0.0, 0.0};
+          for (int i = 0; i < dir.npc; ++i) dir.cpart[b * dir.npc + i] = m;
+        }
+        if (RERUN && dir.cert_inf) dir.aux[b] = INFINITY;
+        return;
+      }
+    }
+    part[b * np + p1] = pv;
     if constexpr (Ar::CERT) {
       part_abs[b * np + p1] = bad_flag ? INFINITY : Ar::bound(acc);
     } else if (KAPPA) {
@@ -1050,7 +1078,7 @@ constexpr int64_t TOR_RERUN_GRID = 8192;
 template <class Ar, int R, int T2, bool KAPPA, bool RERUN, bool BFS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(tor_min_waves<Ar>()))) void tor_real_kernel(
     const cplx* __restrict__ mats, int nm, int T1, int B, typename Ar::P* __restrict__ part,
-    double* __restrict__ part_abs) {
+    double* __restrict__ part_abs, const TorDirect dir) {
   if constexpr (RERUN) {
     const int64_t total = (int64_t)B << T1;
     const int64_t per = (total + gridDim.x - 1) / gridDim.x;
@@ -1067,11 +1095,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(tor_min_wave
         const int k = __builtin_ctzll(todo);
         todo &= todo - 1;
         __syncthreads();  // the previous slot's LDS use is over
-        tor_workgroup<Ar, R, T2, KAPPA, RERUN, BFS>(mats, nm, T1, B, part, part_abs, base + k);
+        tor_workgroup<Ar, R, T2, KAPPA, RERUN, BFS>(mats, nm, T1, B, part, part_abs, base + k, dir);
       }
     }
   } else {
-    tor_workgroup<Ar, R, T2, KAPPA, RERUN, BFS>(mats, nm, T1, B, part, part_abs, blockIdx.x);
+    tor_workgroup<Ar, R, T2, KAPPA, RERUN, BFS>(mats, nm, T1, B, part, part_abs, blockIdx.x, dir);
   }
 }
 
@@ -1490,37 +1518,39 @@ static size_t tor_lds_bytes(int nm, const TorSplit& sp, bool bfs) {
 
 template <class Ar, int R, int T2, bool KAPPA, bool RERUN, bool BFS>
 static void launch_tor_rt1(int nm, int T1, const cplx* d_in, int B, typename Ar::P* part, double* part_abs,
-                           dim3 grid, size_t lds, hipStream_t st) {
+                           dim3 grid, size_t lds, hipStream_t st, const TorDirect& dir) {
   if (lds > 65536)  // dim 64 at the wider tiers: up to gfx950's 160 KiB per workgroup
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&tor_real_kernel<Ar, R, T2, KAPPA, RERUN, BFS>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL((tor_real_kernel<Ar, R, T2, KAPPA, RERUN, BFS>), grid, dim3(64), lds, st, d_in, nm, T1, B, part,
-                     part_abs);
+                     part_abs, dir);
 }
 
 // the whole-wave kernel, with the breadth-first middle walk when `bfs` (the
 // register-task policies only; the depth-first walk otherwise)
 template <class Ar, int R, int T2, bool KAPPA, bool RERUN>
 static void launch_tor_rt(int nm, int T1, const cplx* d_in, int B, typename Ar::P* part, double* part_abs, dim3 grid,
-                          bool bfs, hipStream_t st) {
+                          bool bfs, hipStream_t st, const TorDirect& dir) {
   const TorSplit sp{R, T2, T1};
   if constexpr (Ar::TASK_REGS && Ar::SYM && T2 >= 1) {
     if (bfs) {
       launch_tor_rt1<Ar, R, T2, KAPPA, RERUN, true>(nm, T1, d_in, B, part, part_abs, grid,
-                                                    tor_lds_bytes<Ar>(nm, sp, true), st);
+                                                    tor_lds_bytes<Ar>(nm, sp, true), st, dir);
       return;
     }
   }
   launch_tor_rt1<Ar, R, T2, KAPPA, RERUN, false>(nm, T1, d_in, B, part, part_abs, grid,
-                                                 tor_lds_bytes<Ar>(nm, sp, false), st);
+                                                 tor_lds_bytes<Ar>(nm, sp, false), st, dir);
 }
 
 
 // the split's R and T2 are template arguments: T2 = 6 with R = 0..RMAX
 // (nm >= 6), or T2 = nm < 6 with R = 0
 template <class Ar, bool KAPPA, bool RERUN>
+// dir: see TorDirect; honoured only by the whole-wave kernel (callers pass it
+// when sp.G == 1 and sp.T1 == 0)
 static int launch_tor(int nm, const TorSplit& sp, const cplx* d_in, int B, typename Ar::P* part, double* part_abs,
-                      hipStream_t st) {
+                      hipStream_t st, const TorDirect& dir = TorDirect{}) {
   if constexpr (!RERUN && Ar::TASK_REGS && Ar::SYM) {
     if (sp.G > 1) return launch_tor_small<Ar, KAPPA>(nm, d_in, B, part, part_abs, st);
   }
@@ -1532,7 +1562,7 @@ static int launch_tor(int nm, const TorSplit& sp, const cplx* d_in, int B, typen
     switch (sp.R) {
 #define TOR_CASE(RR)                                                                                             \
   case RR:                                                                                                       \
-    if constexpr (RR <= Ar::RMAX) launch_tor_rt<Ar, RR, TOR_T2, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st); \
+    if constexpr (RR <= Ar::RMAX) launch_tor_rt<Ar, RR, TOR_T2, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st, dir); \
     break;
       TOR_CASE(0)
       TOR_CASE(1)
@@ -1547,11 +1577,11 @@ static int launch_tor(int nm, const TorSplit& sp, const cplx* d_in, int B, typen
   }
   if (sp.R != 0) return GBSK_ERR_DIM;
   switch (sp.T2) {
-    case 1: launch_tor_rt<Ar, 0, 1, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st); break;
-    case 2: launch_tor_rt<Ar, 0, 2, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st); break;
-    case 3: launch_tor_rt<Ar, 0, 3, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st); break;
-    case 4: launch_tor_rt<Ar, 0, 4, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st); break;
-    case 5: launch_tor_rt<Ar, 0, 5, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st); break;
+    case 1: launch_tor_rt<Ar, 0, 1, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st, dir); break;
+    case 2: launch_tor_rt<Ar, 0, 2, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st, dir); break;
+    case 3: launch_tor_rt<Ar, 0, 3, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st, dir); break;
+    case 4: launch_tor_rt<Ar, 0, 4, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st, dir); break;
+    case 5: launch_tor_rt<Ar, 0, 5, KAPPA, RERUN>(nm, sp.T1, d_in, B, part, part_abs, grid, bfs, st, dir); break;
     default: return GBSK_ERR_DIM;
   }
   return 0;
@@ -1577,21 +1607,29 @@ static int run_tor_fp64(int nm, const cplx* d_in, int B, cplx* d_out, double* d_
   // and the complex kernel recomputes whole instances whose sum is NaN
   const int npr = 1 << sr.T1, npc = 1 << sc.T1;
   double* part_abs = reinterpret_cast<double*>(part + (size_t)B * npr);
-  int rc = launch_tor<ArR, KAPPA, false>(nm, sr, d_in, B, part, part_abs, st);
-  if (rc) return rc;
-  // the reduce also marks the partials the complex kernel recomputes
   cplx* cpart = reinterpret_cast<cplx*>(part_abs + (size_t)B * npr);
   double* cpart_abs = reinterpret_cast<double*>(cpart + (size_t)B * npc);
+  // one partial per instance: the whole-wave kernels write the reduce's and
+  // the merge's results themselves (two launches fewer at <= 8 modes)
+  const bool direct_real = !CERT && npr == 1 && sr.G == 1;
+  const bool direct_rerun = npc == 1;  // the rerun is always the whole-wave kernel
+  TorDirect dr, dc;
+  if (direct_real) dr = TorDirect{d_out, d_aux, cpart, npc, false};
+  if (direct_rerun) dc = TorDirect{d_out, d_aux, nullptr, 0, CERT};
+  int rc = launch_tor<ArR, KAPPA, false>(nm, sr, d_in, B, part, part_abs, st, dr);
+  if (rc) return rc;
+  // the reduce also marks the partials the complex kernel recomputes
   if constexpr (CERT)
     hipLaunchKernelGGL(tor_reduce_cert_kernel, dim3(tor_red_grid(B)), dim3(256), 0, st, part, part_abs, npr, B,
                        tor_cert_depth(nm, npr), d_out, d_aux, cpart, npc);
-  else
+  else if (!direct_real)
     hipLaunchKernelGGL((tor_reduce_kernel<false, KAPPA>), dim3(tor_red_grid(B)), dim3(256), 0, st, part, part_abs,
                        npr, B, d_out, d_aux, cpart, npc);
-  rc = launch_tor<RC64, KAPPA, true>(nm, sc, d_in, B, cpart, cpart_abs, st);
+  rc = launch_tor<RC64, KAPPA, true>(nm, sc, d_in, B, cpart, cpart_abs, st, dc);
   if (rc) return rc;
-  hipLaunchKernelGGL((tor_merge_kernel<KAPPA, CERT>), dim3(tor_red_grid(B)), dim3(256), 0, st, cpart, cpart_abs, npc,
-                     B, d_out, d_aux);
+  if (!direct_rerun)
+    hipLaunchKernelGGL((tor_merge_kernel<KAPPA, CERT>), dim3(tor_red_grid(B)), dim3(256), 0, st, cpart, cpart_abs,
+                       npc, B, d_out, d_aux);
   return (int)hipGetLastError();
 }
 
