@@ -57,10 +57,10 @@ EXECUTED_F64 = {
     "lhaf_fp64_n16": (9162176, 78319),
     "lhaf_fp64_n24": (5.749229e+08, 4794570),
     "haf_fp64_config1_n8": (19136, 184.0),
-    "tor_fp64_nm8": (42880, 443),
-    "tor_fp64_nm12": (409600, 4145),
-    "tor_fp64_nm16": (6921728, 69994),
-    "tor_fp64_nm20": (1.183101e+08, 1197207),
+    "tor_fp64_nm8": (60288, 611),
+    "tor_fp64_nm12": (505728, 5099),
+    "tor_fp64_nm16": (8459776, 85258),
+    "tor_fp64_nm20": (1.429188e+08, 1441431),
 }
 # FP64 VALU wave instructions per second at the 78.6 TFLOP/s peak: one per CU
 # per clock (256 CUs, 2.4 GHz; an FMA over 64 lanes is 128 flop)
