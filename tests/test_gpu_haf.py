@@ -225,7 +225,7 @@ def test_fused_small_kernel_equals_per_size_launches(ext, func):
             A = rng_symmetric(37, N, 700 + N)
             _lib.check(L.gbs_haf_set_schedule(0, t0.value), "set_schedule")
             ref = [getattr(ext, func)(A), *getattr(ext, f"{func}_kappa")(A), getattr(ext, f"{func}_dd")(A)]
-            for target in (1, 12, 18, 2048):  # 4, 3, 2, 1 instances per workgroup
+            for target in (0, 1, 12, 18, 2048):  # the cost model; 4, 3, 2, 1 instances per workgroup
                 _lib.check(L.gbs_haf_set_schedule(12, target), "set_schedule")
                 got = [getattr(ext, func)(A), *getattr(ext, f"{func}_kappa")(A), getattr(ext, f"{func}_dd")(A)]
                 for a, b in zip(ref, got):
@@ -263,7 +263,7 @@ def test_fused_small_kernel_real_workgroups(ext, func):
             _lib.check(L.gbs_haf_set_schedule(0, t0.value), "set_schedule")
             ref = [getattr(ext, func)(A), *getattr(ext, f"{func}_kappa")(A)]
             ref_neg = getattr(ext, func)(Aneg)
-            for target in (1, 12, 18, 2048):  # 4, 3, 2, 1 instances per workgroup
+            for target in (0, 1, 12, 18, 2048):  # the cost model; 4, 3, 2, 1 instances per workgroup
                 _lib.check(L.gbs_haf_set_schedule(12, target), "set_schedule")
                 got = [getattr(ext, func)(A), *getattr(ext, f"{func}_kappa")(A)]
                 for a, b in zip(ref, got):
