@@ -927,7 +927,7 @@ __global__ __launch_bounds__(64 * HAF_REDUCE_WAVES) void haf_reduce_kernel(const
 }
 
 // ------------------------------------------- small sizes: one fused launch
-// For N <= HAF_SMALL_MAX_N (fp64 / kappa / dd) one launch replaces the n phase
+// For N <= HAF_SMALL_MAX_N (fp64 / kappa / certified fp64 / dd) one launch replaces the n phase
 // launches and the reduce. A workgroup (one wave) takes `T` consecutive
 // instances of a bucket: it stages their matrices in LDS, evaluates every subset
 // of every size with haf_subset_term (lane groups of S lanes, 64 / S groups per
@@ -935,8 +935,10 @@ __global__ __launch_bounds__(64 * HAF_REDUCE_WAVES) void haf_reduce_kernel(const
 // on its own matrix), keeps the terms in LDS, forms each chunk partial in the
 // phase kernel's order (the chunk's groups summed in group order, idle groups
 // contributing zero) and sums the partials with the reduce kernel's butterfly.
-// Values and kappa sums are therefore bit-identical to the phase path, and the
-// number of instances per workgroup only changes the schedule.
+// Values, kappa sums and certified bounds (each bound operation of the phase
+// and reduce kernels replayed in the same order) are therefore bit-identical
+// to the phase path, and the number of instances per workgroup only changes
+// the schedule.
 constexpr int HAF_SMALL_MAX_N = 12;
 constexpr int HAF_SMALL_MAX_PAIRS = HAF_SMALL_MAX_N / 2;
 constexpr int HAF_SMALL_MAX_T = 4;  // instances per workgroup
@@ -988,8 +990,12 @@ __device__ __forceinline__ void haf_small_phase(const typename Ar::E* As, int N,
     haf_subset_term<Ar, LOOP, S, MODE>(As, __mul24(inst, N * N), N, q, sh, coeff, ecoeff);
     (void)ecoeff;
     if (active && r == 0) {
-      terms[inst][off + q] = Ar::add(Ar::zero(), coeff);
+      const T t0 = Ar::add(Ar::zero(), coeff);
+      terms[inst][off + q] = t0;
       if (MODE == HM_KAPPA) tabs[inst][off + q] = 0.0 + c_abs(Ar::to_c(coeff));
+      // certified: the group's bound after its one subset, as the phase kernel
+      // forms it (cert_round of the group sum, from 0 + the term's bound)
+      if (MODE == HM_CERT) tabs[inst][off + q] = cert_round(Ar::to_c(t0), 0.0 + ecoeff);
     }
   }
   __syncthreads();
@@ -1027,6 +1033,23 @@ __device__ __forceinline__ void haf_small_phase(const typename Ar::E* As, int N,
     }
     parts[i][pbase + slot] = acc;
     if (MODE == HM_KAPPA) pabs[i][pbase + slot] = a;
+    if constexpr (MODE == HM_CERT) {
+      // the chunk bound of the phase kernel, operation for operation: every
+      // one of the G groups in group order (an idle group adds a zero sum with
+      // a zero bound, and cert_round still charges its underflow slack), then
+      // the chunk's finishing inflation
+      T s = ti[q0];
+      double E = tabs[i][off + q0];
+      for (int k = 1; k < G; ++k) {
+        const bool on = q0 + k < count;
+        const int qk = on ? q0 + k : q0;
+        const T x = ti[qk];
+        const double ex = tabs[i][off + qk];
+        s = Ar::add(s, on ? x : Ar::zero());
+        E = cert_round(Ar::to_c(s), E + (on ? ex : 0.0));
+      }
+      pabs[i][pbase + slot] = cert_finish(E, haf_cert_depth(S, n, count));
+    }
   }
 }
 
@@ -1060,8 +1083,9 @@ __device__ __forceinline__ void haf_small_body(cplx* outp, double* auxp, int nin
   T(*parts)[HAF_SMALL_MAX_PARTS] = reinterpret_cast<T(*)[HAF_SMALL_MAX_PARTS]>(parts_raw);
   const int lane = threadIdx.x;
   const int n = N / 2;
-  haf_small_phases<Ar, LOOP, MODE, PAIRS>(As, N, ninst, scratch, terms, KAPPA ? tabs : nullptr, parts,
-                                          KAPPA ? pabs : nullptr, std::make_integer_sequence<int, PAIRS>{});
+  constexpr bool CERT = (MODE == HM_CERT);
+  haf_small_phases<Ar, LOOP, MODE, PAIRS>(As, N, ninst, scratch, terms, (KAPPA || CERT) ? tabs : nullptr, parts,
+                                          (KAPPA || CERT) ? pabs : nullptr, std::make_integer_sequence<int, PAIRS>{});
   __syncthreads();
   // the reduce kernel's butterfly over each instance's partials, for lane 0,
   // whose result is stored: with at most 8 partials (lanes 0..7, exact zeros
@@ -1103,12 +1127,36 @@ __device__ __forceinline__ void haf_small_body(cplx* outp, double* auxp, int nin
       if (KAPPA) a[i] += 0.0;
     }
   }
+  // certified: the reduce kernel's bound, literally: all 64 lanes (lanes at
+  // or past the partial count hold a zero sum and a zero bound) through the
+  // six xor-butterfly levels, each a cert_round of the new sum, then its
+  // finishing inflation
+  double eb[CERT ? HAF_SMALL_MAX_T : 1];
+  if constexpr (CERT) {
+#pragma unroll
+    for (int i = 0; i < HAF_SMALL_MAX_T; ++i) {
+      const bool on = i < ninst && lane < np;
+      const T x = parts[i][on ? lane : 0];
+      const double ex = pabs[i][on ? lane : 0];
+      T sv = on ? x : Ar::zero();
+      double av = on ? ex : 0.0;
+#pragma unroll
+      for (int k = 1; k < 64; k <<= 1) {
+        const T o = Ar::shfl_xor(sv, k);
+        const double ao = shfl_xor_d(av, k);
+        sv = Ar::add(sv, o);
+        av = cert_round(Ar::to_c(sv), av + ao);
+      }
+      eb[i] = cert_finish(av, 4.0 * (np + 6) + 8.0);
+    }
+  }
   if (lane == 0) {
 #pragma unroll
     for (int i = 0; i < HAF_SMALL_MAX_T; ++i) {
       if (i < ninst) {
         outp[i] = Ar::to_c(s[i]);
         if (KAPPA) auxp[i] = a[i];
+        if (CERT) auxp[i] = eb[i];
       }
     }
   }
@@ -1131,9 +1179,10 @@ __global__ __launch_bounds__(64) void haf_small_kernel(const HafSmallJobs jobs) 
   static_assert(sizeof(HafScratch<ArR64, LOOP, 2, MODE, PAIRS + 1>) <= SCR, "real scratch fits");
   __shared__ cplx As[HAF_SMALL_MAX_T * NN];
   __shared__ T terms[HAF_SMALL_MAX_T][TERMS];
-  __shared__ double tabs[KAPPA ? HAF_SMALL_MAX_T : 1][TERMS];
+  constexpr bool AUX = KAPPA || MODE == HM_CERT;  // per-term magnitudes or bounds
+  __shared__ double tabs[AUX ? HAF_SMALL_MAX_T : 1][TERMS];
   __shared__ T parts[HAF_SMALL_MAX_T][HAF_SMALL_MAX_PARTS];
-  __shared__ double pabs[KAPPA ? HAF_SMALL_MAX_T : 1][HAF_SMALL_MAX_PARTS];
+  __shared__ double pabs[AUX ? HAF_SMALL_MAX_T : 1][HAF_SMALL_MAX_PARTS];
   __shared__ __align__(16) unsigned char scratch[SCR];
   const int lane = threadIdx.x;
   int jb = 0;
@@ -1146,7 +1195,7 @@ __global__ __launch_bounds__(64) void haf_small_kernel(const HafSmallJobs jobs) 
   // the output pointers are read from the argument block now, while the
   // matrices load, not after the phases (the asm makes each value ready here)
   cplx* const outp = jobs.out[jb] + b0;
-  double* const auxp = KAPPA ? jobs.aux[jb] + b0 : nullptr;
+  double* const auxp = AUX ? jobs.aux[jb] + b0 : nullptr;
   asm volatile("" : : "s"(outp), "s"(auxp));
   // stage through registers (the ordinary hafnian zeroes the diagonal, as the
   // phase kernels do); a real workgroup stores the real parts only, packed
@@ -1316,10 +1365,10 @@ static void launch_phases(const HafJob* jobs, int njobs, typename Ar::T* const* 
 template <class Ar, bool LOOP, int MODE>
 int run_haf_multi(const HafJob* all, int nall, hipStream_t st) {
   using T = typename Ar::T;
-  // small sizes of the plain and kappa modes: one fused launch
+  // small sizes of the plain, kappa and certified fp64 modes: one fused launch
   HafJob jobs[HAF_MAX_JOBS];
   int njobs = 0;
-  if constexpr (MODE == HM_PLAIN || MODE == HM_KAPPA) {
+  if constexpr (MODE == HM_PLAIN || MODE == HM_KAPPA || (MODE == HM_CERT && std::is_same<Ar, ArF64>::value)) {
     HafJob small[HAF_MAX_JOBS];
     int ns = 0;
     const int small_n = haf_small_max_n < HAF_SMALL_MAX_N ? haf_small_max_n : HAF_SMALL_MAX_N;

@@ -213,8 +213,8 @@ def test_dd_certified_cancellation(ext, oracle):
 def test_fused_small_kernel_equals_per_size_launches(ext, func):
     """The fused small-size kernel (all subset sizes and the final sum in one
     launch, several instances per workgroup) returns the per-subset-size
-    launches' values and kappa sums bit for bit, at every side it covers, for
-    every workgroup packing."""
+    launches' values, kappa sums and certified bounds bit for bit, at every
+    side it covers, for every workgroup packing."""
     import ctypes
     from gbskernels_amd import _lib
     L = _lib.lib()
@@ -224,12 +224,14 @@ def test_fused_small_kernel_equals_per_size_launches(ext, func):
         for N in (2, 4, 6, 8, 10, 12):
             A = rng_symmetric(37, N, 700 + N)
             _lib.check(L.gbs_haf_set_schedule(0, t0.value), "set_schedule")
-            ref = [getattr(ext, func)(A), *getattr(ext, f"{func}_kappa")(A), getattr(ext, f"{func}_dd")(A)]
+            ref = [getattr(ext, func)(A), *getattr(ext, f"{func}_kappa")(A), getattr(ext, f"{func}_dd")(A),
+                   *getattr(ext, f"{func}_certified")(A)]
             for target in (0, 1, 12, 18, 2048):  # the cost model; 4, 3, 2, 1 instances per workgroup
                 _lib.check(L.gbs_haf_set_schedule(12, target), "set_schedule")
-                got = [getattr(ext, func)(A), *getattr(ext, f"{func}_kappa")(A), getattr(ext, f"{func}_dd")(A)]
-                for a, b in zip(ref, got):
-                    assert np.array_equal(a, b), (N, target)
+                got = [getattr(ext, func)(A), *getattr(ext, f"{func}_kappa")(A), getattr(ext, f"{func}_dd")(A),
+                       *getattr(ext, f"{func}_certified")(A)]
+                for k, (a, b) in enumerate(zip(ref, got)):
+                    assert np.array_equal(a, b), (N, target, k)
     finally:
         _lib.check(L.gbs_haf_set_schedule(m0.value, t0.value), "set_schedule")
 
