@@ -927,7 +927,7 @@ __global__ __launch_bounds__(64 * HAF_REDUCE_WAVES) void haf_reduce_kernel(const
 }
 
 // ------------------------------------------- small sizes: one fused launch
-// For N <= HAF_SMALL_MAX_N (fp64 / kappa / certified fp64 / dd) one launch replaces the n phase
+// For N <= HAF_SMALL_MAX_N (fp64 / kappa / certified fp64 / dd / certified dd) one launch replaces the n phase
 // launches and the reduce. A workgroup (one wave) takes `T` consecutive
 // instances of a bucket: it stages their matrices in LDS, evaluates every subset
 // of every size with haf_subset_term (lane groups of S lanes, 64 / S groups per
@@ -942,6 +942,10 @@ __global__ __launch_bounds__(64 * HAF_REDUCE_WAVES) void haf_reduce_kernel(const
 constexpr int HAF_SMALL_MAX_N = 12;
 constexpr int HAF_SMALL_MAX_PAIRS = HAF_SMALL_MAX_N / 2;
 constexpr int HAF_SMALL_MAX_T = 4;  // instances per workgroup
+// certified dd takes the fused kernel up to N = 8 only: measured at B = 4096,
+// N = 8 ran 535 -> 190 us fused, but the N <= 12 class (every phase's dd
+// bound registers in one kernel) made N = 12 1.3x slower than the phases
+constexpr int HAF_SMALL_DDCERT_MAX_N = 8;
 // terms of one instance (every nonempty subset of its pairs, 2^n - 1 <= 63)
 // and chunk partials (haf_nparts(n) <= 8 for n <= 6)
 __host__ __device__ constexpr int haf_small_terms(int pairs) { return (1 << pairs) - 1; }
@@ -990,12 +994,20 @@ __device__ __forceinline__ void haf_small_phase(const typename Ar::E* As, int N,
     haf_subset_term<Ar, LOOP, S, MODE>(As, __mul24(inst, N * N), N, q, sh, coeff, ecoeff);
     (void)ecoeff;
     if (active && r == 0) {
-      const T t0 = Ar::add(Ar::zero(), coeff);
-      terms[inst][off + q] = t0;
-      if (MODE == HM_KAPPA) tabs[inst][off + q] = 0.0 + c_abs(Ar::to_c(coeff));
-      // certified: the group's bound after its one subset, as the phase kernel
-      // forms it (cert_round of the group sum, from 0 + the term's bound)
-      if (MODE == HM_CERT) tabs[inst][off + q] = cert_round(Ar::to_c(t0), 0.0 + ecoeff);
+      if constexpr (MODE == HM_DDCERT) {
+        // certified dd: the group sum and its bound after its one subset, as
+        // the phase kernel forms them (a dd add that reports its rounding)
+        double o;
+        terms[inst][off + q] = cddc_add(Ar::zero(), coeff, &o);
+        tabs[inst][off + q] = 0.0 + ecoeff + o;
+      } else {
+        const T t0 = Ar::add(Ar::zero(), coeff);
+        terms[inst][off + q] = t0;
+        if (MODE == HM_KAPPA) tabs[inst][off + q] = 0.0 + c_abs(Ar::to_c(coeff));
+        // certified: the group's bound after its one subset, as the phase
+        // kernel forms it (cert_round of the group sum, from 0 + the term's bound)
+        if (MODE == HM_CERT) tabs[inst][off + q] = cert_round(Ar::to_c(t0), 0.0 + ecoeff);
+      }
     }
   }
   __syncthreads();
@@ -1050,6 +1062,23 @@ __device__ __forceinline__ void haf_small_phase(const typename Ar::E* As, int N,
       }
       pabs[i][pbase + slot] = cert_finish(E, haf_cert_depth(S, n, count));
     }
+    if constexpr (MODE == HM_DDCERT) {
+      // the phase kernel's certified-dd chunk: the G group sums in group order
+      // with reporting dd adds (value and bound), then 4x the fp64 depth
+      T s = ti[q0];
+      double E = tabs[i][off + q0];
+      for (int k = 1; k < G; ++k) {
+        const bool on = q0 + k < count;
+        const int qk = on ? q0 + k : q0;
+        const T x = ti[qk];
+        const double ex = tabs[i][off + qk];
+        double o;
+        s = cddc_add(s, on ? x : Ar::zero(), &o);
+        E = E + (on ? ex : 0.0) + o;
+      }
+      parts[i][pbase + slot] = s;
+      pabs[i][pbase + slot] = cert_finish(E, 4.0 * haf_cert_depth(S, n, count));
+    }
   }
 }
 
@@ -1084,8 +1113,10 @@ __device__ __forceinline__ void haf_small_body(cplx* outp, double* auxp, int nin
   const int lane = threadIdx.x;
   const int n = N / 2;
   constexpr bool CERT = (MODE == HM_CERT);
-  haf_small_phases<Ar, LOOP, MODE, PAIRS>(As, N, ninst, scratch, terms, (KAPPA || CERT) ? tabs : nullptr, parts,
-                                          (KAPPA || CERT) ? pabs : nullptr, std::make_integer_sequence<int, PAIRS>{});
+  constexpr bool DDC = (MODE == HM_DDCERT);
+  constexpr bool AUX = KAPPA || CERT || DDC;
+  haf_small_phases<Ar, LOOP, MODE, PAIRS>(As, N, ninst, scratch, terms, AUX ? tabs : nullptr, parts,
+                                          AUX ? pabs : nullptr, std::make_integer_sequence<int, PAIRS>{});
   __syncthreads();
   // the reduce kernel's butterfly over each instance's partials, for lane 0,
   // whose result is stored: with at most 8 partials (lanes 0..7, exact zeros
@@ -1150,13 +1181,44 @@ __device__ __forceinline__ void haf_small_body(cplx* outp, double* auxp, int nin
       eb[i] = cert_finish(av, 4.0 * (np + 6) + 8.0);
     }
   }
+  // certified dd: the reduce kernel's reporting-add butterfly, literally (its
+  // value is the stored one), the sum's collapse error and the finishing
+  // inflation, a NaN bound made +inf
+  double ebd[DDC ? HAF_SMALL_MAX_T : 1];
+  T sd[DDC ? HAF_SMALL_MAX_T : 1];
+  if constexpr (DDC) {
+#pragma unroll
+    for (int i = 0; i < HAF_SMALL_MAX_T; ++i) {
+      const bool on = i < ninst && lane < np;
+      const T x = parts[i][on ? lane : 0];
+      const double ex = pabs[i][on ? lane : 0];
+      T sv = on ? x : Ar::zero();
+      double av = on ? ex : 0.0;
+#pragma unroll
+      for (int k = 1; k < 64; k <<= 1) {
+        const T o = Ar::shfl_xor(sv, k);
+        const double ao = shfl_xor_d(av, k);
+        double rr;
+        sv = cddc_add(sv, o, &rr);
+        av = av + ao + rr;
+      }
+      const double f = cert_finish(av + cddc_collapse_err(sv), 8.0 * (np + 6) + 16.0);
+      ebd[i] = (f == f) ? f : INFINITY;
+      sd[i] = sv;
+    }
+  }
   if (lane == 0) {
 #pragma unroll
     for (int i = 0; i < HAF_SMALL_MAX_T; ++i) {
       if (i < ninst) {
-        outp[i] = Ar::to_c(s[i]);
-        if (KAPPA) auxp[i] = a[i];
-        if (CERT) auxp[i] = eb[i];
+        if constexpr (DDC) {
+          outp[i] = Ar::to_c(sd[i]);
+          auxp[i] = ebd[i];
+        } else {
+          outp[i] = Ar::to_c(s[i]);
+          if (KAPPA) auxp[i] = a[i];
+          if (CERT) auxp[i] = eb[i];
+        }
       }
     }
   }
@@ -1179,7 +1241,7 @@ __global__ __launch_bounds__(64) void haf_small_kernel(const HafSmallJobs jobs) 
   static_assert(sizeof(HafScratch<ArR64, LOOP, 2, MODE, PAIRS + 1>) <= SCR, "real scratch fits");
   __shared__ cplx As[HAF_SMALL_MAX_T * NN];
   __shared__ T terms[HAF_SMALL_MAX_T][TERMS];
-  constexpr bool AUX = KAPPA || MODE == HM_CERT;  // per-term magnitudes or bounds
+  constexpr bool AUX = KAPPA || MODE == HM_CERT || MODE == HM_DDCERT;  // per-term magnitudes or bounds
   __shared__ double tabs[AUX ? HAF_SMALL_MAX_T : 1][TERMS];
   __shared__ T parts[HAF_SMALL_MAX_T][HAF_SMALL_MAX_PARTS];
   __shared__ double pabs[AUX ? HAF_SMALL_MAX_T : 1][HAF_SMALL_MAX_PARTS];
@@ -1294,14 +1356,16 @@ inline void haf_small_occupancy(const void* kernel, int* cus, int* per_cu) {
 
 template <class Ar, bool LOOP, int MODE>
 static void launch_small(const HafJob* jobs, int njobs, hipStream_t st) {
-  // one launch per size class present (N <= 4, <= 8, <= 12)
+  // one launch per size class present (N <= 4, <= 8, <= 12); certified dd
+  // stops at N = 8, so its "12" class names the 8 kernel and is never launched
+  constexpr int C12 = MODE == HM_DDCERT ? 8 : 12;
   struct Occ {
     int cus, per_cu;
   };
   static const Occ occ[3] = {
       [] { Occ o; haf_small_occupancy((const void*)haf_small_kernel<Ar, LOOP, MODE, 4>, &o.cus, &o.per_cu); return o; }(),
       [] { Occ o; haf_small_occupancy((const void*)haf_small_kernel<Ar, LOOP, MODE, 8>, &o.cus, &o.per_cu); return o; }(),
-      [] { Occ o; haf_small_occupancy((const void*)haf_small_kernel<Ar, LOOP, MODE, 12>, &o.cus, &o.per_cu); return o; }()};
+      [] { Occ o; haf_small_occupancy((const void*)haf_small_kernel<Ar, LOOP, MODE, C12>, &o.cus, &o.per_cu); return o; }()};
   for (int cls : {4, 8, 12}) {
     const Occ& oc = occ[cls / 4 - 1];
     HafSmallJobs sj{};
@@ -1326,7 +1390,7 @@ static void launch_small(const HafJob* jobs, int njobs, hipStream_t st) {
     else if (cls == 8)
       hipLaunchKernelGGL((haf_small_kernel<Ar, LOOP, MODE, 8>), dim3((unsigned)wg), dim3(64), 0, st, sj);
     else
-      hipLaunchKernelGGL((haf_small_kernel<Ar, LOOP, MODE, 12>), dim3((unsigned)wg), dim3(64), 0, st, sj);
+      hipLaunchKernelGGL((haf_small_kernel<Ar, LOOP, MODE, C12>), dim3((unsigned)wg), dim3(64), 0, st, sj);
   }
 }
 
@@ -1365,13 +1429,15 @@ static void launch_phases(const HafJob* jobs, int njobs, typename Ar::T* const* 
 template <class Ar, bool LOOP, int MODE>
 int run_haf_multi(const HafJob* all, int nall, hipStream_t st) {
   using T = typename Ar::T;
-  // small sizes of the plain, kappa and certified fp64 modes: one fused launch
+  // small sizes of every mode: one fused launch
   HafJob jobs[HAF_MAX_JOBS];
   int njobs = 0;
-  if constexpr (MODE == HM_PLAIN || MODE == HM_KAPPA || (MODE == HM_CERT && std::is_same<Ar, ArF64>::value)) {
+  if constexpr (MODE == HM_PLAIN || MODE == HM_KAPPA || MODE == HM_DDCERT ||
+                (MODE == HM_CERT && std::is_same<Ar, ArF64>::value)) {
     HafJob small[HAF_MAX_JOBS];
     int ns = 0;
-    const int small_n = haf_small_max_n < HAF_SMALL_MAX_N ? haf_small_max_n : HAF_SMALL_MAX_N;
+    const int cap = MODE == HM_DDCERT ? HAF_SMALL_DDCERT_MAX_N : HAF_SMALL_MAX_N;
+    const int small_n = haf_small_max_n < cap ? haf_small_max_n : cap;
     for (int j = 0; j < nall; ++j) (all[j].N <= small_n ? small[ns++] : jobs[njobs++]) = all[j];
     if (ns) launch_small<Ar, LOOP, MODE>(small, ns, st);
     if (njobs == 0) return (int)hipGetLastError();

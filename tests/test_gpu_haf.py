@@ -225,11 +225,11 @@ def test_fused_small_kernel_equals_per_size_launches(ext, func):
             A = rng_symmetric(37, N, 700 + N)
             _lib.check(L.gbs_haf_set_schedule(0, t0.value), "set_schedule")
             ref = [getattr(ext, func)(A), *getattr(ext, f"{func}_kappa")(A), getattr(ext, f"{func}_dd")(A),
-                   *getattr(ext, f"{func}_certified")(A)]
+                   *getattr(ext, f"{func}_certified")(A), *getattr(ext, f"{func}_dd_certified")(A)]
             for target in (0, 1, 12, 18, 2048):  # the cost model; 4, 3, 2, 1 instances per workgroup
                 _lib.check(L.gbs_haf_set_schedule(12, target), "set_schedule")
                 got = [getattr(ext, func)(A), *getattr(ext, f"{func}_kappa")(A), getattr(ext, f"{func}_dd")(A),
-                       *getattr(ext, f"{func}_certified")(A)]
+                       *getattr(ext, f"{func}_certified")(A), *getattr(ext, f"{func}_dd_certified")(A)]
                 for k, (a, b) in enumerate(zip(ref, got)):
                     assert np.array_equal(a, b), (N, target, k)
     finally:
