@@ -946,6 +946,9 @@ constexpr int HAF_SMALL_MAX_T = 4;  // instances per workgroup
 // N = 8 ran 535 -> 190 us fused, but the N <= 12 class (every phase's dd
 // bound registers in one kernel) made N = 12 1.3x slower than the phases
 constexpr int HAF_SMALL_DDCERT_MAX_N = 8;
+// the dd loop hafnian likewise stops at N = 10: at B = 4096 its N <= 12 class
+// took 1978 us at N = 12 against the phases' 1484 (N = 10: 747 against 779)
+constexpr int HAF_SMALL_LHAF_DD_MAX_N = 10;
 // terms of one instance (every nonempty subset of its pairs, 2^n - 1 <= 63)
 // and chunk partials (haf_nparts(n) <= 8 for n <= 6)
 __host__ __device__ constexpr int haf_small_terms(int pairs) { return (1 << pairs) - 1; }
@@ -1436,7 +1439,9 @@ int run_haf_multi(const HafJob* all, int nall, hipStream_t st) {
                 (MODE == HM_CERT && std::is_same<Ar, ArF64>::value)) {
     HafJob small[HAF_MAX_JOBS];
     int ns = 0;
-    const int cap = MODE == HM_DDCERT ? HAF_SMALL_DDCERT_MAX_N : HAF_SMALL_MAX_N;
+    const int cap = MODE == HM_DDCERT                                            ? HAF_SMALL_DDCERT_MAX_N
+                    : (LOOP && MODE == HM_PLAIN && std::is_same<Ar, ArDD>::value) ? HAF_SMALL_LHAF_DD_MAX_N
+                                                                                    : HAF_SMALL_MAX_N;
     const int small_n = haf_small_max_n < cap ? haf_small_max_n : cap;
     for (int j = 0; j < nall; ++j) (all[j].N <= small_n ? small[ns++] : jobs[njobs++]) = all[j];
     if (ns) launch_small<Ar, LOOP, MODE>(small, ns, st);
