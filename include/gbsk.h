@@ -284,13 +284,14 @@ int gbs_perm_split(int family, int n, int* lane_log2, int* workgroups);
 
 /* ---- 11. tuning: the hafnian's fused small-size kernel (no reference
  * counterpart; it replaces the reference's small-cap dispatch,
- * core/host_api.cu:162-167). Matrices of side N <= small_max_n (0..12) of the
- * fp64, kappa and dd tiers run in one fused launch (all subset sizes and the
- * final sum), several instances per workgroup: as many as keep at least
+ * core/host_api.cu:162-167). Matrices of side N <= small_max_n (0..12) of every
+ * tier run in one fused launch (all subset sizes and the final sum); the dd
+ * loop hafnian stops at N = 10 and certified dd at N = 8 whatever small_max_n
+ * says. Several instances share a workgroup: as many as keep at least
  * small_target_wgs workgroups, or, with small_target_wgs = 0 (built in), as
  * many as a cost model of the kernel's rounds and occupancy picks per bucket;
  * larger sides run one launch per subset size.
- * Both paths give identical values. The built-in values come from measurements
+ * Both paths give identical values and bounds. The built-in values come from measurements
  * with gbskernels_amd/autotune.py; setting them is for tuning runs and must not
  * overlap evaluations. */
 int gbs_haf_get_schedule(int* small_max_n, int* small_target_wgs);
